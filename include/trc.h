@@ -1,0 +1,141 @@
+/*
+ * trc.h -- C ABI of the MI355X (gfx950) differentiable path-space integrator.
+ *
+ * This is the drop-in boundary that the TensorRay host layer (the
+ * `TensorRay` Python module, TensorRay.cpp:48-222 in the reference, and its
+ * C++ Scene / Integrator classes) binds.  Every entry point takes plain
+ * pointers, sizes and an explicit HIP stream (`trc_stream`, a hipStream_t;
+ * NULL = the default stream), returns a status code and never throws.  The
+ * message of the last failure is available from trc_last_error().
+ *
+ * Memory: scenes (vertex / index buffers, BVH, emitter tables, textures) are
+ * owned by the library and stay resident in HBM until trc_scene_destroy.
+ * Films, adjoint images and gradient buffers are caller-owned device memory.
+ *
+ * Image layout (as the reference Tensorf image, Shape({W*H}, Vec3)): three
+ * planes [3][H*W] of float, pixel index = y * W + x.
+ * Vertex gradients: [V][3] float, V = total vertex count of the scene, in
+ * shape order.  Material gradients: [M][8] float per BSDF, slots
+ * {roughness, albedo/kd r, g, b, specular/F0 r, g, b, unused}.
+ */
+#ifndef TRC_H
+#define TRC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t trc_status;
+#define TRC_OK 0
+#define TRC_ERR_INVALID 1   /* bad argument or scene */
+#define TRC_ERR_HIP 2       /* HIP runtime failure */
+#define TRC_ERR_NOMEM 3     /* device allocation failed or budget exceeded */
+#define TRC_ERR_STATE 4     /* call order (e.g. render before configure) */
+#define TRC_ERR_OVERFLOW 5  /* a traversal stack overflowed (never expected) */
+
+typedef struct trc_ctx trc_ctx;
+typedef struct trc_scene trc_scene;
+typedef void* trc_stream;
+
+/* Material types (BSDF classes of the reference). */
+#define TRC_MAT_DIFFUSE 0      /* Diffuse.cpp:37-85 */
+#define TRC_MAT_MICROFACET 1   /* Microfacet.cpp:37-165 */
+
+/* Mesh flags. */
+#define TRC_MESH_SMOOTH 1      /* smooth shading from averaged face normals (Primitive.cpp:323-340) */
+#define TRC_MESH_DIFF 2        /* vertex positions receive gradients (Primitive::DiffAllVertexPos) */
+
+typedef struct trc_camera {
+  float pos[3], target[3], up[3];
+  int32_t width, height;
+  float fov; /* vertical field of view, degrees */
+} trc_camera;
+
+/* RenderOptions (Integrator.h:47-78; bound at TensorRay.cpp:165-188). */
+typedef struct trc_options {
+  uint32_t seed;         /* counter-RNG seed: same seed => same sample paths */
+  int32_t max_bounces;   /* B: NEE + BSDF steps after the primary hit */
+  int32_t spp;           /* interior samples per pixel */
+  int32_t sppe;          /* primary-edge samples per pixel */
+  int32_t sppse0;        /* secondary (direct) edge samples per pixel */
+  int32_t sppse1;        /* secondary (indirect) edge samples per pixel */
+  int32_t sppe0;         /* pixel-boundary samples per pixel */
+  int32_t quiet;
+} trc_options;
+
+typedef struct trc_scene_info {
+  int32_t num_vertices, num_triangles, num_materials, num_emitter_triangles;
+  int32_t num_edges;
+  int32_t bvh_nodes, bvh_leaves, bvh_max_depth, bvh_stack_need;
+  double bvh_sah_cost;
+  int64_t device_bytes;
+} trc_scene_info;
+
+/* ---- context: replaces EnvCreate / EnvRelease (pybind_utils.h:38-61) ---- */
+trc_status trc_context_create(int32_t device, trc_ctx** out);
+trc_status trc_context_destroy(trc_ctx* ctx);
+const char* trc_last_error(const trc_ctx* ctx);
+/* Number of compute units of the context's device. */
+int32_t trc_device_cus(const trc_ctx* ctx);
+
+/* ---- scene assembly: replaces the SceneLoader shape/BSDF/emitter setup and
+ *      Scene::ExtractGeometry / ConfigureLight (Scene.cpp:46-124, 471-520) ---- */
+trc_status trc_scene_create(trc_ctx* ctx, trc_scene** out);
+trc_status trc_scene_destroy(trc_scene* scene);
+/* Returns the material index (>= 0) or -1 on error. */
+int32_t trc_scene_add_material(trc_scene* scene, int32_t type, const float albedo[3], const float spec[3],
+                               float roughness, int32_t texture);
+/* texels: [h][w][3] float (host).  Returns the texture index or -1. */
+int32_t trc_scene_add_texture(trc_scene* scene, const float* texels, int32_t w, int32_t h);
+/* pos [nv][3], idx [nt][3] (host).  uv / uv_idx optional (NULL).  emit: RGB
+ * radiance of an area emitter or NULL.  Returns the shape index or -1. */
+int32_t trc_scene_add_mesh(trc_scene* scene, const float* pos, int32_t nv, const uint32_t* idx, int32_t nt,
+                           const float* uv, int32_t nuv, const uint32_t* uv_idx, int32_t material,
+                           const float* emit, int32_t flags);
+/* Scene::Configure (Scene.cpp:522-533): normals, areas, emitter CDF, edge
+ * lists, SAH 4-wide BVH, upload.  Synchronous with respect to `stream`. */
+trc_status trc_scene_configure(trc_scene* scene, trc_stream stream);
+/* Scene::Update (Scene.cpp:535-546): replace all vertex positions ([V][3],
+ * host or device pointer) and refit (refit = 1) or rebuild (0) the BVH. */
+trc_status trc_scene_set_vertices(trc_scene* scene, const float* pos, int32_t nv, int32_t refit,
+                                  trc_stream stream);
+/* Set a material's roughness / albedo / specular (BSDF parameters). */
+trc_status trc_scene_set_material(trc_scene* scene, int32_t material, const float albedo[3],
+                                  const float spec[3], float roughness);
+trc_status trc_scene_get_info(const trc_scene* scene, trc_scene_info* out);
+/* Copies of the flattened host arrays (sizes from trc_scene_get_info). */
+trc_status trc_scene_get_vertices(const trc_scene* scene, float* out /* [V][3] */);
+trc_status trc_scene_get_indices(const trc_scene* scene, uint32_t* out /* [T][3] */);
+/* BVH arrays for validation: nodes [n][32] words, triangle order [T]. */
+trc_status trc_scene_get_bvh(const trc_scene* scene, uint32_t* nodes, uint32_t* tri_order);
+/* Edge list: [E][5] int32 (v0, v1, v2, f0, f1), global ids (ObjMesh::processEdges). */
+trc_status trc_scene_get_edges(const trc_scene* scene, int32_t* out);
+
+/* ---- ray queries: replaces RayIntersectOptix (Optix.cpp:252-305) ----
+ * rays [n][8] = (ox, oy, oz, tmin, dx, dy, dz, tmax), device memory.
+ * tri [n] = hit triangle (-1 miss), uvt [n][3] = (u, v, t) (0, 0, -1 on miss).
+ * any_hit = 1 returns some hit in range (occlusion queries). */
+trc_status trc_intersect(trc_scene* scene, const float* rays, int32_t n, int32_t* tri, float* uvt,
+                         int32_t any_hit, trc_stream stream);
+
+/* ---- interior path tracer: PathTracer::RenderC / RenderD ----
+ * (PathTracer.h:60-68, Integrator.h:106-121, PathTracer.cpp:37-123)
+ * film: [3][H*W] device, overwritten.  */
+trc_status trc_render_fwd(trc_scene* scene, const trc_camera* cam, const trc_options* opt, float* film,
+                          trc_stream stream);
+/* Per-path radiance (before the 1/spp factor), [spp*H*W][3], path index =
+ * sample * H*W + pixel.  Used by tests and the serving loop. */
+trc_status trc_render_paths(trc_scene* scene, const trc_camera* cam, const trc_options* opt, float* paths,
+                            trc_stream stream);
+/* Adjoint: accumulates dL/dV += J^T dLdI into dpos ([V][3] device) and the
+ * material gradients into dmat ([M][8] device, may be NULL). */
+trc_status trc_render_adj(trc_scene* scene, const trc_camera* cam, const trc_options* opt, const float* dLdI,
+                          float* dpos, float* dmat, trc_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRC_H */

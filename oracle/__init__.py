@@ -1,0 +1,197 @@
+"""ORACLE -- test infrastructure only (the checker for the HIP path).
+
+ctypes wrapper of oracle/oracle.cpp, a scalar CPU restatement of the
+reference integrator with its own reverse-mode AD.  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg may import this
+package; the product (tensorray_amd) never does.
+
+Parity status of each oracle function (what pins it) is listed in
+tests/test_oracle.py and DESIGN.md.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "oracle.cpp")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(BUILD, "liboracle.so")
+
+_lib = None
+
+
+def build():
+    """Compile oracle.cpp with g++ (-ffp-contract=off: expressions are
+    evaluated exactly as written)."""
+    os.makedirs(BUILD, exist_ok=True)
+    tmp = LIB + ".tmp"
+    cmd = ["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
+           "-o", tmp, SRC, "-lpthread"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
+            build()
+        L = ctypes.CDLL(LIB)
+        vp = ctypes.c_void_p
+        P = ctypes.POINTER
+        f32, u32, i32, f64 = ctypes.c_float, ctypes.c_uint32, ctypes.c_int32, ctypes.c_double
+        L.oracle_philox.argtypes = [P(u32), P(u32), P(u32)]
+        L.oracle_sincos.argtypes = [P(f32), ctypes.c_int, P(f32), P(f32)]
+        L.oracle_exp2.argtypes = [P(f32), ctypes.c_int, P(f32)]
+        L.oracle_scene_create.restype = vp
+        L.oracle_scene_create.argtypes = [P(f32), ctypes.c_int, P(u32), ctypes.c_int, P(i32), P(i32), P(i32), P(f32),
+                                          ctypes.c_int, P(f32), ctypes.c_int, P(f32), ctypes.c_int, P(u32)]
+        L.oracle_scene_add_texture.argtypes = [vp, P(f32), ctypes.c_int, ctypes.c_int]
+        L.oracle_scene_destroy.argtypes = [vp]
+        L.oracle_scene_derived.argtypes = [vp, P(f32), P(f32), P(f32), P(f32), P(f32), P(f32)]
+        L.oracle_intersect.argtypes = [vp, P(f32), ctypes.c_int, ctypes.c_int, ctypes.c_int, P(i32), P(f32),
+                                       ctypes.c_int]
+        L.oracle_camera.argtypes = [P(f32), ctypes.c_int, ctypes.c_int, P(f32)]
+        L.oracle_camera_dirs.argtypes = [P(f32), ctypes.c_int, ctypes.c_int, P(f32), ctypes.c_int, P(f32)]
+        L.oracle_render_paths.argtypes = [vp, P(f32), ctypes.c_int, ctypes.c_int, u32, ctypes.c_int, ctypes.c_uint64,
+                                          ctypes.c_uint64, P(f32), ctypes.c_int]
+        L.oracle_render_film.argtypes = [vp, P(f32), ctypes.c_int, ctypes.c_int, u32, ctypes.c_int, ctypes.c_int,
+                                         P(f32), ctypes.c_int]
+        L.oracle_render_adj.argtypes = [vp, P(f32), ctypes.c_int, ctypes.c_int, u32, ctypes.c_int, ctypes.c_int,
+                                        P(f32), ctypes.c_uint64, ctypes.c_uint64, P(f64), P(f64), ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(ctypes.POINTER(t)) if a is not None else None
+
+
+def default_threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def philox(ctr, key):
+    c = np.asarray(ctr, np.uint32)
+    k = np.asarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().oracle_philox(_p(c, ctypes.c_uint32), _p(k, ctypes.c_uint32), _p(out, ctypes.c_uint32))
+    return out
+
+
+def sincos(x):
+    x = np.ascontiguousarray(x, np.float32)
+    s = np.zeros_like(x)
+    c = np.zeros_like(x)
+    lib().oracle_sincos(_p(x, ctypes.c_float), len(x), _p(s, ctypes.c_float), _p(c, ctypes.c_float))
+    return s, c
+
+
+def exp2(x):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros_like(x)
+    lib().oracle_exp2(_p(x, ctypes.c_float), len(x), _p(y, ctypes.c_float))
+    return y
+
+
+def camera_params(cam, w, h):
+    """cam = dict(pos, target, up, fov).  Returns 19 floats: eye, right, up,
+    dir, ax, bx, ay, by, tan_half, ratio, inv_area."""
+    c = np.array(list(cam["pos"]) + list(cam["target"]) + list(cam["up"]) + [cam["fov"]], np.float32)
+    out = np.zeros(19, np.float32)
+    lib().oracle_camera(_p(c, ctypes.c_float), w, h, _p(out, ctypes.c_float))
+    return out
+
+
+def camera_dirs(cam, w, h, raster):
+    c = np.array(list(cam["pos"]) + list(cam["target"]) + list(cam["up"]) + [cam["fov"]], np.float32)
+    r = np.ascontiguousarray(raster, np.float32)
+    out = np.zeros((len(r), 3), np.float32)
+    lib().oracle_camera_dirs(_p(c, ctypes.c_float), w, h, _p(r, ctypes.c_float), len(r), _p(out, ctypes.c_float))
+    return out
+
+
+class Scene:
+    """The oracle's own copy of a scene (derived data and a simple BVH are
+    computed by the oracle itself)."""
+
+    def __init__(self, spec):
+        f = spec.flat()
+        self.spec = spec
+        self.flat = f
+        mats = spec.material_table()
+        uv = f.get("uv")
+        uvi = f.get("uv_idx")
+        self.V = len(f["pos"])
+        self.T = len(f["idx"])
+        self.M = len(mats)
+        self.h = lib().oracle_scene_create(
+            _p(f["pos"], ctypes.c_float), self.V, _p(f["idx"], ctypes.c_uint32), self.T,
+            _p(f["tri_mat"], ctypes.c_int32), _p(f["tri_emitter"], ctypes.c_int32), _p(f["tri_flags"], ctypes.c_int32),
+            _p(f["emit_rad"], ctypes.c_float), len(f["emit_rad"]), _p(mats, ctypes.c_float), len(mats),
+            _p(uv, ctypes.c_float), len(uv) if uv is not None else 0, _p(uvi, ctypes.c_uint32))
+        for t in spec.textures:
+            t = np.ascontiguousarray(t, np.float32)
+            lib().oracle_scene_add_texture(self.h, _p(t, ctypes.c_float), t.shape[1], t.shape[0])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_scene_destroy(self.h)
+            self.h = None
+
+    def derived(self):
+        nl = int((self.flat["tri_emitter"] >= 0).sum())
+        fn = np.zeros((self.T, 3), np.float32)
+        area = np.zeros(self.T, np.float32)
+        vn = np.zeros((self.V, 3), np.float32)
+        cdf = np.zeros(nl, np.float32)
+        lf = np.zeros(nl, np.float32)
+        li = np.zeros(1, np.float32)
+        fp = lambda a: _p(a, ctypes.c_float)
+        lib().oracle_scene_derived(self.h, fp(fn), fp(area), fp(vn), fp(cdf), fp(lf), fp(li))
+        return dict(face_normal=fn, area=area, vertex_normal=vn, light_cdf=cdf, light_f=lf, light_integral=li[0])
+
+    def intersect(self, rays, brute=False, any_hit=False, threads=None):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = len(rays)
+        tri = np.zeros(n, np.int32)
+        uvt = np.zeros((n, 3), np.float32)
+        lib().oracle_intersect(self.h, _p(rays, ctypes.c_float), n, int(brute), int(any_hit), _p(tri, ctypes.c_int32),
+                               _p(uvt, ctypes.c_float), threads or default_threads())
+        return tri, uvt
+
+    @staticmethod
+    def _cam(cam):
+        return np.array(list(cam["pos"]) + list(cam["target"]) + list(cam["up"]) + [cam["fov"]], np.float32)
+
+    def render_paths(self, cam, w, h, seed, max_bounces, first, count, threads=None):
+        out = np.zeros((count, 3), np.float32)
+        c = self._cam(cam)
+        lib().oracle_render_paths(self.h, _p(c, ctypes.c_float), w, h, seed, max_bounces, first, count,
+                                  _p(out, ctypes.c_float), threads or default_threads())
+        return out
+
+    def render_film(self, cam, w, h, seed, max_bounces, spp, threads=None):
+        film = np.zeros((3, w * h), np.float32)
+        c = self._cam(cam)
+        lib().oracle_render_film(self.h, _p(c, ctypes.c_float), w, h, seed, max_bounces, spp,
+                                 _p(film, ctypes.c_float), threads or default_threads())
+        return film
+
+    def render_adj(self, cam, w, h, seed, max_bounces, spp, dLdI, first=0, count=None, threads=None):
+        """Returns (dpos [V,3], dmat [M,8]) in float64."""
+        dLdI = np.ascontiguousarray(dLdI, np.float32).reshape(3, w * h)
+        if count is None:
+            count = spp * w * h
+        dpos = np.zeros((self.V, 3), np.float64)
+        dmat = np.zeros((self.M, 8), np.float64)
+        c = self._cam(cam)
+        lib().oracle_render_adj(self.h, _p(c, ctypes.c_float), w, h, seed, max_bounces, spp, _p(dLdI, ctypes.c_float),
+                                first, count, _p(dpos, ctypes.c_double), _p(dmat, ctypes.c_double),
+                                threads or default_threads())
+        return dpos, dmat

@@ -1,0 +1,1197 @@
+// ORACLE -- test infrastructure only.  A scalar CPU restatement of the
+// reference integrator (TensorRay, LibTensorRay/Renderer) used as the checker
+// for the HIP path: only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg load it.  The product never links or calls this file.
+//
+// What it restates (reference file:line):
+//   camera rays          Camera.cpp:224-277, BaseCamera.cpp:45-88, Matrix.cpp:181-220
+//   primary weight       Camera.cpp:279-303 (EvalFilter, box filter)
+//   PostIntersect        Scene.cpp:668-730
+//   face normal / area   Primitive.cpp:299-321; vertex normals Primitive.cpp:323-340
+//   light                Scene.cpp:471-520 (ConfigureLight), Light.cpp:134-184
+//   distribution         Distribution.cpp:37-58, Distribution.cu:37-48
+//   BSDFs                BSDF.cpp:100-302, Diffuse.cpp:37-85, Microfacet.cpp:37-165
+//   sampling             Utils.cpp:70-140
+//   path tracer          PathTracer.cpp:37-123, Integrator.cpp:55-303
+//   hit records          ptx.cu:92-113 (miss: tri -1, u = v = 0, t = -1)
+//   reverse-mode AD      Tensor.cpp:601-635 (a scalar tape instead of tensors)
+//   RNG                  Philox4x32-10 (Salmon et al., SC'11) in place of the
+//                        reference's cuRAND stream (Tensor.h:2792-2802)
+//
+// Numerics: every float expression is written in the order of the reference
+// expression it restates and the file is compiled with -ffp-contract=off, so
+// the values equal the HIP kernels' bit for bit.  Elementary functions use
+// their own range reduction + Taylor series (sin/cos, exp2) so that host and
+// device agree; their coefficients are the Taylor coefficients 1/n!.
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace oracle {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10
+// ---------------------------------------------------------------------------
+struct U4 {
+  uint32_t x, y, z, w;
+};
+static inline U4 philox(U4 c, uint32_t k0, uint32_t k1) {
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+    U4 n;
+    n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+    n.y = (uint32_t)p1;
+    n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+    n.w = (uint32_t)p0;
+    c = n;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+static inline float unitf(uint32_t v) { return (float)(v >> 8) * (1.0f / 16777216.0f); }
+struct R4 {
+  float a, b, c, d;
+};
+static inline R4 rnd4(uint32_t seed, uint32_t domain, uint64_t index, uint32_t block) {
+  U4 v = philox(U4{(uint32_t)index, (uint32_t)(index >> 32), block, 0u}, seed, domain);
+  return R4{unitf(v.x), unitf(v.y), unitf(v.z), unitf(v.w)};
+}
+constexpr uint32_t kDomInterior = 0x11u;
+
+// ---------------------------------------------------------------------------
+// Elementary functions (Cody-Waite reduction with a two-part pi/2 and
+// Taylor series; see the header comment)
+// ---------------------------------------------------------------------------
+static inline void f_sincos(float x, float* so, float* co) {
+  float kf = rintf(x * 0.636619772367581343f);
+  float r = fmaf(-kf, 1.57079637050628662109375f, x);
+  r = fmaf(-kf, -4.37113900018624283e-8f, r);
+  float r2 = r * r;
+  float ps = -2.50521083854417188e-8f;
+  ps = fmaf(ps, r2, 2.75573192239858907e-6f);
+  ps = fmaf(ps, r2, -1.98412698412698413e-4f);
+  ps = fmaf(ps, r2, 8.33333333333333333e-3f);
+  ps = fmaf(ps, r2, -1.66666666666666667e-1f);
+  float sr = fmaf(ps * r2, r, r);
+  float pc = 2.08767569878680990e-9f;
+  pc = fmaf(pc, r2, -2.75573192239858907e-7f);
+  pc = fmaf(pc, r2, 2.48015873015873016e-5f);
+  pc = fmaf(pc, r2, -1.38888888888888889e-3f);
+  pc = fmaf(pc, r2, 4.16666666666666667e-2f);
+  pc = fmaf(pc, r2, -0.5f);
+  float cr = fmaf(pc, r2, 1.0f);
+  int k = (int)kf & 3;
+  if (k == 0) { *so = sr; *co = cr; }
+  else if (k == 1) { *so = cr; *co = -sr; }
+  else if (k == 2) { *so = -sr; *co = -cr; }
+  else { *so = -cr; *co = sr; }
+}
+static inline float f_exp2(float x) {
+  if (x < -126.0f) return 0.0f;
+  if (x > 127.0f) return INFINITY;
+  float kf = floorf(x);
+  float y = (x - kf) * 0.693147180559945309f;
+  float p = 2.75573192239858907e-7f;
+  p = fmaf(p, y, 2.75573192239858907e-6f);
+  p = fmaf(p, y, 2.48015873015873016e-5f);
+  p = fmaf(p, y, 1.98412698412698413e-4f);
+  p = fmaf(p, y, 1.38888888888888889e-3f);
+  p = fmaf(p, y, 8.33333333333333333e-3f);
+  p = fmaf(p, y, 4.16666666666666667e-2f);
+  p = fmaf(p, y, 1.66666666666666667e-1f);
+  p = fmaf(p, y, 0.5f);
+  p = fmaf(p, y, 1.0f);
+  p = fmaf(p, y, 1.0f);
+  return ldexpf(p, (int)kf);
+}
+
+// ---------------------------------------------------------------------------
+// Scalar reverse-mode AD (tape per thread, per path)
+// ---------------------------------------------------------------------------
+struct Node {
+  int a, b;
+  double da, db;
+};
+struct Tape {
+  std::vector<Node> nodes;
+  std::vector<double> adj;
+  void clear() { nodes.clear(); }
+};
+static thread_local Tape* g_tape = nullptr;
+
+struct Var {
+  float v;
+  int i;  // tape index, -1 = constant
+  Var() : v(0.0f), i(-1) {}
+  Var(float x) : v(x), i(-1) {}
+  Var(float x, int idx) : v(x), i(idx) {}
+};
+static inline int push_node(int a, double da, int b, double db) {
+  if (a < 0 && b < 0) return -1;
+  g_tape->nodes.push_back(Node{a, b, da, db});
+  return (int)g_tape->nodes.size() - 1;
+}
+static inline Var leaf(float v) {
+  g_tape->nodes.push_back(Node{-1, -1, 0.0, 0.0});
+  return Var(v, (int)g_tape->nodes.size() - 1);
+}
+static inline Var operator+(Var a, Var b) { return Var(a.v + b.v, push_node(a.i, 1.0, b.i, 1.0)); }
+static inline Var operator-(Var a, Var b) { return Var(a.v - b.v, push_node(a.i, 1.0, b.i, -1.0)); }
+static inline Var operator-(Var a) { return Var(-a.v, push_node(a.i, -1.0, -1, 0.0)); }
+static inline Var operator*(Var a, Var b) { return Var(a.v * b.v, push_node(a.i, (double)b.v, b.i, (double)a.v)); }
+static inline Var operator/(Var a, Var b) {
+  double ib = 1.0 / (double)b.v;
+  return Var(a.v / b.v, push_node(a.i, ib, b.i, -(double)a.v * ib * ib));
+}
+static inline Var sqrt_(Var a) {
+  float r = sqrtf(a.v);
+  return Var(r, push_node(a.i, 0.5 / (double)r, -1, 0.0));
+}
+static inline Var abs_(Var a) {
+  return Var(fabsf(a.v), push_node(a.i, a.v > 0.0f ? 1.0 : (a.v < 0.0f ? -1.0 : 0.0), -1, 0.0));
+}
+static inline Var exp2_(Var a) {
+  float r = f_exp2(a.v);
+  return Var(r, push_node(a.i, (double)r * 0.693147180559945309, -1, 0.0));
+}
+static inline Var detach(Var a) { return Var(a.v); }
+static inline float val(Var a) { return a.v; }
+static inline float sqrt_(float a) { return sqrtf(a); }
+static inline float abs_(float a) { return fabsf(a); }
+static inline float exp2_(float a) { return f_exp2(a); }
+static inline float detach(float a) { return a; }
+static inline float val(float a) { return a; }
+
+template <class S>
+struct V3 {
+  S x, y, z;
+};
+template <class S> static inline V3<S> mk(S x, S y, S z) { return V3<S>{x, y, z}; }
+template <class S> static inline V3<S> operator+(V3<S> a, V3<S> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class S> static inline V3<S> operator-(V3<S> a, V3<S> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class S> static inline V3<S> operator-(V3<S> a) { return {-a.x, -a.y, -a.z}; }
+template <class S> static inline V3<S> operator*(V3<S> a, V3<S> b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+template <class S> static inline V3<S> operator*(V3<S> a, S s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class S> static inline V3<S> operator*(S s, V3<S> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class S> static inline V3<S> operator/(V3<S> a, S s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class S> static inline S dot(V3<S> a, V3<S> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class S> static inline V3<S> cross(V3<S> a, V3<S> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class S> static inline S len2(V3<S> a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+template <class S> static inline S length(V3<S> a) { return sqrt_(len2(a)); }
+template <class S> static inline V3<S> normalize(V3<S> a) { return a / sqrt_(len2(a)); }
+template <class S> static inline V3<S> detach3(V3<S> a) { return {detach(a.x), detach(a.y), detach(a.z)}; }
+template <class S> static inline V3<float> val3(V3<S> a) { return {val(a.x), val(a.y), val(a.z)}; }
+template <class S> static inline V3<S> cast3(V3<float> a) { return {S(a.x), S(a.y), S(a.z)}; }
+template <class S> static inline bool finite3(V3<S> a) {
+  return std::isfinite(val(a.x)) && std::isfinite(val(a.y)) && std::isfinite(val(a.z));
+}
+template <class S> static inline S sel(bool c, S a, S b) { return c ? a : b; }
+using F3 = V3<float>;
+
+// ---------------------------------------------------------------------------
+// Scene
+// ---------------------------------------------------------------------------
+constexpr float EPSILON_ = 1e-6f;
+constexpr float EDGE_EPSILON_ = 1e-8f;
+constexpr float SHADOW_EPSILON_ = 1e-2f;
+constexpr float INV_PI = 0.31830988618379067154f;
+constexpr float PI = 3.14159265358979323846f;
+
+struct Mat {
+  int type, tex;
+  float roughness;
+  F3 albedo, spec;
+};
+
+struct BNode {
+  float lo[3], hi[3];
+  int left, right;  // inner children or -1
+  int first, count; // leaf range
+};
+
+struct Scene {
+  int V = 0, T = 0;
+  std::vector<float> pos;        // [V][3]
+  std::vector<uint32_t> idx;     // [T][3]
+  std::vector<int> tri_mat, tri_emitter, tri_flags;
+  std::vector<float> emit_rad;   // [n][3]
+  std::vector<Mat> mats;
+  std::vector<float> uv;
+  std::vector<uint32_t> uv_idx;
+  std::vector<std::vector<float>> tex;
+  std::vector<int> tex_w, tex_h;
+  // derived (float)
+  std::vector<F3> fn;            // face normals
+  std::vector<float> area;
+  std::vector<F3> vn;            // vertex normals
+  std::vector<std::vector<int>> vfaces;  // vertex -> incident smooth faces (triangle order)
+  std::vector<int> light_tri, emit_tri;  // emitter triangle table and inverse
+  std::vector<float> light_f, light_cdf;
+  float light_int = 0.0f;
+  // simple binary BVH for the oracle's own ray queries
+  std::vector<BNode> bvh;
+  std::vector<int> bvh_tris;
+};
+
+static inline F3 P(const Scene& s, uint32_t v) { return F3{s.pos[3 * v], s.pos[3 * v + 1], s.pos[3 * v + 2]}; }
+
+static void scene_derive(Scene& s) {
+  // Primitive.cpp:299-321
+  s.fn.resize(s.T);
+  s.area.resize(s.T);
+  for (int t = 0; t < s.T; ++t) {
+    F3 p0 = P(s, s.idx[3 * t]), p1 = P(s, s.idx[3 * t + 1]), p2 = P(s, s.idx[3 * t + 2]);
+    F3 c = cross(p1 - p0, p2 - p0);
+    float l = length(c);
+    s.fn[t] = c / l;
+    s.area[t] = l * 0.5f;
+  }
+  // Primitive.cpp:323-340: normalize(sum of face normals / count)
+  s.vn.assign(s.V, F3{0, 0, 0});
+  s.vfaces.assign(s.V, {});
+  for (int t = 0; t < s.T; ++t) {
+    if (!(s.tri_flags[t] & 1)) continue;
+    for (int j = 0; j < 3; ++j) s.vfaces[s.idx[3 * t + j]].push_back(t);
+  }
+  for (int v = 0; v < s.V; ++v) {
+    if (s.vfaces[v].empty()) continue;
+    F3 a{0, 0, 0};
+    for (int t : s.vfaces[v]) a = a + s.fn[t];
+    float c = (float)s.vfaces[v].size();
+    s.vn[v] = normalize(F3{a.x / c, a.y / c, a.z / c});
+  }
+  // Scene.cpp:471-520 + Distribution.cpp:45-58
+  s.light_tri.clear();
+  s.emit_tri.assign(s.T, -1);
+  for (int t = 0; t < s.T; ++t)
+    if (s.tri_emitter[t] >= 0) {
+      s.emit_tri[t] = (int)s.light_tri.size();
+      s.light_tri.push_back(t);
+    }
+  int n = (int)s.light_tri.size();
+  s.light_f.assign(n, 0.0f);
+  s.light_cdf.assign(n, 0.0f);
+  s.light_int = 0.0f;
+  if (n > 0) {
+    for (int i = 0; i < n; ++i) {
+      int t = s.light_tri[i];
+      const float* I = &s.emit_rad[3 * s.tri_emitter[t]];
+      float lum = 0.212671f * I[0] + 0.715160f * I[1] + 0.072169f * I[2];
+      s.light_f[i] = lum * s.area[t];
+    }
+    float inv = 1.0f / (float)n, run = 0.0f;
+    for (int i = 0; i < n; ++i) {
+      run += s.light_f[i];
+      s.light_cdf[i] = run * inv;
+    }
+    s.light_int = s.light_cdf[n - 1];
+    if (s.light_int > 0.0f)
+      for (int i = 0; i < n; ++i) s.light_cdf[i] = s.light_cdf[i] / s.light_int;
+  }
+}
+
+// Median-split binary BVH (independent of the product's SAH 4-wide BVH).
+static int bvh_rec(Scene& s, std::vector<int>& tris, int first, int count, std::vector<float>& cent) {
+  BNode n;
+  for (int k = 0; k < 3; ++k) { n.lo[k] = FLT_MAX; n.hi[k] = -FLT_MAX; }
+  for (int i = first; i < first + count; ++i)
+    for (int j = 0; j < 3; ++j) {
+      F3 p = P(s, s.idx[3 * tris[i] + j]);
+      float c[3] = {p.x, p.y, p.z};
+      for (int k = 0; k < 3; ++k) { n.lo[k] = std::min(n.lo[k], c[k]); n.hi[k] = std::max(n.hi[k], c[k]); }
+    }
+  // pad outward so the slab test is conservative
+  for (int k = 0; k < 3; ++k) {
+    n.lo[k] -= std::fabs(n.lo[k]) * 1e-6f + 1e-6f;
+    n.hi[k] += std::fabs(n.hi[k]) * 1e-6f + 1e-6f;
+  }
+  int self = (int)s.bvh.size();
+  s.bvh.push_back(n);
+  if (count <= 4) {
+    s.bvh[self].left = s.bvh[self].right = -1;
+    s.bvh[self].first = first;
+    s.bvh[self].count = count;
+    return self;
+  }
+  int axis = 0;
+  float ext = 0.0f;
+  for (int k = 0; k < 3; ++k)
+    if (n.hi[k] - n.lo[k] > ext) { ext = n.hi[k] - n.lo[k]; axis = k; }
+  int mid = first + count / 2;
+  std::nth_element(tris.begin() + first, tris.begin() + mid, tris.begin() + first + count,
+                   [&](int a, int b) { return cent[3 * a + axis] < cent[3 * b + axis]; });
+  int l = bvh_rec(s, tris, first, mid - first, cent);
+  int r = bvh_rec(s, tris, mid, first + count - mid, cent);
+  s.bvh[self].left = l;
+  s.bvh[self].right = r;
+  s.bvh[self].first = 0;
+  s.bvh[self].count = 0;
+  return self;
+}
+
+static void scene_build_bvh(Scene& s) {
+  s.bvh.clear();
+  s.bvh_tris.resize(s.T);
+  std::vector<float> cent(3 * (size_t)s.T);
+  for (int t = 0; t < s.T; ++t) {
+    s.bvh_tris[t] = t;
+    F3 c = (P(s, s.idx[3 * t]) + P(s, s.idx[3 * t + 1])) + P(s, s.idx[3 * t + 2]);
+    cent[3 * t] = c.x; cent[3 * t + 1] = c.y; cent[3 * t + 2] = c.z;
+  }
+  bvh_rec(s, s.bvh_tris, 0, s.T, cent);
+}
+
+// ---------------------------------------------------------------------------
+// Watertight ray / triangle test (Woop, Benthin, Wald, JCGT 2013)
+// ---------------------------------------------------------------------------
+struct Ray {
+  F3 o, d;
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+};
+static inline float cmp(F3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+static inline Ray make_ray(F3 o, F3 d) {
+  Ray r;
+  r.o = o; r.d = d;
+  float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  r.kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+  r.kx = r.kz == 2 ? 0 : r.kz + 1;
+  r.ky = r.kx == 2 ? 0 : r.kx + 1;
+  float dz = cmp(d, r.kz);
+  if (dz < 0.0f) std::swap(r.kx, r.ky);
+  r.Sx = cmp(d, r.kx) / dz;
+  r.Sy = cmp(d, r.ky) / dz;
+  r.Sz = 1.0f / dz;
+  return r;
+}
+static inline bool tri_test(const Ray& r, F3 p0, F3 p1, F3 p2, float* t, float* u, float* v) {
+  F3 A = p0 - r.o, B = p1 - r.o, C = p2 - r.o;
+  float Akz = cmp(A, r.kz), Bkz = cmp(B, r.kz), Ckz = cmp(C, r.kz);
+  float Ax = cmp(A, r.kx) - r.Sx * Akz, Ay = cmp(A, r.ky) - r.Sy * Akz;
+  float Bx = cmp(B, r.kx) - r.Sx * Bkz, By = cmp(B, r.ky) - r.Sy * Bkz;
+  float Cx = cmp(C, r.kx) - r.Sx * Ckz, Cy = cmp(C, r.ky) - r.Sy * Ckz;
+  float U = Cx * By - Cy * Bx, Vv = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+  if (U == 0.0f || Vv == 0.0f || W == 0.0f) {
+    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+    Vv = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+  }
+  if ((U < 0.0f || Vv < 0.0f || W < 0.0f) && (U > 0.0f || Vv > 0.0f || W > 0.0f)) return false;
+  float det = U + Vv + W;
+  if (det == 0.0f) return false;
+  float T = U * (r.Sz * Akz) + Vv * (r.Sz * Bkz) + W * (r.Sz * Ckz);
+  float inv = 1.0f / det;
+  *t = T * inv;
+  *u = Vv * inv;
+  *v = W * inv;
+  return true;
+}
+
+struct Hit {
+  int tri;
+  float u, v, t;
+};
+
+// Closest hit: smallest t in (tmin, tmax), ties to the smallest triangle id.
+static Hit intersect(const Scene& s, F3 o, F3 d, float tmin, float tmax, bool any, bool brute) {
+  Ray r = make_ray(o, d);
+  Hit h{-1, 0.0f, 0.0f, -1.0f};
+  float best = tmax;
+  auto test = [&](int t) {
+    float tt, uu, vv;
+    if (!tri_test(r, P(s, s.idx[3 * t]), P(s, s.idx[3 * t + 1]), P(s, s.idx[3 * t + 2]), &tt, &uu, &vv)) return false;
+    if (!(tt > tmin && tt < tmax)) return false;
+    if (h.tri < 0 || tt < best || (tt == best && t < h.tri)) {
+      best = tt; h.tri = t; h.u = uu; h.v = vv; h.t = tt;
+    }
+    return true;
+  };
+  if (brute) {
+    for (int t = 0; t < s.T; ++t)
+      if (test(t) && any) return h;
+    return h;
+  }
+  // robust slab test in double with generous padding (boxes are padded)
+  double inv[3] = {1.0 / (double)d.x, 1.0 / (double)d.y, 1.0 / (double)d.z};
+  double oo[3] = {o.x, o.y, o.z};
+  int stack[128];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp) {
+    const BNode& n = s.bvh[stack[--sp]];
+    double t0 = tmin, t1 = (double)best * (1.0 + 1e-6);
+    bool miss = false;
+    for (int k = 0; k < 3 && !miss; ++k) {
+      if (!std::isfinite(inv[k])) {
+        if (oo[k] < n.lo[k] || oo[k] > n.hi[k]) miss = true;
+        continue;
+      }
+      double a = (n.lo[k] - oo[k]) * inv[k], b = (n.hi[k] - oo[k]) * inv[k];
+      if (a > b) std::swap(a, b);
+      t0 = std::max(t0, a);
+      t1 = std::min(t1, b);
+      if (t0 > t1) miss = true;
+    }
+    if (miss) continue;
+    if (n.left < 0) {
+      for (int i = n.first; i < n.first + n.count; ++i)
+        if (test(s.bvh_tris[i]) && any) return h;
+    } else {
+      stack[sp++] = n.left;
+      stack[sp++] = n.right;
+    }
+  }
+  return h;
+}
+
+// ---------------------------------------------------------------------------
+// Camera (reduced form of the reference matrices; see make_camera)
+// ---------------------------------------------------------------------------
+struct Cam {
+  F3 eye, right, up, dir;
+  float ax, bx, ay, by, tan_half, ratio, inv_area;
+  int W, H;
+};
+
+static Cam make_camera(const float pos[3], const float target[3], const float upv[3], int w, int h, float fov) {
+  // LookAt (Matrix.cpp:181-209) columns and Perspective (Matrix.cpp:211-220)
+  // with the screen-to-raster map of BaseCamera::Resize (BaseCamera.cpp:73-88):
+  // raster (rx, ry) lies on the z = 1 plane at
+  // (ratio tan(fov/2) (1 - 2 rx / W), tan(fov/2) (1 - 2 ry / H), 1).
+  double e[3] = {pos[0], pos[1], pos[2]};
+  double d[3] = {(double)target[0] - e[0], (double)target[1] - e[1], (double)target[2] - e[2]};
+  double dl = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  for (double& x : d) x /= dl;
+  double u[3] = {upv[0], upv[1], upv[2]};
+  double ul = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  for (double& x : u) x /= ul;
+  double r[3] = {u[1] * d[2] - u[2] * d[1], u[2] * d[0] - u[0] * d[2], u[0] * d[1] - u[1] * d[0]};
+  double rl = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+  for (double& x : r) x /= rl;
+  double nu[3] = {d[1] * r[2] - d[2] * r[1], d[2] * r[0] - d[0] * r[2], d[0] * r[1] - d[1] * r[0]};
+  double th = std::tan(fov * 3.14159265358979323846 / 180.0 / 2.0);
+  double ratio = (double)w / (double)h;
+  Cam c;
+  c.eye = F3{(float)e[0], (float)e[1], (float)e[2]};
+  c.right = F3{(float)r[0], (float)r[1], (float)r[2]};
+  c.up = F3{(float)nu[0], (float)nu[1], (float)nu[2]};
+  c.dir = F3{(float)d[0], (float)d[1], (float)d[2]};
+  c.ax = (float)(-2.0 * ratio * th / w);
+  c.bx = (float)(ratio * th);
+  c.ay = (float)(-2.0 * th / h);
+  c.by = (float)th;
+  c.tan_half = (float)th;
+  c.ratio = (float)ratio;
+  c.inv_area = (float)(1.0 / ((2.0 * ratio * th) * (2.0 * th)));
+  c.W = w;
+  c.H = h;
+  return c;
+}
+
+static inline F3 camera_ray_dir(const Cam& c, float rx, float ry) {
+  F3 v = normalize(F3{c.ax * rx + c.bx, c.ay * ry + c.by, 1.0f});
+  return (v.x * c.right + v.y * c.up) + v.z * c.dir;
+}
+
+// ---------------------------------------------------------------------------
+// Differentiable scene access.  In AD mode every vertex position read inside
+// one path becomes a tape leaf (cached per path); material parameters too.
+// ---------------------------------------------------------------------------
+template <class S>
+struct Access;
+
+template <>
+struct Access<float> {
+  const Scene& s;
+  explicit Access(const Scene& sc) : s(sc) {}
+  F3 pos(uint32_t v) { return P(s, v); }
+  float rough(int m) { return s.mats[m].roughness; }
+  F3 albedo(int m) { return s.mats[m].albedo; }
+  F3 spec(int m) { return s.mats[m].spec; }
+};
+
+template <>
+struct Access<Var> {
+  const Scene& s;
+  std::unordered_map<uint32_t, V3<Var>> vcache;
+  std::unordered_map<int, Var> rcache;
+  std::unordered_map<int, V3<Var>> acache, scache;
+  explicit Access(const Scene& sc) : s(sc) {}
+  V3<Var> pos(uint32_t v) {
+    auto it = vcache.find(v);
+    if (it != vcache.end()) return it->second;
+    F3 p = P(s, v);
+    V3<Var> r{leaf(p.x), leaf(p.y), leaf(p.z)};
+    vcache.emplace(v, r);
+    return r;
+  }
+  Var rough(int m) {
+    auto it = rcache.find(m);
+    if (it != rcache.end()) return it->second;
+    Var r = leaf(s.mats[m].roughness);
+    rcache.emplace(m, r);
+    return r;
+  }
+  V3<Var> albedo(int m) {
+    auto it = acache.find(m);
+    if (it != acache.end()) return it->second;
+    F3 a = s.mats[m].albedo;
+    V3<Var> r{leaf(a.x), leaf(a.y), leaf(a.z)};
+    acache.emplace(m, r);
+    return r;
+  }
+  V3<Var> spec(int m) {
+    auto it = scache.find(m);
+    if (it != scache.end()) return it->second;
+    F3 a = s.mats[m].spec;
+    V3<Var> r{leaf(a.x), leaf(a.y), leaf(a.z)};
+    scache.emplace(m, r);
+    return r;
+  }
+};
+
+template <class S>
+struct Its {
+  V3<S> p, ng, ns, t, b;
+  float tu, tv;
+  int tri, mat, emitter;
+};
+
+// face normal and area from differentiable positions (Primitive.cpp:299-321)
+template <class S>
+static void face_nA(Access<S>& A, int t, V3<S>* n, S* area) {
+  const Scene& s = A.s;
+  V3<S> p0 = A.pos(s.idx[3 * t]), p1 = A.pos(s.idx[3 * t + 1]), p2 = A.pos(s.idx[3 * t + 2]);
+  V3<S> c = cross(p1 - p0, p2 - p0);
+  S l = length(c);
+  *n = c / l;
+  *area = l * S(0.5f);
+}
+
+// vertex normal (Primitive.cpp:323-340), differentiable
+template <class S>
+static V3<S> vertex_normal(Access<S>& A, uint32_t v) {
+  const Scene& s = A.s;
+  V3<S> a{S(0.0f), S(0.0f), S(0.0f)};
+  for (int t : s.vfaces[v]) {
+    V3<S> n;
+    S ar;
+    face_nA(A, t, &n, &ar);
+    a = a + n;
+  }
+  S c((float)s.vfaces[v].size());
+  return normalize(V3<S>{a.x / c, a.y / c, a.z / c});
+}
+
+// CoordinateSystem (Utils.cpp:118-140)
+template <class S>
+static void coordinate_system(V3<S> n, V3<S>* t, V3<S>* b) {
+  bool cond = fabsf(val(n.x)) > fabsf(val(n.y));
+  S q = cond ? n.x * n.x + n.z * n.z : n.y * n.y + n.z * n.z;
+  S qm = val(q) > 1e-3f ? q : S(1e-3f);
+  S il = S(1.0f) / sqrt_(qm);
+  *t = cond ? V3<S>{-n.z * il, S(0.0f), n.x * il} : V3<S>{S(0.0f), n.z * il, -n.y * il};
+  *b = cross(n, *t);
+}
+
+// Scene::PostIntersect (Scene.cpp:668-730)
+template <class S>
+static Its<S> post_intersect(Access<S>& A, int tri, float u, float v) {
+  const Scene& s = A.s;
+  Its<S> it;
+  uint32_t i0 = s.idx[3 * tri], i1 = s.idx[3 * tri + 1], i2 = s.idx[3 * tri + 2];
+  float w = 1.0f - u - v;
+  it.p = (S(w) * A.pos(i0) + S(u) * A.pos(i1)) + S(v) * A.pos(i2);
+  S ar;
+  face_nA(A, tri, &it.ng, &ar);
+  if (s.tri_flags[tri] & 1) {
+    it.ns = (S(w) * vertex_normal(A, i0) + S(u) * vertex_normal(A, i1)) + S(v) * vertex_normal(A, i2);
+  } else {
+    it.ns = it.ng;
+  }
+  if (!s.uv.empty()) {
+    uint32_t t0 = s.uv_idx[3 * tri], t1 = s.uv_idx[3 * tri + 1], t2 = s.uv_idx[3 * tri + 2];
+    it.tu = w * s.uv[2 * t0] + u * s.uv[2 * t1] + v * s.uv[2 * t2];
+    it.tv = w * s.uv[2 * t0 + 1] + u * s.uv[2 * t1 + 1] + v * s.uv[2 * t2 + 1];
+  } else {
+    it.tu = it.tv = 0.0f;
+  }
+  coordinate_system(it.ns, &it.t, &it.b);
+  it.tri = tri;
+  it.mat = s.tri_mat[tri];
+  it.emitter = s.tri_emitter[tri];
+  return it;
+}
+
+template <class S>
+static S area_of(Access<S>& A, int tri) {
+  V3<S> n;
+  S ar;
+  face_nA(A, tri, &n, &ar);
+  return ar;
+}
+
+template <class S> static V3<S> to_local(const Its<S>& it, V3<S> v) { return {dot(v, it.t), dot(v, it.b), dot(v, it.ns)}; }
+template <class S> static V3<S> to_world(const Its<S>& it, V3<S> l) { return (l.x * it.t + l.y * it.b) + l.z * it.ns; }
+
+// ImageTexture::Eval (BSDF.cpp:72-98)
+static F3 tex_lookup(const Scene& s, int ti, float u, float v) {
+  int w = s.tex_w[ti], h = s.tex_h[ti];
+  float fu = u - floorf(u), fv = v - floorf(v);
+  int x = (int)(fu * (float)(w - 1)), y = (int)(fv * (float)(h - 1));
+  const float* p = &s.tex[ti][3 * ((size_t)y * w + x)];
+  return F3{p[0], p[1], p[2]};
+}
+
+template <class S>
+static V3<S> albedo_at(Access<S>& A, const Its<S>& it) {
+  const Mat& m = A.s.mats[it.mat];
+  if (m.tex >= 0) return cast3<S>(tex_lookup(A.s, m.tex, it.tu, it.tv));
+  return A.albedo(it.mat);
+}
+
+// ---- sampling (Utils.cpp:70-115) ----
+static F3 cosine_hemisphere(float s0, float s1) {
+  float r1 = 2.0f * s0 - 1.0f, r2 = 2.0f * s1 - 1.0f;
+  bool cond = (r1 * r1) > (r2 * r2);
+  float r = cond ? r1 : r2;
+  bool zero = (r1 == 0.0f) && (r2 == 0.0f);
+  float phi = 0.0f;
+  if (!zero) phi = cond ? 0.78539816339744830962f * (r2 / r1) : 1.57079632679489661923f - (r1 / r2) * 0.78539816339744830962f;
+  float rr = zero ? 0.0f : r;
+  float sp, cp;
+  f_sincos(phi, &sp, &cp);
+  float x = rr * cp, y = rr * sp;
+  float z = sqrtf(std::max(1e-4f, 1.0f - x * x - y * y));
+  return F3{x, y, z};
+}
+static void disk_concentric(float s0, float s1, float* ox, float* oy) {
+  float x = 2.0f * s0 - 1.0f, y = 2.0f * s1 - 1.0f;
+  bool zero = (x == 0.0f && y == 0.0f);
+  bool q13 = fabsf(x) < fabsf(y);
+  float r = q13 ? y : x, rp = q13 ? x : y;
+  float phi = 0.0f;
+  if (!zero) {
+    float p_ = (0.25f * PI) * rp / r;
+    phi = q13 ? 0.5f * PI - p_ : p_;
+  }
+  float sp, cp;
+  f_sincos(phi, &sp, &cp);
+  *ox = r * cp;
+  *oy = r * sp;
+}
+
+// ---- GGX (BSDF.cpp:232-302), all float: only used for sampling and pdfs ----
+static float tan2f_(F3 v) { float c2 = v.z * v.z; return std::max(1.0f - c2, 0.0f) / c2; }
+static float sinth(F3 v) { return sqrtf(std::max(1.0f - v.z * v.z, 0.0f)); }
+static float cosphi(F3 v) { float st = sinth(v); return st == 0.0f ? 1.0f : std::min(std::max(v.x / st, -1.0f), 1.0f); }
+static float sinphi(F3 v) { float st = sinth(v); return st == 0.0f ? 0.0f : std::min(std::max(v.y / st, -1.0f), 1.0f); }
+static float GGX_D(F3 wh, float a) {
+  float t2 = tan2f_(wh);
+  bool ok = t2 < 3.402823466e38f;
+  float c2 = wh.z * wh.z;
+  float e = 1.0f / (a * a) * t2;
+  float root = a * c2 * (1.0f + e);
+  float D = INV_PI / (root * root);
+  return ok ? D : 0.0f;
+}
+static float Smith_G(F3 v, F3 wh, float a) {
+  float t2 = tan2f_(wh);
+  bool ok = t2 < 3.402823466e38f;
+  float root = (a * a) * t2;
+  float G = 2.0f / (1.0f + sqrtf(1.0f + root * root));
+  G = ok ? G : 0.0f;
+  float voh = dot(v, wh);
+  G = t2 == 0.0f ? 1.0f : G;
+  G = (voh * v.z) <= 0.0f ? 0.0f : G;
+  return G;
+}
+static float VNDF_pdf(F3 v, F3 wh, float a) {
+  return GGX_D(wh, a) * Smith_G(v, wh, a) * fabsf(dot(v, wh)) / fabsf(v.z);
+}
+static F3 VNDF_sample(F3 v, float a, float s0, float s1) {
+  F3 Vv = normalize(F3{v.x * a, v.y * a, v.z * 1.0f});
+  float sp = sinphi(Vv), cp = cosphi(Vv), ct = Vv.z;
+  float px, py;
+  disk_concentric(s0, s1, &px, &py);
+  float sh = 0.5f * (1.0f + ct);
+  float x = px;
+  float aa = sqrtf(std::max(1.0f - x * x, 0.0f));
+  float y = (1.0f - sh) * aa + sh * py;
+  float z = sqrtf(std::max(1.0f - (px * px + py * py), 0.0f));
+  float st = sqrtf(std::max(1.0f - ct * ct, 0.0f));
+  float norm = 1.0f / (st * y + ct * z);
+  float slx = (ct * y - st * z) * norm, sly = x * norm;
+  float sx = (cp * slx - sp * sly) * a, sy = (sp * slx + cp * sly) * a;
+  return normalize(F3{-sx, -sy, 1.0f});
+}
+static F3 reflect_(F3 i, F3 n) { return i + (2.0f * dot(-i, n)) * n; }
+
+// BSDF value (BSDF::Eval without correction), templated for AD.
+template <class S>
+static V3<S> bsdf_eval(Access<S>& A, const Its<S>& it, V3<S> wo_w, V3<S> wi_w) {
+  const Mat& m = A.s.mats[it.mat];
+  V3<S> wo = to_local(it, wo_w), wi = to_local(it, wi_w);
+  if (m.type == 0) {
+    bool refl = val(wo.z) > 0.0f && (val(wo.z) * val(wi.z)) > 0.0f;
+    S inner = S(INV_PI) * S(refl ? 1.0f : 0.0f);
+    return albedo_at(A, it) * inner;
+  }
+  // Microfacet::EvalInner (Microfacet.cpp:37-73)
+  S nv = wo.z, nl = wi.z;
+  bool active = val(nv) > 0.0f && val(nl) > 0.0f;
+  V3<S> kd = albedo_at(A, it);
+  V3<S> diff = kd * S(INV_PI);
+  V3<S> wh = normalize(wo + wi);
+  S nh = wh.z, vh = dot(wo, wh);
+  S r = A.rough(it.mat);
+  S alpha = r * r;
+  S k = (r + S(1.0f)) * (r + S(1.0f)) * S(0.125f);
+  S tmp = alpha / (nh * nh * (alpha * alpha - S(1.0f)) + S(1.0f));
+  S ggx = tmp * tmp * S(INV_PI);
+  S coeff = vh * (vh * S(-5.55473f) - S(6.8316f));
+  S e = exp2_(coeff);
+  V3<S> F0 = A.spec(it.mat);
+  V3<S> one{S(1.0f), S(1.0f), S(1.0f)};
+  V3<S> fres = F0 + (one - F0) * e;
+  S g1 = nv / (nv * (S(1.0f) - k) + k);
+  S g2 = nl / (nl * (S(1.0f) - k) + k);
+  S G = g1 * g2;
+  V3<S> num = (ggx * G) * fres;
+  S den = S(4.0f) * abs_(nv) * abs_(nl);
+  V3<S> spec = num / (den + S(1e-6f));
+  V3<S> v = diff + spec;
+  if (!active) return V3<S>{S(0.0f), S(0.0f), S(0.0f)};
+  return v;
+}
+
+static float bsdf_pdf(const Scene& s, const Its<float>& it, F3 wo_w, F3 wi_w) {
+  const Mat& m = s.mats[it.mat];
+  F3 wo = to_local(it, wo_w), wi = to_local(it, wi_w);
+  if (m.type == 0) {
+    bool refl = wo.z > 0.0f && (wo.z * wi.z) > 0.0f;
+    return fabsf(wi.z) / PI * (refl ? 1.0f : 0.0f);
+  }
+  F3 wh = normalize(wo + wi);
+  float mp = VNDF_pdf(wo, wh, m.roughness);
+  float pdf = fabsf(mp * (1.0f / (4.0f * dot(wi, wh))));
+  bool refl = wo.z > 0.0f && mp != 0.0f && wi.z > 0.0f;
+  return refl ? pdf : 0.0f;
+}
+
+static bool dir_valid(const Scene& s, const Its<float>& it, F3 w) {
+  if (s.mats[it.mat].type == 0) return dot(it.ns, w) > EDGE_EPSILON_ && dot(it.ng, w) > EDGE_EPSILON_;
+  return dot(it.ns, w) * dot(it.ng, w) > 0.0f;
+}
+
+static F3 bsdf_sample(const Scene& s, const Its<float>& it, F3 wo_w, float s0, float s1, float* pdf) {
+  const Mat& m = s.mats[it.mat];
+  F3 wo = to_local(it, wo_w);
+  if (m.type == 0) {
+    F3 wi = cosine_hemisphere(s0, s1);
+    bool refl = wo.z > 0.0f && (wo.z * wi.z) > 0.0f;
+    *pdf = fabsf(wi.z) / PI * (refl ? 1.0f : 0.0f);
+    return to_world(it, wi);
+  }
+  bool refl = wo.z > 0.0f;
+  F3 wh = VNDF_sample(wo, m.roughness, s0, s1);
+  float mp = VNDF_pdf(wo, wh, m.roughness);
+  F3 wi = reflect_(-wo, wh);
+  refl = refl && mp != 0.0f && wi.z > 0.0f;
+  float p = fabsf(mp * (1.0f / (4.0f * dot(wi, wh))));
+  *pdf = refl ? p : 0.0f;
+  return to_world(it, wi);
+}
+
+// Distribution1D sampling: first i with cdf[i] >= u, clamped.
+static int cdf_find(const std::vector<float>& cdf, float u) {
+  int n = (int)cdf.size();
+  int i = (int)(std::lower_bound(cdf.begin(), cdf.end(), u) - cdf.begin());
+  return std::min(i, n - 1);
+}
+
+template <class S>
+struct LightS {
+  V3<S> p, n;
+  S J;
+  float pdf;
+  int tri, emitter;
+};
+
+// AreaLight::Sample (Light.cpp:146-175)
+template <class S>
+static LightS<S> light_sample(Access<S>& A, float u0, float u1) {
+  const Scene& s = A.s;
+  LightS<S> L;
+  int n = (int)s.light_tri.size();
+  int i = cdf_find(s.light_cdf, u0);
+  float pmf = s.light_f[i] / (s.light_int * (float)n);
+  float lo = i > 0 ? s.light_cdf[i - 1] : 0.0f;
+  float reuse = (u0 - lo) / (s.light_cdf[i] - lo);
+  float t = sqrtf(reuse);
+  float bu = 1.0f - t, bv = t * u1, bw = 1.0f - bu - bv;
+  int tri = s.light_tri[i];
+  L.p = (S(bw) * A.pos(s.idx[3 * tri]) + S(bu) * A.pos(s.idx[3 * tri + 1])) + S(bv) * A.pos(s.idx[3 * tri + 2]);
+  S ar;
+  face_nA(A, tri, &L.n, &ar);
+  L.J = ar / detach(ar);
+  L.pdf = pmf / s.area[tri];
+  L.tri = tri;
+  L.emitter = s.tri_emitter[tri];
+  return L;
+}
+
+static float light_pdf(const Scene& s, int tri) {
+  int n = (int)s.light_tri.size();
+  float pmf = s.light_f[s.emit_tri[tri]] / (s.light_int * (float)n);
+  return pmf / s.area[tri];
+}
+
+static F3 emitted(const Scene& s, int emitter, F3 ng, F3 w) {
+  float on = dot(w, ng) > 0.0f ? 1.0f : 0.0f;
+  const float* I = &s.emit_rad[3 * emitter];
+  return F3{I[0], I[1], I[2]} * on;
+}
+
+// ---------------------------------------------------------------------------
+// One path (PathTracer::Radiance + EvalRadianceDirect), scalar or AD.
+// ---------------------------------------------------------------------------
+template <class S>
+static V3<S> radiance(Access<S>& A, const Cam& C, uint32_t seed, int B, uint64_t path) {
+  const Scene& s = A.s;
+  const V3<S> zero{S(0.0f), S(0.0f), S(0.0f)};
+  const uint64_t WH = (uint64_t)C.W * C.H;
+  const uint32_t pix = (uint32_t)(path % WH);
+  const uint32_t px = pix % C.W, py = pix / C.W;
+  R4 rc = rnd4(seed, kDomInterior, path, 0u);
+  F3 d0 = camera_ray_dir(C, (float)px + rc.a, (float)py + rc.b);
+  Hit h = intersect(s, C.eye, d0, SHADOW_EPSILON_, 1e32f, false, false);
+  if (h.tri < 0) return zero;
+  Its<S> it = post_intersect(A, h.tri, h.u, h.v);
+  V3<S> eye = cast3<S>(C.eye);
+  // Camera::EvalFilter (box): G / detach(G), G = |cos_x| / (cos_y^3 d^2); times J
+  S beta0;
+  {
+    V3<S> cd = it.p - eye;
+    S dist = length(cd);
+    cd = cd / dist;
+    S cosy = dot(cd, cast3<S>(C.dir));
+    S cosx = dot(-cd, it.ng);
+    S G = abs_(cosx) / (cosy * cosy * cosy * (dist * dist));
+    S ar = area_of(A, h.tri);
+    beta0 = (G / detach(G)) * (ar / detach(ar));
+  }
+  V3<S> beta{beta0, beta0, beta0};
+  V3<S> raydir = normalize(it.p - eye);
+  V3<S> contrib = zero;
+  if (it.emitter >= 0) {
+    V3<S> e0 = beta * cast3<S>(emitted(s, it.emitter, val3(it.ng), val3(-raydir)));
+    if (finite3(e0)) contrib = contrib + e0;
+  }
+  for (int b = 0; b < B; ++b) {
+    R4 rl = rnd4(seed, kDomInterior, path, 1u + 2u * (uint32_t)b);
+    R4 rb = rnd4(seed, kDomInterior, path, 2u + 2u * (uint32_t)b);
+    V3<S> ret = zero;
+    V3<S> wo = -raydir;
+    Its<float> itf;
+    itf.p = val3(it.p); itf.ng = val3(it.ng); itf.ns = val3(it.ns); itf.t = val3(it.t); itf.b = val3(it.b);
+    itf.tu = it.tu; itf.tv = it.tv; itf.tri = it.tri; itf.mat = it.mat; itf.emitter = it.emitter;
+    // NEE (Integrator.cpp:183-229)
+    LightS<S> ls = light_sample(A, rl.a, rl.b);
+    {
+      V3<S> wi = ls.p - it.p;
+      S d2 = len2(wi);
+      S dist = sqrt_(d2);
+      wi = wi / dist;
+      S dsh = dot(it.ns, wi), dgeo = dot(it.ng, wi);
+      bool stable = val(dist) > SHADOW_EPSILON_ && fabsf(val(dsh)) > EDGE_EPSILON_;
+      bool noleak = (val(dsh) * val(dgeo)) > 0.0f;
+      bool visible = false;
+      if (stable && noleak)
+        visible = intersect(s, itf.p, val3(wi), SHADOW_EPSILON_, val(dist) - SHADOW_EPSILON_, true, false).tri < 0;
+      S valid((stable && noleak && visible) ? 1.0f : 0.0f);
+      S Gl = abs_(dot(ls.n, -wi)) / d2;
+      V3<S> f = bsdf_eval(A, it, wo, wi) * abs_(dsh);
+      V3<S> Le = cast3<S>(emitted(s, ls.emitter, val3(ls.n), val3(-wi)));
+      V3<S> v = valid * beta * f * ls.J * Gl * Le / S(ls.pdf);
+      float pb = bsdf_pdf(s, itf, val3(wo), val3(wi)) * val(Gl);
+      float mis = (ls.pdf * ls.pdf) / (ls.pdf * ls.pdf + pb * pb);
+      v = v * S(mis);
+      ret = ret + v;
+    }
+    // BSDF sampling (Integrator.cpp:231-294)
+    float pdfb;
+    F3 wib = bsdf_sample(s, itf, val3(wo), rb.a, rb.b, &pdfb);
+    Hit h2 = intersect(s, itf.p, wib, SHADOW_EPSILON_, 1e32f, false, false);
+    bool ok = h2.tri >= 0 && pdfb > 0.0f && dir_valid(s, itf, wib);
+    if (!ok) {
+      if (finite3(ret)) contrib = contrib + ret;
+      break;
+    }
+    Its<S> it2 = post_intersect(A, h2.tri, h2.u, h2.v);
+    V3<S> dd = it2.p - it.p;
+    S dd2 = len2(dd);
+    V3<S> nd = dd / sqrt_(dd2);
+    S G2 = abs_(dot(it2.ng, -nd)) / dd2;
+    V3<S> f2 = bsdf_eval(A, it, wo, nd) * abs_(dot(it.ns, nd));
+    float ppdf = pdfb * val(G2);
+    S ar2 = area_of(A, h2.tri);
+    S J2 = ar2 / detach(ar2);
+    V3<S> beta2 = beta * f2 * G2 * J2 / S(ppdf);
+    if (!(val(G2) > 0.0f)) beta2 = zero;
+    if (it2.emitter >= 0) {
+      V3<S> v2 = beta2 * cast3<S>(emitted(s, it2.emitter, val3(it2.ng), val3(-nd)));
+      float pn = light_pdf(s, h2.tri);
+      float mis2 = (ppdf * ppdf) / (ppdf * ppdf + pn * pn);
+      v2 = v2 * S(mis2);
+      ret = ret + v2;
+    }
+    if (finite3(ret)) contrib = contrib + ret;
+    it = it2;
+    beta = beta2;
+    raydir = nd;
+  }
+  return contrib;
+}
+
+static void parallel_for(uint64_t n, int threads, const std::function<void(uint64_t, uint64_t, int)>& f) {
+  if (threads <= 1 || n < 64) {
+    f(0, n, 0);
+    return;
+  }
+  std::atomic<uint64_t> next(0);
+  const uint64_t chunk = 256;
+  std::vector<std::thread> ts;
+  for (int t = 0; t < threads; ++t)
+    ts.emplace_back([&, t] {
+      for (;;) {
+        uint64_t a = next.fetch_add(chunk);
+        if (a >= n) break;
+        f(a, std::min(n, a + chunk), t);
+      }
+    });
+  for (auto& t : ts) t.join();
+}
+
+}  // namespace oracle
+
+// ---------------------------------------------------------------------------
+// C entry points (ctypes)
+// ---------------------------------------------------------------------------
+using namespace oracle;
+
+extern "C" {
+
+void oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  U4 r = philox(U4{ctr[0], ctr[1], ctr[2], ctr[3]}, key[0], key[1]);
+  out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
+}
+
+void oracle_sincos(const float* x, int n, float* s, float* c) {
+  for (int i = 0; i < n; ++i) f_sincos(x[i], &s[i], &c[i]);
+}
+void oracle_exp2(const float* x, int n, float* y) {
+  for (int i = 0; i < n; ++i) y[i] = f_exp2(x[i]);
+}
+
+// materials: [nmat][12] = type, tex, roughness, unused, albedo rgb, spec rgb, unused x2
+void* oracle_scene_create(const float* pos, int nv, const uint32_t* idx, int nt, const int32_t* tri_mat,
+                          const int32_t* tri_emitter, const int32_t* tri_flags, const float* emit_rad, int n_emit,
+                          const float* mats, int nmat, const float* uv, int nuv, const uint32_t* uv_idx) {
+  Scene* s = new Scene();
+  s->V = nv;
+  s->T = nt;
+  s->pos.assign(pos, pos + 3 * (size_t)nv);
+  s->idx.assign(idx, idx + 3 * (size_t)nt);
+  s->tri_mat.assign(tri_mat, tri_mat + nt);
+  s->tri_emitter.assign(tri_emitter, tri_emitter + nt);
+  s->tri_flags.assign(tri_flags, tri_flags + nt);
+  s->emit_rad.assign(emit_rad, emit_rad + 3 * (size_t)n_emit);
+  for (int m = 0; m < nmat; ++m) {
+    const float* q = mats + 12 * m;
+    Mat mm;
+    mm.type = (int)q[0];
+    mm.tex = (int)q[1];
+    mm.roughness = q[2];
+    mm.albedo = F3{q[4], q[5], q[6]};
+    mm.spec = F3{q[7], q[8], q[9]};
+    s->mats.push_back(mm);
+  }
+  if (uv && nuv > 0) {
+    s->uv.assign(uv, uv + 2 * (size_t)nuv);
+    s->uv_idx.assign(uv_idx, uv_idx + 3 * (size_t)nt);
+  }
+  scene_derive(*s);
+  scene_build_bvh(*s);
+  return s;
+}
+
+void oracle_scene_add_texture(void* sp, const float* texels, int w, int h) {
+  Scene* s = (Scene*)sp;
+  s->tex.emplace_back(texels, texels + 3 * (size_t)w * h);
+  s->tex_w.push_back(w);
+  s->tex_h.push_back(h);
+}
+
+void oracle_scene_destroy(void* s) { delete (Scene*)s; }
+
+void oracle_scene_derived(void* sp, float* fn, float* area, float* vn, float* light_cdf, float* light_f,
+                          float* light_int) {
+  Scene* s = (Scene*)sp;
+  for (int t = 0; t < s->T; ++t) {
+    if (fn) { fn[3 * t] = s->fn[t].x; fn[3 * t + 1] = s->fn[t].y; fn[3 * t + 2] = s->fn[t].z; }
+    if (area) area[t] = s->area[t];
+  }
+  if (vn)
+    for (int v = 0; v < s->V; ++v) { vn[3 * v] = s->vn[v].x; vn[3 * v + 1] = s->vn[v].y; vn[3 * v + 2] = s->vn[v].z; }
+  for (size_t i = 0; i < s->light_tri.size(); ++i) {
+    if (light_cdf) light_cdf[i] = s->light_cdf[i];
+    if (light_f) light_f[i] = s->light_f[i];
+  }
+  if (light_int) *light_int = s->light_int;
+}
+
+// rays [n][8] (ox,oy,oz,tmin,dx,dy,dz,tmax)
+void oracle_intersect(void* sp, const float* rays, int n, int brute, int any_hit, int32_t* tri, float* uvt,
+                      int threads) {
+  const Scene& s = *(Scene*)sp;
+  parallel_for((uint64_t)n, threads, [&](uint64_t a, uint64_t b, int) {
+    for (uint64_t i = a; i < b; ++i) {
+      const float* r = rays + 8 * i;
+      Hit h = intersect(s, F3{r[0], r[1], r[2]}, F3{r[4], r[5], r[6]}, r[3], r[7], any_hit != 0, brute != 0);
+      tri[i] = h.tri;
+      uvt[3 * i] = h.u;
+      uvt[3 * i + 1] = h.v;
+      uvt[3 * i + 2] = h.t;
+    }
+  });
+}
+
+// camera: [pos3, target3, up3, fov] + w, h.  out: [12] = eye, right, up, dir
+// as used by the kernels plus ax, bx, ay, by, tan_half, ratio, inv_area.
+void oracle_camera(const float* cam, int w, int h, float* out) {
+  Cam c = make_camera(cam, cam + 3, cam + 6, w, h, cam[9]);
+  float o[19] = {c.eye.x, c.eye.y, c.eye.z, c.right.x, c.right.y, c.right.z, c.up.x, c.up.y, c.up.z,
+                 c.dir.x, c.dir.y, c.dir.z, c.ax, c.bx, c.ay, c.by, c.tan_half, c.ratio, c.inv_area};
+  std::memcpy(out, o, sizeof(o));
+}
+
+void oracle_camera_dirs(const float* cam, int w, int h, const float* raster, int n, float* dirs) {
+  Cam c = make_camera(cam, cam + 3, cam + 6, w, h, cam[9]);
+  for (int i = 0; i < n; ++i) {
+    F3 d = camera_ray_dir(c, raster[2 * i], raster[2 * i + 1]);
+    dirs[3 * i] = d.x; dirs[3 * i + 1] = d.y; dirs[3 * i + 2] = d.z;
+  }
+}
+
+// Per-path radiance for paths [first, first + count): out [count][3].
+void oracle_render_paths(void* sp, const float* cam, int w, int h, uint32_t seed, int max_bounces,
+                         uint64_t first, uint64_t count, float* out, int threads) {
+  const Scene& s = *(Scene*)sp;
+  Cam c = make_camera(cam, cam + 3, cam + 6, w, h, cam[9]);
+  parallel_for(count, threads, [&](uint64_t a, uint64_t b, int) {
+    Access<float> A(s);
+    for (uint64_t i = a; i < b; ++i) {
+      F3 r = radiance(A, c, seed, max_bounces, first + i);
+      out[3 * i] = r.x; out[3 * i + 1] = r.y; out[3 * i + 2] = r.z;
+    }
+  });
+}
+
+// Film [3][H*W] = (sum_s path(s, pix)) / spp, samples summed in order.
+void oracle_render_film(void* sp, const float* cam, int w, int h, uint32_t seed, int max_bounces, int spp,
+                        float* film, int threads) {
+  const Scene& s = *(Scene*)sp;
+  Cam c = make_camera(cam, cam + 3, cam + 6, w, h, cam[9]);
+  const uint64_t WH = (uint64_t)w * h;
+  parallel_for(WH, threads, [&](uint64_t a, uint64_t b, int) {
+    Access<float> A(s);
+    for (uint64_t pix = a; pix < b; ++pix) {
+      float sx = 0, sy = 0, sz = 0;
+      for (int k = 0; k < spp; ++k) {
+        F3 r = radiance(A, c, seed, max_bounces, (uint64_t)k * WH + pix);
+        sx += r.x; sy += r.y; sz += r.z;
+      }
+      float inv = 1.0f / (float)spp;
+      film[pix] = sx * inv;
+      film[WH + pix] = sy * inv;
+      film[2 * WH + pix] = sz * inv;
+    }
+  });
+}
+
+// Reverse mode: dpos [V][3] += d/dV sum_paths <dLdI[pix], path(pix)/spp>,
+// dmat [M][8] (roughness, albedo rgb, spec rgb).  Accumulated in double.
+void oracle_render_adj(void* sp, const float* cam, int w, int h, uint32_t seed, int max_bounces, int spp,
+                       const float* dLdI, uint64_t first, uint64_t count, double* dpos, double* dmat,
+                       int threads) {
+  const Scene& s = *(Scene*)sp;
+  Cam c = make_camera(cam, cam + 3, cam + 6, w, h, cam[9]);
+  const uint64_t WH = (uint64_t)w * h;
+  int nt = std::max(1, threads);
+  std::vector<std::vector<double>> tp(nt, std::vector<double>(3 * (size_t)s.V, 0.0));
+  std::vector<std::vector<double>> tm(nt, std::vector<double>(8 * s.mats.size(), 0.0));
+  const float inv_spp = 1.0f / (float)spp;
+  parallel_for(count, threads, [&](uint64_t a, uint64_t b, int tid) {
+    Tape tape;
+    g_tape = &tape;
+    for (uint64_t i = a; i < b; ++i) {
+      uint64_t path = first + i;
+      uint32_t pix = (uint32_t)(path % WH);
+      float g[3] = {dLdI[pix] * inv_spp, dLdI[WH + pix] * inv_spp, dLdI[2 * WH + pix] * inv_spp};
+      if (g[0] == 0.0f && g[1] == 0.0f && g[2] == 0.0f) continue;
+      tape.clear();
+      Access<Var> A(s);
+      V3<Var> r = radiance(A, c, seed, max_bounces, path);
+      size_t nn = tape.nodes.size();
+      tape.adj.assign(nn, 0.0);
+      if (r.x.i >= 0) tape.adj[r.x.i] += g[0];
+      if (r.y.i >= 0) tape.adj[r.y.i] += g[1];
+      if (r.z.i >= 0) tape.adj[r.z.i] += g[2];
+      for (size_t k = nn; k-- > 0;) {
+        double a_ = tape.adj[k];
+        if (a_ == 0.0) continue;
+        const Node& nd = tape.nodes[k];
+        if (nd.a >= 0) tape.adj[nd.a] += a_ * nd.da;
+        if (nd.b >= 0) tape.adj[nd.b] += a_ * nd.db;
+      }
+      for (auto& kv : A.vcache) {
+        const V3<Var>& lv = kv.second;
+        double* d = &tp[tid][3 * (size_t)kv.first];
+        d[0] += tape.adj[lv.x.i]; d[1] += tape.adj[lv.y.i]; d[2] += tape.adj[lv.z.i];
+      }
+      for (auto& kv : A.rcache) tm[tid][8 * kv.first] += tape.adj[kv.second.i];
+      for (auto& kv : A.acache) {
+        tm[tid][8 * kv.first + 1] += tape.adj[kv.second.x.i];
+        tm[tid][8 * kv.first + 2] += tape.adj[kv.second.y.i];
+        tm[tid][8 * kv.first + 3] += tape.adj[kv.second.z.i];
+      }
+      for (auto& kv : A.scache) {
+        tm[tid][8 * kv.first + 4] += tape.adj[kv.second.x.i];
+        tm[tid][8 * kv.first + 5] += tape.adj[kv.second.y.i];
+        tm[tid][8 * kv.first + 6] += tape.adj[kv.second.z.i];
+      }
+    }
+    g_tape = nullptr;
+  });
+  for (int t = 0; t < nt; ++t) {
+    for (size_t k = 0; k < tp[t].size(); ++k) dpos[k] += tp[t][k];
+    if (dmat)
+      for (size_t k = 0; k < tm[t].size(); ++k) dmat[k] += tm[t][k];
+  }
+}
+
+}  // extern "C"

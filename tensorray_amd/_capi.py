@@ -1,0 +1,248 @@
+"""ctypes binding of the C ABI in include/trc.h (libtrc.so).
+
+This is the thin Python edge over the native library: device buffers come
+from PyTorch-ROCm tensors (passed by `data_ptr()`), streams from
+`torch.cuda.current_stream().cuda_stream`.  There is no CPU fallback: if the
+library or a GPU is missing every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtrc.so")
+
+TRC_OK = 0
+STATUS_NAMES = {1: "TRC_ERR_INVALID", 2: "TRC_ERR_HIP", 3: "TRC_ERR_NOMEM", 4: "TRC_ERR_STATE",
+                5: "TRC_ERR_OVERFLOW"}
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_u32_p = ctypes.POINTER(ctypes.c_uint32)
+c_i32_p = ctypes.POINTER(ctypes.c_int32)
+vp = ctypes.c_void_p
+
+
+class trc_camera(ctypes.Structure):
+    _fields_ = [("pos", ctypes.c_float * 3), ("target", ctypes.c_float * 3), ("up", ctypes.c_float * 3),
+                ("width", ctypes.c_int32), ("height", ctypes.c_int32), ("fov", ctypes.c_float)]
+
+
+class trc_options(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint32), ("max_bounces", ctypes.c_int32), ("spp", ctypes.c_int32),
+                ("sppe", ctypes.c_int32), ("sppse0", ctypes.c_int32), ("sppse1", ctypes.c_int32),
+                ("sppe0", ctypes.c_int32), ("quiet", ctypes.c_int32)]
+
+
+class trc_scene_info(ctypes.Structure):
+    _fields_ = [("num_vertices", ctypes.c_int32), ("num_triangles", ctypes.c_int32),
+                ("num_materials", ctypes.c_int32), ("num_emitter_triangles", ctypes.c_int32),
+                ("num_edges", ctypes.c_int32), ("bvh_nodes", ctypes.c_int32), ("bvh_leaves", ctypes.c_int32),
+                ("bvh_max_depth", ctypes.c_int32), ("bvh_stack_need", ctypes.c_int32),
+                ("bvh_sah_cost", ctypes.c_double), ("device_bytes", ctypes.c_int64)]
+
+
+# (name, restype, argtypes) of every exported entry point
+SIGNATURES = [
+    ("trc_context_create", ctypes.c_int32, [ctypes.c_int32, ctypes.POINTER(vp)]),
+    ("trc_context_destroy", ctypes.c_int32, [vp]),
+    ("trc_last_error", ctypes.c_char_p, [vp]),
+    ("trc_device_cus", ctypes.c_int32, [vp]),
+    ("trc_scene_create", ctypes.c_int32, [vp, ctypes.POINTER(vp)]),
+    ("trc_scene_destroy", ctypes.c_int32, [vp]),
+    ("trc_scene_add_material", ctypes.c_int32, [vp, ctypes.c_int32, c_float_p, c_float_p, ctypes.c_float,
+                                                ctypes.c_int32]),
+    ("trc_scene_add_texture", ctypes.c_int32, [vp, c_float_p, ctypes.c_int32, ctypes.c_int32]),
+    ("trc_scene_add_mesh", ctypes.c_int32, [vp, c_float_p, ctypes.c_int32, c_u32_p, ctypes.c_int32, c_float_p,
+                                            ctypes.c_int32, c_u32_p, ctypes.c_int32, c_float_p, ctypes.c_int32]),
+    ("trc_scene_configure", ctypes.c_int32, [vp, vp]),
+    ("trc_scene_set_vertices", ctypes.c_int32, [vp, vp, ctypes.c_int32, ctypes.c_int32, vp]),
+    ("trc_scene_set_material", ctypes.c_int32, [vp, ctypes.c_int32, c_float_p, c_float_p, ctypes.c_float]),
+    ("trc_scene_get_info", ctypes.c_int32, [vp, ctypes.POINTER(trc_scene_info)]),
+    ("trc_scene_get_vertices", ctypes.c_int32, [vp, c_float_p]),
+    ("trc_scene_get_indices", ctypes.c_int32, [vp, c_u32_p]),
+    ("trc_scene_get_bvh", ctypes.c_int32, [vp, c_u32_p, c_u32_p]),
+    ("trc_scene_get_edges", ctypes.c_int32, [vp, c_i32_p]),
+    ("trc_intersect", ctypes.c_int32, [vp, vp, ctypes.c_int32, vp, vp, ctypes.c_int32, vp]),
+    ("trc_render_fwd", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options), vp, vp]),
+    ("trc_render_paths", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options), vp, vp]),
+    ("trc_render_adj", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options), vp, vp, vp,
+                                        vp]),
+]
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libtrc.so and declare every entry point (no device calls)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"native library {path} is missing: run tensorray_amd.build.build() first")
+    lib = ctypes.CDLL(path)
+    for name, res, args in SIGNATURES:
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _fp(a):
+    return a.ctypes.data_as(c_float_p) if a is not None else None
+
+
+def _up(a):
+    return a.ctypes.data_as(c_u32_p) if a is not None else None
+
+
+class TRCError(RuntimeError):
+    pass
+
+
+class Context:
+    """A device context (EnvCreate of the reference)."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        h = vp()
+        st = self.lib.trc_context_create(device, ctypes.byref(h))
+        self.h = h
+        if st != TRC_OK:
+            msg = self.lib.trc_last_error(h).decode() if h.value else "no context"
+            raise TRCError(f"trc_context_create: {STATUS_NAMES.get(st, st)}: {msg}")
+
+    def check(self, st, what):
+        if st != TRC_OK:
+            raise TRCError(f"{what}: {STATUS_NAMES.get(st, st)}: {self.lib.trc_last_error(self.h).decode()}")
+
+    @property
+    def num_cus(self):
+        return self.lib.trc_device_cus(self.h)
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            self.lib.trc_context_destroy(self.h)
+            self.h = None
+
+
+def make_camera(width, height, pos=(0.0, 1.0, 3.0), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=45.0):
+    c = trc_camera()
+    c.pos[:] = pos
+    c.target[:] = target
+    c.up[:] = up
+    c.width = width
+    c.height = height
+    c.fov = fov
+    return c
+
+
+def make_options(seed=1234, max_bounces=1, spp=1, sppe=0, sppse0=0, sppse1=0, sppe0=0, quiet=1):
+    o = trc_options()
+    o.seed = seed
+    o.max_bounces = max_bounces
+    o.spp = spp
+    o.sppe = sppe
+    o.sppse0 = sppse0
+    o.sppse1 = sppse1
+    o.sppe0 = sppe0
+    o.quiet = quiet
+    return o
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+class DeviceScene:
+    """A scene resident in HBM (Scene + Scene::Configure of the reference)."""
+
+    def __init__(self, ctx, spec, stream=None):
+        self.ctx = ctx
+        lib = ctx.lib
+        h = vp()
+        ctx.check(lib.trc_scene_create(ctx.h, ctypes.byref(h)), "trc_scene_create")
+        self.h = h
+        self.spec = spec
+        for t in spec.textures:
+            t = np.ascontiguousarray(t, np.float32)
+            if lib.trc_scene_add_texture(h, _fp(t), t.shape[1], t.shape[0]) < 0:
+                ctx.check(1, "trc_scene_add_texture")
+        for m in spec.materials:
+            a = np.asarray(m.albedo, np.float32)
+            s = np.asarray(m.spec, np.float32)
+            if lib.trc_scene_add_material(h, m.type, _fp(a), _fp(s), m.roughness, m.texture) < 0:
+                ctx.check(1, "trc_scene_add_material")
+        self._keep = []
+        for m in spec.meshes:
+            pos = np.ascontiguousarray(m.pos, np.float32)
+            idx = np.ascontiguousarray(m.idx, np.uint32)
+            uv = np.ascontiguousarray(m.uv, np.float32) if m.uv is not None else None
+            uvi = np.ascontiguousarray(m.uv_idx, np.uint32) if m.uv_idx is not None else None
+            em = np.asarray(m.emit, np.float32) if m.emit is not None else None
+            r = lib.trc_scene_add_mesh(h, _fp(pos), len(pos), _up(idx), len(idx), _fp(uv),
+                                       len(uv) if uv is not None else 0, _up(uvi), m.material, _fp(em), m.flags)
+            if r < 0:
+                ctx.check(1, "trc_scene_add_mesh")
+        ctx.check(lib.trc_scene_configure(h, _stream_ptr(stream)), "trc_scene_configure")
+
+    def info(self):
+        i = trc_scene_info()
+        self.ctx.check(self.ctx.lib.trc_scene_get_info(self.h, ctypes.byref(i)), "trc_scene_get_info")
+        return i
+
+    def vertices(self):
+        i = self.info()
+        out = np.zeros((i.num_vertices, 3), np.float32)
+        self.ctx.check(self.ctx.lib.trc_scene_get_vertices(self.h, _fp(out)), "trc_scene_get_vertices")
+        return out
+
+    def bvh(self):
+        i = self.info()
+        nodes = np.zeros((i.bvh_nodes, 32), np.uint32)
+        order = np.zeros(i.num_triangles, np.uint32)
+        self.ctx.check(self.ctx.lib.trc_scene_get_bvh(self.h, _up(nodes), _up(order)), "trc_scene_get_bvh")
+        return nodes, order
+
+    def edges(self):
+        i = self.info()
+        out = np.zeros((i.num_edges, 5), np.int32)
+        self.ctx.check(self.ctx.lib.trc_scene_get_edges(self.h, out.ctypes.data_as(c_i32_p)), "trc_scene_get_edges")
+        return out
+
+    def set_vertices(self, pos_ptr, nv, refit=True, stream=None):
+        self.ctx.check(self.ctx.lib.trc_scene_set_vertices(self.h, pos_ptr, nv, int(refit), _stream_ptr(stream)),
+                       "trc_scene_set_vertices")
+
+    def set_material(self, mi, albedo=None, spec=None, roughness=0.0):
+        a = np.asarray(albedo, np.float32) if albedo is not None else None
+        s = np.asarray(spec, np.float32) if spec is not None else None
+        self.ctx.check(self.ctx.lib.trc_scene_set_material(self.h, mi, _fp(a), _fp(s), roughness),
+                       "trc_scene_set_material")
+
+    # ---- device work (pointers are device addresses) ----
+    def intersect(self, rays_ptr, n, tri_ptr, uvt_ptr, any_hit=False, stream=None):
+        self.ctx.check(self.ctx.lib.trc_intersect(self.h, rays_ptr, n, tri_ptr, uvt_ptr, int(any_hit),
+                                                  _stream_ptr(stream)), "trc_intersect")
+
+    def render_fwd(self, cam, opt, film_ptr, stream=None):
+        self.ctx.check(self.ctx.lib.trc_render_fwd(self.h, ctypes.byref(cam), ctypes.byref(opt), film_ptr,
+                                                   _stream_ptr(stream)), "trc_render_fwd")
+
+    def render_paths(self, cam, opt, paths_ptr, stream=None):
+        self.ctx.check(self.ctx.lib.trc_render_paths(self.h, ctypes.byref(cam), ctypes.byref(opt), paths_ptr,
+                                                     _stream_ptr(stream)), "trc_render_paths")
+
+    def render_adj(self, cam, opt, dldi_ptr, dpos_ptr, dmat_ptr=None, stream=None):
+        self.ctx.check(self.ctx.lib.trc_render_adj(self.h, ctypes.byref(cam), ctypes.byref(opt), dldi_ptr, dpos_ptr,
+                                                   dmat_ptr, _stream_ptr(stream)), "trc_render_adj")
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            self.ctx.lib.trc_scene_destroy(self.h)
+            self.h = None

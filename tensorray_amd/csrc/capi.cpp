@@ -1,0 +1,435 @@
+// C ABI implementation (include/trc.h): context, resident scenes, launches.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+
+#include "trc_internal.h"
+
+using namespace trc;
+
+namespace {
+
+struct Fail : std::runtime_error {
+  trc_status code;
+  Fail(trc_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    trc_status c = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? TRC_ERR_NOMEM : TRC_ERR_HIP;
+    throw Fail(c, std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+template <class F>
+trc_status guarded(trc_ctx* ctx, F&& f) {
+  try {
+    f();
+    return TRC_OK;
+  } catch (const Fail& e) {
+    if (ctx) ctx->last_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->last_error = e.what();
+    return TRC_ERR_INVALID;
+  }
+}
+
+constexpr int kSpillDepth = 64;
+constexpr int kBlocksPerCU = 8;
+
+std::map<std::pair<trc_ctx*, hipStream_t>, uint32_t*>& spill_map() {
+  static std::map<std::pair<trc_ctx*, hipStream_t>, uint32_t*> m;
+  return m;
+}
+
+LaunchCfg make_cfg(trc_ctx* ctx, const trc_scene* s, hipStream_t st, long long work_items) {
+  LaunchCfg L;
+  int max_grid = ctx->num_cus * kBlocksPerCU;
+  long long need = (work_items + kBlock - 1) / kBlock;
+  L.grid = (int)(need < max_grid ? (need < 1 ? 1 : need) : max_grid);
+  int stack_need = s->dev.stack_need;
+  if (stack_need > kStackLds + kSpillDepth)
+    throw Fail(TRC_ERR_INVALID, "BVH too deep for the traversal stack (" + std::to_string(stack_need) + ")");
+  L.cap = kStackLds + kSpillDepth;
+  L.err = ctx->err_flag;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  auto key = std::make_pair(ctx, st);
+  auto it = spill_map().find(key);
+  if (it == spill_map().end()) {
+    uint32_t* p = nullptr;
+    size_t words = (size_t)kSpillDepth * (size_t)max_grid * kBlock;
+    hip_check(hipMalloc(&p, words * 4), "hipMalloc(spill)");
+    it = spill_map().emplace(key, p).first;
+  }
+  L.spill = it->second;
+  return L;
+}
+
+template <class T>
+T* upload(trc_scene* s, const std::vector<T>& v, hipStream_t st) {
+  if (v.empty()) return nullptr;
+  DevBuf b;
+  b.bytes = v.size() * sizeof(T);
+  hip_check(hipMalloc(&b.ptr, b.bytes), "hipMalloc(scene)");
+  s->bufs.push_back(b);
+  s->device_bytes += (int64_t)b.bytes;
+  hip_check(hipMemcpyAsync(b.ptr, v.data(), b.bytes, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  return (T*)b.ptr;
+}
+
+template <class T>
+void reupload(const T* dst, const std::vector<T>& v, hipStream_t st) {
+  if (v.empty()) return;
+  hip_check(hipMemcpyAsync((void*)dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st),
+            "hipMemcpyAsync");
+}
+
+void free_scene_bufs(trc_scene* s) {
+  for (auto& b : s->bufs) hipFree(b.ptr);
+  s->bufs.clear();
+  s->device_bytes = 0;
+  s->dev_texs = nullptr;
+}
+
+void upload_scene(trc_scene* s, hipStream_t st) {
+  free_scene_bufs(s);
+  SceneHost& h = s->host;
+  SceneDev d{};
+  d.pos = upload(s, h.pos, st);
+  d.idx = upload(s, h.idx, st);
+  d.vnrm = upload(s, h.vnrm, st);
+  d.uv = upload(s, h.uv, st);
+  d.uv_idx = upload(s, h.uv_idx, st);
+  d.shade = upload(s, h.shade, st);
+  d.tris = upload(s, h.tris, st);
+  d.nodes = upload(s, h.nodes, st);
+  d.mats = upload(s, h.materials, st);
+  std::vector<TexDesc> texs;
+  for (size_t i = 0; i < h.textures.size(); ++i) {
+    TexDesc t;
+    t.texels = upload(s, h.textures[i], st);
+    t.w = h.tex_w[i];
+    t.h = h.tex_h[i];
+    texs.push_back(t);
+  }
+  d.texs = upload(s, texs, st);
+  d.light_tri = upload(s, h.light_tri, st);
+  d.light_cdf = upload(s, h.light_cdf, st);
+  d.light_f = upload(s, h.light_f, st);
+  d.emit_rad = upload(s, h.emit_rad, st);
+  d.n_light = (int32_t)h.light_tri.size();
+  d.light_integral = h.light_integral;
+  d.n_tris = h.num_triangles();
+  d.n_mats = (int32_t)h.materials.size();
+  d.stack_need = h.bvh_stats.stack_need;
+  s->dev = d;
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+CameraDev to_cam(const trc_camera* c) {
+  if (!c) throw Fail(TRC_ERR_INVALID, "camera is NULL");
+  if (c->width <= 0 || c->height <= 0) throw Fail(TRC_ERR_INVALID, "camera size must be positive");
+  if (!(c->fov > 0.0f && c->fov < 180.0f)) throw Fail(TRC_ERR_INVALID, "camera fov out of range");
+  return make_camera(c->pos, c->target, c->up, c->width, c->height, c->fov);
+}
+
+void check_render(const trc_scene* s, const trc_options* o) {
+  if (!s->configured) throw Fail(TRC_ERR_STATE, "scene is not configured");
+  if (!o) throw Fail(TRC_ERR_INVALID, "options are NULL");
+  if (o->spp <= 0) throw Fail(TRC_ERR_INVALID, "spp must be positive");
+  if (o->max_bounces < 0 || o->max_bounces > kMaxBounces)
+    throw Fail(TRC_ERR_INVALID, "max_bounces must be in [0, " + std::to_string(kMaxBounces) + "]");
+  if (s->dev.n_light <= 0) throw Fail(TRC_ERR_INVALID, "scene has no area emitter");
+}
+
+}  // namespace
+
+extern "C" {
+
+trc_status trc_context_create(int32_t device, trc_ctx** out) {
+  if (!out) return TRC_ERR_INVALID;
+  *out = nullptr;
+  trc_ctx* c = new trc_ctx();
+  trc_status st = guarded(c, [&] {
+    int n = 0;
+    hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n) throw Fail(TRC_ERR_INVALID, "no such device");
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    c->device = device;
+    hip_check(hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device),
+              "hipDeviceGetAttribute");
+    hip_check(hipMalloc(&c->err_flag, 4), "hipMalloc(err)");
+    hip_check(hipMemset(c->err_flag, 0, 4), "hipMemset");
+  });
+  if (st != TRC_OK) {
+    // keep the context so the caller can read the message
+    *out = c;
+    return st;
+  }
+  *out = c;
+  return TRC_OK;
+}
+
+trc_status trc_context_destroy(trc_ctx* ctx) {
+  if (!ctx) return TRC_ERR_INVALID;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    for (auto it = spill_map().begin(); it != spill_map().end();) {
+      if (it->first.first == ctx) {
+        hipFree(it->second);
+        it = spill_map().erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+  if (ctx->err_flag) hipFree(ctx->err_flag);
+  delete ctx;
+  return TRC_OK;
+}
+
+const char* trc_last_error(const trc_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+int32_t trc_device_cus(const trc_ctx* ctx) { return ctx ? ctx->num_cus : 0; }
+
+trc_status trc_scene_create(trc_ctx* ctx, trc_scene** out) {
+  if (!ctx || !out) return TRC_ERR_INVALID;
+  trc_scene* s = new trc_scene();
+  s->ctx = ctx;
+  *out = s;
+  return TRC_OK;
+}
+
+trc_status trc_scene_destroy(trc_scene* s) {
+  if (!s) return TRC_ERR_INVALID;
+  hipSetDevice(s->ctx->device);
+  free_scene_bufs(s);
+  delete s;
+  return TRC_OK;
+}
+
+int32_t trc_scene_add_material(trc_scene* s, int32_t type, const float albedo[3], const float spec[3],
+                               float roughness, int32_t texture) {
+  if (!s) return -1;
+  if (type != TRC_MAT_DIFFUSE && type != TRC_MAT_MICROFACET) {
+    s->ctx->last_error = "unknown material type";
+    return -1;
+  }
+  Material m{};
+  m.type = type;
+  m.tex = texture;
+  m.roughness = roughness;
+  for (int k = 0; k < 3; ++k) {
+    m.albedo[k] = albedo ? albedo[k] : 0.0f;
+    m.spec[k] = spec ? spec[k] : 0.0f;
+  }
+  s->host.materials.push_back(m);
+  s->configured = false;
+  return (int32_t)s->host.materials.size() - 1;
+}
+
+int32_t trc_scene_add_texture(trc_scene* s, const float* texels, int32_t w, int32_t h) {
+  if (!s || !texels || w <= 0 || h <= 0) return -1;
+  s->host.textures.emplace_back(texels, texels + (size_t)w * h * 3);
+  s->host.tex_w.push_back(w);
+  s->host.tex_h.push_back(h);
+  s->configured = false;
+  return (int32_t)s->host.textures.size() - 1;
+}
+
+int32_t trc_scene_add_mesh(trc_scene* s, const float* pos, int32_t nv, const uint32_t* idx, int32_t nt,
+                           const float* uv, int32_t nuv, const uint32_t* uv_idx, int32_t material,
+                           const float* emit, int32_t flags) {
+  if (!s || !pos || !idx || nv <= 0 || nt <= 0) {
+    if (s) s->ctx->last_error = "add_mesh: empty or NULL buffers";
+    return -1;
+  }
+  MeshHost m;
+  m.pos.assign(pos, pos + 3 * (size_t)nv);
+  m.idx.assign(idx, idx + 3 * (size_t)nt);
+  if (uv && uv_idx && nuv > 0) {
+    m.uv.assign(uv, uv + 2 * (size_t)nuv);
+    m.uv_idx.assign(uv_idx, uv_idx + 3 * (size_t)nt);
+  }
+  m.material = material;
+  if (emit) {
+    m.is_emitter = true;
+    for (int k = 0; k < 3; ++k) m.radiance[k] = emit[k];
+  }
+  m.smooth = (flags & TRC_MESH_SMOOTH) != 0;
+  m.diff = (flags & TRC_MESH_DIFF) != 0;
+  s->host.meshes.push_back(std::move(m));
+  s->configured = false;
+  return (int32_t)s->host.meshes.size() - 1;
+}
+
+trc_status trc_scene_configure(trc_scene* s, trc_stream stream) {
+  if (!s) return TRC_ERR_INVALID;
+  return guarded(s->ctx, [&] {
+    hip_check(hipSetDevice(s->ctx->device), "hipSetDevice");
+    for (auto& m : s->host.materials)
+      if (m.tex >= (int32_t)s->host.textures.size()) throw Fail(TRC_ERR_INVALID, "material texture out of range");
+    s->host.flatten();
+    s->host.build_bvh();
+    upload_scene(s, (hipStream_t)stream);
+    s->configured = true;
+  });
+}
+
+trc_status trc_scene_set_vertices(trc_scene* s, const float* pos, int32_t nv, int32_t refit, trc_stream stream) {
+  if (!s) return TRC_ERR_INVALID;
+  return guarded(s->ctx, [&] {
+    if (!s->configured) throw Fail(TRC_ERR_STATE, "scene is not configured");
+    if (!pos || nv != s->host.num_vertices()) throw Fail(TRC_ERR_INVALID, "vertex count mismatch");
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<float> tmp(3 * (size_t)nv);
+    hip_check(hipMemcpyAsync(tmp.data(), pos, tmp.size() * 4, hipMemcpyDefault, st), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    s->host.update_positions(tmp.data());
+    SceneDev& d = s->dev;
+    if (refit) {
+      s->host.refit_bvh();
+      reupload(d.pos, s->host.pos, st);
+      reupload(d.vnrm, s->host.vnrm, st);
+      reupload(d.shade, s->host.shade, st);
+      reupload(d.tris, s->host.tris, st);
+      reupload(d.nodes, s->host.nodes, st);
+      reupload(d.light_cdf, s->host.light_cdf, st);
+      reupload(d.light_f, s->host.light_f, st);
+      d.light_integral = s->host.light_integral;
+      hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    } else {
+      s->host.build_bvh();
+      upload_scene(s, st);
+    }
+  });
+}
+
+trc_status trc_scene_set_material(trc_scene* s, int32_t mi, const float albedo[3], const float spec[3],
+                                  float roughness) {
+  if (!s) return TRC_ERR_INVALID;
+  return guarded(s->ctx, [&] {
+    if (mi < 0 || mi >= (int32_t)s->host.materials.size()) throw Fail(TRC_ERR_INVALID, "material out of range");
+    Material& m = s->host.materials[mi];
+    for (int k = 0; k < 3; ++k) {
+      if (albedo) m.albedo[k] = albedo[k];
+      if (spec) m.spec[k] = spec[k];
+    }
+    m.roughness = roughness;
+    if (s->configured) {
+      hip_check(hipMemcpy((void*)(s->dev.mats + mi), &m, sizeof(Material), hipMemcpyHostToDevice), "hipMemcpy");
+    }
+  });
+}
+
+trc_status trc_scene_get_info(const trc_scene* s, trc_scene_info* out) {
+  if (!s || !out) return TRC_ERR_INVALID;
+  const SceneHost& h = s->host;
+  out->num_vertices = h.num_vertices();
+  out->num_triangles = h.num_triangles();
+  out->num_materials = (int32_t)h.materials.size();
+  out->num_emitter_triangles = (int32_t)h.light_tri.size();
+  out->num_edges = (int32_t)h.edges.size();
+  out->bvh_nodes = h.bvh_stats.nodes;
+  out->bvh_leaves = h.bvh_stats.leaves;
+  out->bvh_max_depth = h.bvh_stats.max_depth;
+  out->bvh_stack_need = h.bvh_stats.stack_need;
+  out->bvh_sah_cost = h.bvh_stats.sah_cost;
+  out->device_bytes = s->device_bytes;
+  return TRC_OK;
+}
+
+trc_status trc_scene_get_vertices(const trc_scene* s, float* out) {
+  if (!s || !out) return TRC_ERR_INVALID;
+  std::memcpy(out, s->host.pos.data(), s->host.pos.size() * 4);
+  return TRC_OK;
+}
+
+trc_status trc_scene_get_indices(const trc_scene* s, uint32_t* out) {
+  if (!s || !out) return TRC_ERR_INVALID;
+  std::memcpy(out, s->host.idx.data(), s->host.idx.size() * 4);
+  return TRC_OK;
+}
+
+trc_status trc_scene_get_bvh(const trc_scene* s, uint32_t* nodes, uint32_t* order) {
+  if (!s) return TRC_ERR_INVALID;
+  if (nodes) std::memcpy(nodes, s->host.nodes.data(), s->host.nodes.size() * sizeof(Bvh4Node));
+  if (order) std::memcpy(order, s->host.tri_order.data(), s->host.tri_order.size() * 4);
+  return TRC_OK;
+}
+
+trc_status trc_scene_get_edges(const trc_scene* s, int32_t* out) {
+  if (!s || !out) return TRC_ERR_INVALID;
+  for (size_t i = 0; i < s->host.edges.size(); ++i) {
+    const EdgeHost& e = s->host.edges[i];
+    out[5 * i] = (int32_t)e.v0; out[5 * i + 1] = (int32_t)e.v1; out[5 * i + 2] = (int32_t)e.v2;
+    out[5 * i + 3] = e.f0; out[5 * i + 4] = e.f1;
+  }
+  return TRC_OK;
+}
+
+trc_status trc_intersect(trc_scene* s, const float* rays, int32_t n, int32_t* tri, float* uvt, int32_t any_hit,
+                         trc_stream stream) {
+  if (!s) return TRC_ERR_INVALID;
+  return guarded(s->ctx, [&] {
+    if (!s->configured) throw Fail(TRC_ERR_STATE, "scene is not configured");
+    if (n < 0 || (n > 0 && (!rays || !tri || !uvt))) throw Fail(TRC_ERR_INVALID, "bad ray buffers");
+    if (n == 0) return;
+    LaunchCfg L = make_cfg(s->ctx, s, (hipStream_t)stream, n);
+    hip_check(launch_intersect(s->dev, L, rays, n, tri, uvt, any_hit, (hipStream_t)stream), "k_intersect");
+  });
+}
+
+trc_status trc_render_paths(trc_scene* s, const trc_camera* cam, const trc_options* opt, float* paths,
+                            trc_stream stream) {
+  if (!s) return TRC_ERR_INVALID;
+  return guarded(s->ctx, [&] {
+    check_render(s, opt);
+    CameraDev C = to_cam(cam);
+    if (!paths) throw Fail(TRC_ERR_INVALID, "paths buffer is NULL");
+    long long np = (long long)opt->spp * C.width * C.height;
+    LaunchCfg L = make_cfg(s->ctx, s, (hipStream_t)stream, np);
+    hip_check(launch_render_paths(s->dev, C, opt->seed, opt->max_bounces, opt->spp, L, paths, (hipStream_t)stream),
+              "k_render_paths");
+  });
+}
+
+trc_status trc_render_fwd(trc_scene* s, const trc_camera* cam, const trc_options* opt, float* film,
+                          trc_stream stream) {
+  if (!s) return TRC_ERR_INVALID;
+  return guarded(s->ctx, [&] {
+    check_render(s, opt);
+    CameraDev C = to_cam(cam);
+    if (!film) throw Fail(TRC_ERR_INVALID, "film is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    long long np = (long long)opt->spp * C.width * C.height;
+    float* paths = nullptr;
+    hip_check(hipMallocAsync((void**)&paths, (size_t)np * 12, st), "hipMallocAsync(paths)");
+    LaunchCfg L = make_cfg(s->ctx, s, st, np);
+    hipError_t e = launch_render_paths(s->dev, C, opt->seed, opt->max_bounces, opt->spp, L, paths, st);
+    if (e == hipSuccess) e = launch_film_reduce(paths, opt->spp, C.width * C.height, film, st);
+    hipFreeAsync(paths, st);
+    hip_check(e, "k_render_paths/k_film_reduce");
+  });
+}
+
+trc_status trc_render_adj(trc_scene* s, const trc_camera* cam, const trc_options* opt, const float* dLdI,
+                          float* dpos, float* dmat, trc_stream stream) {
+  if (!s) return TRC_ERR_INVALID;
+  return guarded(s->ctx, [&] {
+    check_render(s, opt);
+    CameraDev C = to_cam(cam);
+    if (!dLdI || !dpos) throw Fail(TRC_ERR_INVALID, "dLdI / dpos is NULL");
+    long long np = (long long)opt->spp * C.width * C.height;
+    LaunchCfg L = make_cfg(s->ctx, s, (hipStream_t)stream, np);
+    hip_check(launch_render_adj(s->dev, C, opt->seed, opt->max_bounces, opt->spp, L, dLdI, dpos, dmat,
+                                (hipStream_t)stream),
+              "k_render_adj");
+  });
+}
+
+}  // extern "C"

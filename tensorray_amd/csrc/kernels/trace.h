@@ -1,0 +1,203 @@
+// Ray queries against the 4-wide BVH: watertight FP32 ray/triangle test and
+// closest-hit / any-hit traversal with a short stack in LDS that spills to a
+// per-thread HBM slice only when a path is deeper than the LDS part.
+//
+// Replaces RayIntersectOptix (LibTensorRay/Renderer/Optix.cpp:252-305) and
+// the OptiX programs __raygen__/__miss__/__closesthit__ (ptx.cu:61-113).
+// Hit record semantics follow ptx.cu:92-113: a miss has tri = -1, u = v = 0,
+// t = -1; barycentrics (u, v) weight vertices 1 and 2.
+#pragma once
+#include "../common/tr_layout.h"
+
+namespace trc {
+
+// Ray precomputation of the watertight test (Woop, Benthin, Wald, "Watertight
+// Ray/Triangle Intersection", JCGT 2(1), 2013): kz is the dominant axis of
+// the direction, kx / ky the other two (swapped when d[kz] < 0 to keep the
+// winding), S the shear constants.
+struct RayPre {
+  F3 o;
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+};
+
+TR_HD float comp(F3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+TR_HD RayPre ray_pre(F3 o, F3 d) {
+  RayPre r;
+  r.o = o;
+  float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  int kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+  int kx = kz == 2 ? 0 : kz + 1;
+  int ky = kx == 2 ? 0 : kx + 1;
+  float dz = comp(d, kz);
+  if (dz < 0.0f) { int t = kx; kx = ky; ky = t; }
+  r.kx = kx; r.ky = ky; r.kz = kz;
+  r.Sx = comp(d, kx) / dz;
+  r.Sy = comp(d, ky) / dz;
+  r.Sz = 1.0f / dz;
+  return r;
+}
+
+// Watertight test.  Returns true and (t, u, v) when the ray's line crosses
+// the triangle (edges included, degenerate triangles excluded); the caller
+// applies the (tmin, tmax) interval.
+TR_HD bool tri_intersect(const RayPre& r, F3 p0, F3 p1, F3 p2, float* t_out, float* u_out,
+                         float* v_out) {
+  F3 A = p0 - r.o, B = p1 - r.o, C = p2 - r.o;
+  float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
+  float Ax = comp(A, r.kx) - r.Sx * Akz;
+  float Ay = comp(A, r.ky) - r.Sy * Akz;
+  float Bx = comp(B, r.kx) - r.Sx * Bkz;
+  float By = comp(B, r.ky) - r.Sy * Bkz;
+  float Cx = comp(C, r.kx) - r.Sx * Ckz;
+  float Cy = comp(C, r.ky) - r.Sy * Ckz;
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    // edge case: redo the 2-D edge functions in double precision
+    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+    V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  float det = U + V + W;
+  if (det == 0.0f) return false;
+  float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+  float T = U * Az + V * Bz + W * Cz;
+  float inv = 1.0f / det;
+  *t_out = T * inv;
+  *u_out = V * inv;
+  *v_out = W * inv;
+  return true;
+}
+
+struct Hit {
+  int32_t tri;
+  float u, v, t;
+};
+
+// Conservative slab test factor 1 + 2*gamma(3) (gamma(n) = n eps / (1 - n eps),
+// eps = 2^-24): the far distance is inflated so rounding never culls a box
+// the exact ray enters.
+constexpr float kSlabScale = 1.0000003576278687f;
+
+TR_HD float safe_inv(float d) {
+  return fabsf(d) > 1e-30f ? 1.0f / d : (d >= 0.0f ? 1e30f : -1e30f);
+}
+
+// Per-thread traversal context: LDS part of the stack (stride = block size,
+// lane-interleaved so consecutive lanes use consecutive banks) and an HBM
+// spill slice for deeper entries.
+struct TraceCtx {
+  uint32_t* lds;        // &lds_stack[threadIdx.x]
+  int lds_stride;       // blockDim.x
+  int lds_depth;        // entries held in LDS
+  uint32_t* spill;      // &spill[gtid]
+  int spill_stride;     // total threads of the grid
+  int cap;              // total stack capacity (lds_depth + spill depth)
+  uint32_t* err;        // set to 1 if a traversal ever overflows (never expected)
+};
+
+__device__ inline void stack_push(TraceCtx& c, int& sp, uint32_t v) {
+  if (sp < c.lds_depth) c.lds[sp * c.lds_stride] = v;
+  else c.spill[(sp - c.lds_depth) * c.spill_stride] = v;
+  ++sp;
+}
+__device__ inline uint32_t stack_pop(TraceCtx& c, int& sp) {
+  --sp;
+  return sp < c.lds_depth ? c.lds[sp * c.lds_stride] : c.spill[(sp - c.lds_depth) * c.spill_stride];
+}
+
+__device__ inline F3 tri_vertex(const float* v) { return F3{v[0], v[1], v[2]}; }
+
+// Closest hit (any_hit = false) or first found hit (any_hit = true) in
+// (tmin, tmax).  Ties in t resolve to the smallest triangle id, so the result
+// does not depend on traversal order.
+template <bool kAnyHit>
+__device__ Hit trace_ray(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, float tmin, float tmax) {
+  RayPre rp = ray_pre(o, d);
+  const float ix = safe_inv(d.x), iy = safe_inv(d.y), iz = safe_inv(d.z);
+  Hit h;
+  h.tri = -1; h.u = 0.0f; h.v = 0.0f; h.t = -1.0f;
+  float best_t = tmax;
+  uint32_t best_id = 0xFFFFFFFFu;
+  int sp = 0;
+  uint32_t node = 0;
+  for (;;) {
+    const float4* np = reinterpret_cast<const float4*>(S.nodes + node);
+    float4 lox = np[0], loy = np[1], loz = np[2], hix = np[3], hiy = np[4], hiz = np[5];
+    uint4 ch = reinterpret_cast<const uint4*>(np)[6];
+    float lo_x[4] = {lox.x, lox.y, lox.z, lox.w}, lo_y[4] = {loy.x, loy.y, loy.z, loy.w};
+    float lo_z[4] = {loz.x, loz.y, loz.z, loz.w}, hi_x[4] = {hix.x, hix.y, hix.z, hix.w};
+    float hi_y[4] = {hiy.x, hiy.y, hiy.z, hiy.w}, hi_z[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+    uint32_t cid[4] = {ch.x, ch.y, ch.z, ch.w};
+    // inner children hit: (distance, node); others keep (inf, kEmpty)
+    float kt[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+    uint32_t kc[4] = {kEmpty, kEmpty, kEmpty, kEmpty};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      float t0x = (lo_x[s] - o.x) * ix, t1x = (hi_x[s] - o.x) * ix;
+      float t0y = (lo_y[s] - o.y) * iy, t1y = (hi_y[s] - o.y) * iy;
+      float t0z = (lo_z[s] - o.z) * iz, t1z = (hi_z[s] - o.z) * iz;
+      float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+      float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best_t));
+      uint32_t c = cid[s];
+      bool hit = c != kEmpty && tn <= tf * kSlabScale;
+      if (!hit) continue;
+      if (c & kLeafBit) {
+        uint32_t first = leaf_first(c), cnt = leaf_count(c);
+        for (uint32_t i = 0; i < cnt; ++i) {
+          const TriRecord& tr = S.tris[first + i];
+          float4 a = *reinterpret_cast<const float4*>(tr.v0);
+          float4 b = *reinterpret_cast<const float4*>(tr.v1);
+          float4 e = *reinterpret_cast<const float4*>(tr.v2);
+          float tt, uu, vv;
+          if (!tri_intersect(rp, F3{a.x, a.y, a.z}, F3{b.x, b.y, b.z}, F3{e.x, e.y, e.z}, &tt, &uu, &vv))
+            continue;
+          uint32_t id = __float_as_uint(a.w);
+          if (!(tt > tmin && tt < tmax)) continue;
+          if (kAnyHit) {
+            h.tri = (int32_t)id; h.u = uu; h.v = vv; h.t = tt;
+            return h;
+          }
+          if (tt < best_t || (tt == best_t && id < best_id)) {
+            best_t = tt; best_id = id; h.u = uu; h.v = vv;
+          }
+        }
+      } else {
+        kt[s] = tn;
+        kc[s] = c;
+      }
+    }
+    // 4-element sorting network, near to far
+#define TR_CSWAP(i, j)                                                   \
+    if (kt[j] < kt[i]) {                                                 \
+      float _t = kt[i]; kt[i] = kt[j]; kt[j] = _t;                       \
+      uint32_t _c = kc[i]; kc[i] = kc[j]; kc[j] = _c;                    \
+    }
+    TR_CSWAP(0, 1) TR_CSWAP(2, 3) TR_CSWAP(0, 2) TR_CSWAP(1, 3) TR_CSWAP(1, 2)
+#undef TR_CSWAP
+    if (kc[0] != kEmpty) {
+      int npush = (kc[1] != kEmpty) + (kc[2] != kEmpty) + (kc[3] != kEmpty);
+      if (sp + npush > ctx.cap) { *ctx.err = 1u; break; }
+      if (kc[3] != kEmpty) stack_push(ctx, sp, kc[3]);
+      if (kc[2] != kEmpty) stack_push(ctx, sp, kc[2]);
+      if (kc[1] != kEmpty) stack_push(ctx, sp, kc[1]);
+      node = kc[0];
+      continue;
+    }
+    if (sp == 0) break;
+    node = stack_pop(ctx, sp);
+  }
+  if (best_id != 0xFFFFFFFFu) {
+    h.tri = (int32_t)best_id;
+    h.t = best_t;
+  } else {
+    h.u = 0.0f; h.v = 0.0f;
+  }
+  return h;
+}
+
+}  // namespace trc

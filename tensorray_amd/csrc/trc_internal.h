@@ -1,0 +1,60 @@
+// Internal state behind the C ABI: context and resident scene.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/trc.h"
+#include "common/tr_layout.h"
+#include "host/scene_host.h"
+
+struct trc_ctx {
+  int device = 0;
+  int num_cus = 0;
+  std::string last_error;
+  uint32_t* err_flag = nullptr;   // device: traversal overflow flag
+  // traversal stack spill, grown on demand: [depth][threads]
+  uint32_t* spill = nullptr;
+  size_t spill_words = 0;
+  std::mutex mu;
+};
+
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct trc_scene {
+  trc_ctx* ctx = nullptr;
+  trc::SceneHost host;
+  bool configured = false;
+  std::vector<DevBuf> bufs;       // all device allocations of this scene
+  trc::SceneDev dev{};
+  trc::TexDesc* dev_texs = nullptr;
+  int64_t device_bytes = 0;
+};
+
+namespace trc {
+
+// launch configuration shared by the megakernels
+constexpr int kBlock = 128;
+constexpr int kStackLds = 16;
+
+struct LaunchCfg {
+  int grid;             // persistent grid size (blocks)
+  uint32_t* spill;      // spill buffer [depth][grid*kBlock]
+  int cap;              // stack capacity
+  uint32_t* err;
+};
+
+hipError_t launch_intersect(const SceneDev& S, const LaunchCfg& L, const float* rays, int n, int32_t* tri,
+                            float* uvt, int any_hit, hipStream_t st);
+hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
+                               const LaunchCfg& L, float* paths, hipStream_t st);
+hipError_t launch_film_reduce(const float* paths, int spp, int wh, float* film, hipStream_t st);
+hipError_t launch_render_adj(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
+                             const LaunchCfg& L, const float* dLdI, float* dpos, float* dmat, hipStream_t st);
+
+}  // namespace trc
