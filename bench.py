@@ -1,0 +1,206 @@
+"""Benchmark: Mpaths/s of forward + adjoint on one MI355X.
+
+Workload (metric workload M): 512x512 pinhole camera (fov 45) at (0, 1, 3)
+looking at the origin; an icosphere of subdivision 5 (20,480 triangles)
+displaced along its normals by seeded 3-octave value noise (amplitude 0.05,
+noise seed 1) with a GGX microfacet BSDF (roughness 0.3, F0 0.04, kd 0) on a
+Lambertian ground quad under a 1x1 area light (radiance 10) at y = 2.
+16 spp, max depth 4 (3 NEE + BSDF bounces after the primary hit), FP32.
+
+One step = forward render of all W*H*spp paths (per-path radiance + film
+reduction) + dL/dI = I - I_target on the device + the adjoint (path replay
+and backward sweep scattering into vertex and material gradients).
+I_target is the same scene with displacement amplitude 0.06, rendered once
+with seed 99.  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line (the contract of the driver).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+MI355X_HBM_PEAK = 8.0e12  # B/s, published
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--width", type=int, default=512)
+    p.add_argument("--height", type=int, default=512)
+    p.add_argument("--spp", type=int, default=16)
+    p.add_argument("--depth", type=int, default=4)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-spp", type=int, default=8, help="samples per pixel of the CPU baseline sample")
+    return p.parse_args()
+
+
+def algorithmic_bytes(stats, spp_paths, B, n_light_tris):
+    """Algorithmic HBM bytes per path of the forward and the adjoint pass.
+
+    Per ray query: 32 B ray + 16 B hit + 128 B per BVH node visited + 48 B
+    per triangle tested (averages measured on the same rays and BVH by the
+    statistics kernel).  Per bounce: 56 B shading data + NEE light-table
+    search 4*ceil(log2 T_L) + 48 B light triangle; film read-modify-write 24 B.
+    The adjoint replays the forward (same queries) and adds 12 B of dL/dI and
+    per path vertex the vertex-gradient RMW (3 vertices * 12 B * 2) and the
+    material-gradient RMW (2 * 4 B).
+    """
+    paths, queries, nodes, tris = stats
+    q_per_path = queries / paths
+    b_query = 32 + 16 + 128 * (nodes / queries) + 48 * (tris / queries)
+    bounces = max(0.0, (q_per_path - 1.0) / 2.0)  # 1 primary + (shadow + bsdf) per bounce
+    b_shade = 56 * (1 + bounces)
+    b_nee = bounces * (4 * math.ceil(math.log2(max(2, n_light_tris))) + 48)
+    b_fwd = q_per_path * b_query + b_shade + b_nee + 24
+    b_adj = b_fwd + 12 + (1 + bounces) * (3 * 12 * 2 + 2 * 4)
+    return b_fwd, b_adj, dict(queries_per_path=q_per_path, nodes_per_query=nodes / queries,
+                              tris_per_query=tris / queries, bytes_per_query=b_query)
+
+
+def cpu_baseline(spec, cam, W, H, B, spp, cpu_spp):
+    """The oracle (scalar C++ restatement, all host threads) on a bounded
+    sample of the same workload: the first cpu_spp samples of every pixel,
+    forward + adjoint."""
+    import numpy as np
+    import oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    S = oracle.Scene(spec)
+    n = cpu_spp * W * H
+    t0 = time.perf_counter()
+    S.render_paths(cam, W, H, 1234, B, 0, n, threads=threads)
+    t1 = time.perf_counter()
+    dl = np.full((3, W * H), 0.01, np.float32)
+    S.render_adj(cam, W, H, 1234, B, spp, dl, first=0, count=n, threads=threads)
+    t2 = time.perf_counter()
+    return dict(value=n / (t2 - t0) / 1e6, unit="Mpaths/s", cores=threads, kind="port",
+                sample=f"{cpu_spp} spp x {W}x{H} = {n} paths of the same scene, depth {B + 1}, forward + adjoint; "
+                       f"fwd {t1 - t0:.2f} s, adj {t2 - t1:.2f} s")
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): every box of this build has a single "
+              f"MI355X; see the sharding paragraph of DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from tensorray_amd import _capi, scenes
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.set_device(0)
+    W, H, spp, B = a.width, a.height, a.spp, a.depth - 1
+    cam_d = scenes.CAMERA_DEFAULT
+    ctx = _capi.Context(0)
+    spec = scenes.metric_scene(amplitude=0.05, noise_seed=1)
+    tspec = scenes.metric_scene(amplitude=0.06, noise_seed=1)
+    scene = _capi.DeviceScene(ctx, spec)
+    tscene = _capi.DeviceScene(ctx, tspec)
+    info = scene.info()
+    cam = _capi.make_camera(W, H, **cam_d)
+    opt = _capi.make_options(seed=1234, max_bounces=B, spp=spp)
+    topt = _capi.make_options(seed=99, max_bounces=B, spp=spp)
+    stream = torch.cuda.current_stream()
+    P = W * H * spp
+    target = torch.zeros(3, W * H, device="cuda")
+    tscene.render_fwd(cam, topt, target.data_ptr(), stream)
+    film = torch.zeros(3, W * H, device="cuda")
+    paths = torch.empty(P, 3, device="cuda")
+    dpos = torch.zeros(info.num_vertices, 3, device="cuda")
+    dmat = torch.zeros(info.num_materials, 8, device="cuda")
+    lib = ctx.lib
+    torch.cuda.synchronize()
+    stats = scene.render_stats(cam, opt, stream)
+
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    fwd_ms, adj_ms = [], []
+
+    def step(timed):
+        if timed:
+            ev[0].record(stream)
+        scene.render_paths(cam, opt, paths.data_ptr(), stream)
+        if timed:
+            ev[1].record(stream)
+        # film = mean over samples (deterministic reduction kernel)
+        scene.render_fwd_reduce(paths.data_ptr(), spp, W * H, film.data_ptr(), stream)
+        dldi = film - target
+        dpos.zero_()
+        dmat.zero_()
+        if timed:
+            ev[2].record(stream)
+        scene.render_adj(cam, opt, dldi.data_ptr(), dpos.data_ptr(), dmat.data_ptr(), stream)
+        if timed:
+            ev[3].record(stream)
+
+    for _ in range(a.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step(True)
+        torch.cuda.synchronize()
+        fwd_ms.append(ev[0].elapsed_time(ev[1]))
+        adj_ms.append(ev[2].elapsed_time(ev[3]))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / a.steps
+    mpaths = P / (ms_per_step * 1e-3) / 1e6
+
+    b_fwd, b_adj, st = algorithmic_bytes(stats, P, B, info.num_emitter_triangles)
+    fwd_avg = sum(fwd_ms) / len(fwd_ms)
+    adj_avg = sum(adj_ms) / len(adj_ms)
+    if adj_avg >= fwd_avg:
+        kname, kms, kbytes = "k_render_adj", adj_avg, b_adj * P
+    else:
+        kname, kms, kbytes = "k_render_paths", fwd_avg, b_fwd * P
+    achieved = kbytes / (kms * 1e-3)
+    traffic = None
+    tpath = os.path.join(HERE, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        try:
+            with open(tpath) as f:
+                traffic = json.load(f).get(kname)
+        except (OSError, ValueError):
+            traffic = None
+    roofline = dict(bound="hbm", achieved=achieved / 1e9, peak=MI355X_HBM_PEAK / 1e9, unit="GB/s",
+                    frac=achieved / MI355X_HBM_PEAK, traffic=traffic, kernel=kname, kernel_ms=kms,
+                    algorithmic_bytes_per_launch=kbytes, fwd_kernel_ms=fwd_avg, adj_kernel_ms=adj_avg, **st)
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline(spec, cam_d, W, H, B, spp, a.cpu_spp)
+    out = {
+        "metric": "Mpaths/s fwd+adjoint (512², depth 4) & p99 latency, served vs plain, 1 GPU",
+        "value": mpaths,
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded displaced icosphere scene)",
+        "config": {"workload": f"plain: one request per launch sequence, {W}x{H}, {spp} spp, depth {B + 1}, "
+                               f"{info.num_triangles} triangles (20,480-tri displaced icosphere + ground + light), "
+                               f"forward + adjoint", "mode": "plain", "paths_per_step": P},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    scene.close()
+    tscene.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -75,6 +75,9 @@ typedef struct trc_scene_info {
 
 /* ---- context: replaces EnvCreate / EnvRelease (pybind_utils.h:38-61) ---- */
 trc_status trc_context_create(int32_t device, trc_ctx** out);
+/* A context without a device: scene assembly and trc_scene_prepare only
+ * (host-side validation of the flattened scene, edge lists and BVH). */
+trc_status trc_context_create_host(trc_ctx** out);
 trc_status trc_context_destroy(trc_ctx* ctx);
 const char* trc_last_error(const trc_ctx* ctx);
 /* Number of compute units of the context's device. */
@@ -97,6 +100,9 @@ int32_t trc_scene_add_mesh(trc_scene* scene, const float* pos, int32_t nv, const
 /* Scene::Configure (Scene.cpp:522-533): normals, areas, emitter CDF, edge
  * lists, SAH 4-wide BVH, upload.  Synchronous with respect to `stream`. */
 trc_status trc_scene_configure(trc_scene* scene, trc_stream stream);
+/* Host half of trc_scene_configure (flatten, derived data, edges, BVH)
+ * without any device work. */
+trc_status trc_scene_prepare(trc_scene* scene);
 /* Scene::Update (Scene.cpp:535-546): replace all vertex positions ([V][3],
  * host or device pointer) and refit (refit = 1) or rebuild (0) the BVH. */
 trc_status trc_scene_set_vertices(trc_scene* scene, const float* pos, int32_t nv, int32_t refit,
@@ -128,6 +134,14 @@ trc_status trc_render_fwd(trc_scene* scene, const trc_camera* cam, const trc_opt
 /* Per-path radiance (before the 1/spp factor), [spp*H*W][3], path index =
  * sample * H*W + pixel.  Used by tests and the serving loop. */
 trc_status trc_render_paths(trc_scene* scene, const trc_camera* cam, const trc_options* opt, float* paths,
+                            trc_stream stream);
+/* film[c][pix] = (sum_s paths[s*wh + pix][c]) / spp, summed in sample order
+ * (deterministic); the reduction that follows trc_render_paths. */
+trc_status trc_film_from_paths(trc_ctx* ctx, const float* paths, int32_t spp, int32_t wh, float* film,
+                               trc_stream stream);
+/* Diagnostic: work counters of the forward paths, counters[4] = (paths,
+ * ray queries, BVH node visits, triangle tests).  Synchronous. */
+trc_status trc_render_stats(trc_scene* scene, const trc_camera* cam, const trc_options* opt, uint64_t* counters,
                             trc_stream stream);
 /* Adjoint: accumulates dL/dV += J^T dLdI into dpos ([V][3] device) and the
  * material gradients into dmat ([M][8] device, may be NULL). */

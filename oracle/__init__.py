@@ -62,6 +62,8 @@ def lib():
                                           ctypes.c_uint64, P(f32), ctypes.c_int]
         L.oracle_render_film.argtypes = [vp, P(f32), ctypes.c_int, ctypes.c_int, u32, ctypes.c_int, ctypes.c_int,
                                          P(f32), ctypes.c_int]
+        L.oracle_path_twin.argtypes = [vp, P(f32), ctypes.c_int, ctypes.c_int, u32, ctypes.c_int, ctypes.c_uint64,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, P(f64)]
         L.oracle_render_adj.argtypes = [vp, P(f32), ctypes.c_int, ctypes.c_int, u32, ctypes.c_int, ctypes.c_int,
                                         P(f32), ctypes.c_uint64, ctypes.c_uint64, P(f64), P(f64), ctypes.c_int]
         _lib = L
@@ -182,6 +184,17 @@ class Scene:
         lib().oracle_render_film(self.h, _p(c, ctypes.c_float), w, h, seed, max_bounces, spp,
                                  _p(film, ctypes.c_float), threads or default_threads())
         return film
+
+    PARAM_KINDS = {"vertex": 0, "roughness": 1, "albedo": 2, "spec": 3}
+
+    def path_twin_fd(self, cam, w, h, seed, max_bounces, path, kind, index, comp=0, delta=1e-4):
+        """Central difference (float64) of one path's material-form estimator
+        with respect to one parameter; returns an RGB vector."""
+        out = np.zeros(6, np.float64)
+        c = self._cam(cam)
+        lib().oracle_path_twin(self.h, _p(c, ctypes.c_float), w, h, seed, max_bounces, int(path),
+                               self.PARAM_KINDS[kind], int(index), int(comp), float(delta), _p(out, ctypes.c_double))
+        return (out[:3] - out[3:]) / (2.0 * delta)
 
     def render_adj(self, cam, w, h, seed, max_bounces, spp, dLdI, first=0, count=None, threads=None):
         """Returns (dpos [V,3], dmat [M,8]) in float64."""

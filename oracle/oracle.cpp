@@ -173,6 +173,30 @@ static inline float exp2_(float a) { return f_exp2(a); }
 static inline float detach(float a) { return a; }
 static inline float val(float a) { return a; }
 
+// Twin evaluation for finite differences of the material-form estimator:
+// `b` is the unperturbed float value that drives every comparison, branch
+// and detach(); `s` is the perturbed evaluation in float64.  With a vertex
+// coordinate shifted by +-delta in `s` only, (F(+d) - F(-d)) / 2d is a
+// central difference of exactly the function the reverse-mode AD
+// differentiates (same hit triangles, barycentrics and detached factors).
+struct Pair {
+  float b;
+  double s;
+  Pair() : b(0.0f), s(0.0) {}
+  Pair(float x) : b(x), s((double)x) {}
+  Pair(float x, double y) : b(x), s(y) {}
+};
+static inline Pair operator+(Pair a, Pair c) { return Pair(a.b + c.b, a.s + c.s); }
+static inline Pair operator-(Pair a, Pair c) { return Pair(a.b - c.b, a.s - c.s); }
+static inline Pair operator-(Pair a) { return Pair(-a.b, -a.s); }
+static inline Pair operator*(Pair a, Pair c) { return Pair(a.b * c.b, a.s * c.s); }
+static inline Pair operator/(Pair a, Pair c) { return Pair(a.b / c.b, a.s / c.s); }
+static inline Pair sqrt_(Pair a) { return Pair(sqrtf(a.b), std::sqrt(a.s)); }
+static inline Pair abs_(Pair a) { return Pair(fabsf(a.b), a.b >= 0.0f ? a.s : -a.s); }
+static inline Pair exp2_(Pair a) { return Pair(f_exp2(a.b), std::exp2(a.s)); }
+static inline Pair detach(Pair a) { return Pair(a.b, (double)a.b); }
+static inline float val(Pair a) { return a.b; }
+
 template <class S>
 struct V3 {
   S x, y, z;
@@ -561,6 +585,41 @@ struct Access<Var> {
     scache.emplace(m, r);
     return r;
   }
+};
+
+// Twin access: one parameter shifted by `delta` in the perturbed lane.
+// kind 0: vertex coordinate (index, comp); 1: roughness (index = material);
+// 2: albedo (index, comp); 3: specular (index, comp).
+template <>
+struct Access<Pair> {
+  const Scene& s;
+  int kind, index, comp;
+  double delta;
+  Access(const Scene& sc, int k, int i, int c, double d) : s(sc), kind(k), index(i), comp(c), delta(d) {}
+  V3<Pair> pos(uint32_t v) {
+    F3 p = P(s, v);
+    V3<Pair> r{Pair(p.x), Pair(p.y), Pair(p.z)};
+    if (kind == 0 && (int)v == index) {
+      Pair* c = comp == 0 ? &r.x : (comp == 1 ? &r.y : &r.z);
+      c->s += delta;
+    }
+    return r;
+  }
+  Pair rough(int m) {
+    Pair r(s.mats[m].roughness);
+    if (kind == 1 && m == index) r.s += delta;
+    return r;
+  }
+  V3<Pair> vec(F3 a, int m, int k) {
+    V3<Pair> r{Pair(a.x), Pair(a.y), Pair(a.z)};
+    if (kind == k && m == index) {
+      Pair* c = comp == 0 ? &r.x : (comp == 1 ? &r.y : &r.z);
+      c->s += delta;
+    }
+    return r;
+  }
+  V3<Pair> albedo(int m) { return vec(s.mats[m].albedo, m, 2); }
+  V3<Pair> spec(int m) { return vec(s.mats[m].spec, m, 3); }
 };
 
 template <class S>
@@ -1131,6 +1190,21 @@ void oracle_render_film(void* sp, const float* cam, int w, int h, uint32_t seed,
       film[2 * WH + pix] = sz * inv;
     }
   });
+}
+
+// Central difference of one path's material-form estimator: out[0..2] =
+// F(+delta), out[3..5] = F(-delta) (see Pair).
+void oracle_path_twin(void* sp, const float* cam, int w, int h, uint32_t seed, int max_bounces, uint64_t path,
+                      int kind, int index, int comp, double delta, double* out) {
+  const Scene& s = *(Scene*)sp;
+  Cam c = make_camera(cam, cam + 3, cam + 6, w, h, cam[9]);
+  for (int sgn = 0; sgn < 2; ++sgn) {
+    Access<Pair> A(s, kind, index, comp, sgn == 0 ? delta : -delta);
+    V3<Pair> r = radiance(A, c, seed, max_bounces, path);
+    out[3 * sgn] = r.x.s;
+    out[3 * sgn + 1] = r.y.s;
+    out[3 * sgn + 2] = r.z.s;
+  }
 }
 
 // Reverse mode: dpos [V][3] += d/dV sum_paths <dLdI[pix], path(pix)/spp>,

@@ -45,6 +45,7 @@ class trc_scene_info(ctypes.Structure):
 # (name, restype, argtypes) of every exported entry point
 SIGNATURES = [
     ("trc_context_create", ctypes.c_int32, [ctypes.c_int32, ctypes.POINTER(vp)]),
+    ("trc_context_create_host", ctypes.c_int32, [ctypes.POINTER(vp)]),
     ("trc_context_destroy", ctypes.c_int32, [vp]),
     ("trc_last_error", ctypes.c_char_p, [vp]),
     ("trc_device_cus", ctypes.c_int32, [vp]),
@@ -56,6 +57,7 @@ SIGNATURES = [
     ("trc_scene_add_mesh", ctypes.c_int32, [vp, c_float_p, ctypes.c_int32, c_u32_p, ctypes.c_int32, c_float_p,
                                             ctypes.c_int32, c_u32_p, ctypes.c_int32, c_float_p, ctypes.c_int32]),
     ("trc_scene_configure", ctypes.c_int32, [vp, vp]),
+    ("trc_scene_prepare", ctypes.c_int32, [vp]),
     ("trc_scene_set_vertices", ctypes.c_int32, [vp, vp, ctypes.c_int32, ctypes.c_int32, vp]),
     ("trc_scene_set_material", ctypes.c_int32, [vp, ctypes.c_int32, c_float_p, c_float_p, ctypes.c_float]),
     ("trc_scene_get_info", ctypes.c_int32, [vp, ctypes.POINTER(trc_scene_info)]),
@@ -66,6 +68,9 @@ SIGNATURES = [
     ("trc_intersect", ctypes.c_int32, [vp, vp, ctypes.c_int32, vp, vp, ctypes.c_int32, vp]),
     ("trc_render_fwd", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options), vp, vp]),
     ("trc_render_paths", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options), vp, vp]),
+    ("trc_film_from_paths", ctypes.c_int32, [vp, vp, ctypes.c_int32, ctypes.c_int32, vp, vp]),
+    ("trc_render_stats", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options),
+                                          ctypes.POINTER(ctypes.c_uint64), vp]),
     ("trc_render_adj", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options), vp, vp, vp,
                                         vp]),
 ]
@@ -104,10 +109,15 @@ class TRCError(RuntimeError):
 class Context:
     """A device context (EnvCreate of the reference)."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, host_only=False):
+        """host_only: no device (scene assembly and validation only)."""
         self.lib = load_library()
+        self.host_only = host_only
         h = vp()
-        st = self.lib.trc_context_create(device, ctypes.byref(h))
+        if host_only:
+            st = self.lib.trc_context_create_host(ctypes.byref(h))
+        else:
+            st = self.lib.trc_context_create(device, ctypes.byref(h))
         self.h = h
         if st != TRC_OK:
             msg = self.lib.trc_last_error(h).decode() if h.value else "no context"
@@ -189,7 +199,10 @@ class DeviceScene:
                                        len(uv) if uv is not None else 0, _up(uvi), m.material, _fp(em), m.flags)
             if r < 0:
                 ctx.check(1, "trc_scene_add_mesh")
-        ctx.check(lib.trc_scene_configure(h, _stream_ptr(stream)), "trc_scene_configure")
+        if ctx.host_only:
+            ctx.check(lib.trc_scene_prepare(h), "trc_scene_prepare")
+        else:
+            ctx.check(lib.trc_scene_configure(h, _stream_ptr(stream)), "trc_scene_configure")
 
     def info(self):
         i = trc_scene_info()
@@ -237,6 +250,17 @@ class DeviceScene:
     def render_paths(self, cam, opt, paths_ptr, stream=None):
         self.ctx.check(self.ctx.lib.trc_render_paths(self.h, ctypes.byref(cam), ctypes.byref(opt), paths_ptr,
                                                      _stream_ptr(stream)), "trc_render_paths")
+
+    def render_fwd_reduce(self, paths_ptr, spp, wh, film_ptr, stream=None):
+        self.ctx.check(self.ctx.lib.trc_film_from_paths(self.ctx.h, paths_ptr, spp, wh, film_ptr,
+                                                        _stream_ptr(stream)), "trc_film_from_paths")
+
+    def render_stats(self, cam, opt, stream=None):
+        """(paths, queries, node visits, triangle tests) of the forward paths."""
+        out = (ctypes.c_uint64 * 4)()
+        self.ctx.check(self.ctx.lib.trc_render_stats(self.h, ctypes.byref(cam), ctypes.byref(opt), out,
+                                                     _stream_ptr(stream)), "trc_render_stats")
+        return tuple(int(x) for x in out)
 
     def render_adj(self, cam, opt, dldi_ptr, dpos_ptr, dmat_ptr=None, stream=None):
         self.ctx.check(self.ctx.lib.trc_render_adj(self.h, ctypes.byref(cam), ctypes.byref(opt), dldi_ptr, dpos_ptr,

@@ -150,6 +150,14 @@ void check_render(const trc_scene* s, const trc_options* o) {
 
 extern "C" {
 
+trc_status trc_context_create_host(trc_ctx** out) {
+  if (!out) return TRC_ERR_INVALID;
+  trc_ctx* c = new trc_ctx();
+  c->device = -1;
+  *out = c;
+  return TRC_OK;
+}
+
 trc_status trc_context_create(int32_t device, trc_ctx** out) {
   if (!out) return TRC_ERR_INVALID;
   *out = nullptr;
@@ -206,8 +214,10 @@ trc_status trc_scene_create(trc_ctx* ctx, trc_scene** out) {
 
 trc_status trc_scene_destroy(trc_scene* s) {
   if (!s) return TRC_ERR_INVALID;
-  hipSetDevice(s->ctx->device);
-  free_scene_bufs(s);
+  if (s->ctx->device >= 0) {
+    hipSetDevice(s->ctx->device);
+    free_scene_bufs(s);
+  }
   delete s;
   return TRC_OK;
 }
@@ -267,14 +277,25 @@ int32_t trc_scene_add_mesh(trc_scene* s, const float* pos, int32_t nv, const uin
   return (int32_t)s->host.meshes.size() - 1;
 }
 
+namespace {
+void prepare_host(trc_scene* s) {
+  for (auto& m : s->host.materials)
+    if (m.tex >= (int32_t)s->host.textures.size()) throw Fail(TRC_ERR_INVALID, "material texture out of range");
+  s->host.flatten();
+  s->host.build_bvh();
+}
+}  // namespace
+
+trc_status trc_scene_prepare(trc_scene* s) {
+  if (!s) return TRC_ERR_INVALID;
+  return guarded(s->ctx, [&] { prepare_host(s); });
+}
+
 trc_status trc_scene_configure(trc_scene* s, trc_stream stream) {
   if (!s) return TRC_ERR_INVALID;
   return guarded(s->ctx, [&] {
     hip_check(hipSetDevice(s->ctx->device), "hipSetDevice");
-    for (auto& m : s->host.materials)
-      if (m.tex >= (int32_t)s->host.textures.size()) throw Fail(TRC_ERR_INVALID, "material texture out of range");
-    s->host.flatten();
-    s->host.build_bvh();
+    prepare_host(s);
     upload_scene(s, (hipStream_t)stream);
     s->configured = true;
   });
@@ -414,6 +435,36 @@ trc_status trc_render_fwd(trc_scene* s, const trc_camera* cam, const trc_options
     if (e == hipSuccess) e = launch_film_reduce(paths, opt->spp, C.width * C.height, film, st);
     hipFreeAsync(paths, st);
     hip_check(e, "k_render_paths/k_film_reduce");
+  });
+}
+
+trc_status trc_film_from_paths(trc_ctx* ctx, const float* paths, int32_t spp, int32_t wh, float* film,
+                               trc_stream stream) {
+  if (!ctx) return TRC_ERR_INVALID;
+  return guarded(ctx, [&] {
+    if (!paths || !film || spp <= 0 || wh <= 0) throw Fail(TRC_ERR_INVALID, "bad film reduction arguments");
+    hip_check(launch_film_reduce(paths, spp, wh, film, (hipStream_t)stream), "k_film_reduce");
+  });
+}
+
+trc_status trc_render_stats(trc_scene* s, const trc_camera* cam, const trc_options* opt, uint64_t* counters,
+                            trc_stream stream) {
+  if (!s) return TRC_ERR_INVALID;
+  return guarded(s->ctx, [&] {
+    check_render(s, opt);
+    CameraDev C = to_cam(cam);
+    if (!counters) throw Fail(TRC_ERR_INVALID, "counters is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    long long np = (long long)opt->spp * C.width * C.height;
+    unsigned long long* d = nullptr;
+    hip_check(hipMalloc((void**)&d, 4 * sizeof(unsigned long long)), "hipMalloc(counters)");
+    hipError_t e = hipMemsetAsync(d, 0, 4 * sizeof(unsigned long long), st);
+    LaunchCfg L = make_cfg(s->ctx, s, st, np);
+    if (e == hipSuccess) e = launch_render_stats(s->dev, C, opt->seed, opt->max_bounces, opt->spp, L, d, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(counters, d, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(d);
+    hip_check(e, "k_render_stats");
   });
 }
 

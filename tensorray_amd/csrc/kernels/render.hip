@@ -26,6 +26,7 @@ __device__ inline TraceCtx make_trace_ctx(uint32_t* lds, const LaunchCfg& L) {
   t.spill_stride = L.grid * kBlock;
   t.cap = L.cap;
   t.err = L.err;
+  t.n_nodes = t.n_tris = t.n_queries = 0u;
   return t;
 }
 
@@ -56,6 +57,25 @@ __global__ void __launch_bounds__(kBlock) k_render_paths(SceneDev S, CameraDev C
     paths[3 * p + 1] = c.y;
     paths[3 * p + 2] = c.z;
   }
+}
+
+// Work statistics of the forward paths: counters[0..3] += (paths, queries,
+// node visits, triangle tests).  Used to compute algorithmic bytes.
+__global__ void __launch_bounds__(kBlock) k_render_stats(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
+                                                         uint64_t npaths, unsigned long long* counters) {
+  __shared__ uint32_t lds_stack[kStackLds * kBlock];
+  TraceCtx tc = make_trace_ctx(lds_stack, L);
+  const uint64_t stride = (uint64_t)L.grid * kBlock;
+  uint64_t n = 0;
+  for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < npaths; p += stride) {
+    F3 c = path_forward<false>(S, C, P, p, tc, nullptr);
+    if (c.x == -1.0f) tc.n_queries++;  // keeps the path live
+    ++n;
+  }
+  atomicAdd(&counters[0], (unsigned long long)n);
+  atomicAdd(&counters[1], (unsigned long long)tc.n_queries);
+  atomicAdd(&counters[2], (unsigned long long)tc.n_nodes);
+  atomicAdd(&counters[3], (unsigned long long)tc.n_tris);
 }
 
 // film[c][pix] = (sum_s paths[s*WH + pix][c]) * (1/spp), summed in sample order.
@@ -113,6 +133,14 @@ hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t s
   PathParams P{seed, max_bounces, spp, 0u};
   uint64_t np = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
   hipLaunchKernelGGL(k_render_paths, dim3(L.grid), dim3(kBlock), 0, st, S, C, P, L, np, paths);
+  return hipGetLastError();
+}
+
+hipError_t launch_render_stats(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
+                               const LaunchCfg& L, unsigned long long* counters, hipStream_t st) {
+  PathParams P{seed, max_bounces, spp, 0u};
+  uint64_t np = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
+  hipLaunchKernelGGL(k_render_stats, dim3(L.grid), dim3(kBlock), 0, st, S, C, P, L, np, counters);
   return hipGetLastError();
 }
 

@@ -98,6 +98,9 @@ struct TraceCtx {
   int spill_stride;     // total threads of the grid
   int cap;              // total stack capacity (lds_depth + spill depth)
   uint32_t* err;        // set to 1 if a traversal ever overflows (never expected)
+  // work counters (node visits, triangle tests, queries); read only by the
+  // statistics kernel, otherwise dead code
+  uint32_t n_nodes, n_tris, n_queries;
 };
 
 __device__ inline void stack_push(TraceCtx& c, int& sp, uint32_t v) {
@@ -117,6 +120,7 @@ __device__ inline F3 tri_vertex(const float* v) { return F3{v[0], v[1], v[2]}; }
 // does not depend on traversal order.
 template <bool kAnyHit>
 __device__ Hit trace_ray(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, float tmin, float tmax) {
+  ctx.n_queries++;
   RayPre rp = ray_pre(o, d);
   const float ix = safe_inv(d.x), iy = safe_inv(d.y), iz = safe_inv(d.z);
   Hit h;
@@ -126,6 +130,7 @@ __device__ Hit trace_ray(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, float tmi
   int sp = 0;
   uint32_t node = 0;
   for (;;) {
+    ctx.n_nodes++;
     const float4* np = reinterpret_cast<const float4*>(S.nodes + node);
     float4 lox = np[0], loy = np[1], loz = np[2], hix = np[3], hiy = np[4], hiz = np[5];
     uint4 ch = reinterpret_cast<const uint4*>(np)[6];
@@ -149,6 +154,7 @@ __device__ Hit trace_ray(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, float tmi
       if (c & kLeafBit) {
         uint32_t first = leaf_first(c), cnt = leaf_count(c);
         for (uint32_t i = 0; i < cnt; ++i) {
+          ctx.n_tris++;
           const TriRecord& tr = S.tris[first + i];
           float4 a = *reinterpret_cast<const float4*>(tr.v0);
           float4 b = *reinterpret_cast<const float4*>(tr.v1);

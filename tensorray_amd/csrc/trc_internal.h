@@ -53,6 +53,8 @@ hipError_t launch_intersect(const SceneDev& S, const LaunchCfg& L, const float* 
                             float* uvt, int any_hit, hipStream_t st);
 hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                                const LaunchCfg& L, float* paths, hipStream_t st);
+hipError_t launch_render_stats(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
+                               const LaunchCfg& L, unsigned long long* counters, hipStream_t st);
 hipError_t launch_film_reduce(const float* paths, int spp, int wh, float* film, hipStream_t st);
 hipError_t launch_render_adj(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                              const LaunchCfg& L, const float* dLdI, float* dpos, float* dmat, hipStream_t st);
