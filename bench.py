@@ -39,6 +39,9 @@ def parse():
     p.add_argument("--depth", type=int, default=4)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-spp", type=int, default=8, help="samples per pixel of the CPU baseline sample")
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="after the timed region, copy a MIB-sized device buffer (reference kernel of known "
+                        "traffic for scaling the FETCH_SIZE / WRITE_SIZE counters)")
     return p.parse_args()
 
 
@@ -174,6 +177,13 @@ def main():
     roofline = dict(bound="hbm", achieved=achieved / 1e9, peak=MI355X_HBM_PEAK / 1e9, unit="GB/s",
                     frac=achieved / MI355X_HBM_PEAK, traffic=traffic, kernel=kname, kernel_ms=kms,
                     algorithmic_bytes_per_launch=kbytes, fwd_kernel_ms=fwd_avg, adj_kernel_ms=adj_avg, **st)
+    if a.copy_calibration:
+        src = torch.ones(a.copy_calibration * (1 << 20) // 4, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(spec, cam_d, W, H, B, spp, a.cpu_spp)
