@@ -143,6 +143,14 @@ trc_status trc_film_from_paths(trc_ctx* ctx, const float* paths, int32_t spp, in
  * ray queries, BVH node visits, triangle tests).  Synchronous. */
 trc_status trc_render_stats(trc_scene* scene, const trc_camera* cam, const trc_options* opt, uint64_t* counters,
                             trc_stream stream);
+/* Boundary (edge) terms of the gradient, accumulated into dpos ([V][3]):
+ *   kind 0  primary edges      (PrimaryBoundaryIntegrator::RenderD, BoundaryPrimary.cpp:36-181), opt->sppe
+ *   kind 1  pixel boundary     (PixelBoundaryIntegrator::RenderD, BoundaryPixel.cpp:36-159), opt->sppe0
+ *   kind 2  secondary, direct  (DirectBoundaryIntegrator::RenderD, BoundaryDirect.cpp:38-143), opt->sppse0
+ *   kind 3  secondary, indirect (IndirectBoundaryIntegrator::RenderD, BoundaryIndirect.cpp:38-192), opt->sppse1
+ * Samples per pixel 0 skips the term.  Their forward value is zero. */
+trc_status trc_render_boundary(trc_scene* scene, const trc_camera* cam, const trc_options* opt, int32_t kind,
+                               const float* dLdI, float* dpos, trc_stream stream);
 /* Adjoint: accumulates dL/dV += J^T dLdI into dpos ([V][3] device) and the
  * material gradients into dmat ([M][8] device, may be NULL). */
 trc_status trc_render_adj(trc_scene* scene, const trc_camera* cam, const trc_options* opt, const float* dLdI,

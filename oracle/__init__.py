@@ -208,3 +208,27 @@ class Scene:
                                 first, count, _p(dpos, ctypes.c_double), _p(dmat, ctypes.c_double),
                                 threads or default_threads())
         return dpos, dmat
+
+
+def _boundary_kinds():
+    return {"primary": 0, "pixel": 1, "direct": 2, "indirect": 3}
+
+
+def boundary_adj(scene, cam, w, h, seed, max_bounces, kind, spp, dLdI, first=0, count=None, threads=None):
+    """Gradient [V,3] (float64) of one boundary estimator (primary, pixel,
+    direct, indirect) for samples [first, first + count) of spp * w * h."""
+    L = lib()
+    if not hasattr(L, "_bnd_decl"):
+        P = ctypes.POINTER
+        L.oracle_boundary_adj.argtypes = [ctypes.c_void_p, P(ctypes.c_float), ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_float), ctypes.c_uint64,
+                                          ctypes.c_uint64, P(ctypes.c_double), ctypes.c_int]
+        L._bnd_decl = True
+    dLdI = np.ascontiguousarray(dLdI, np.float32).reshape(3, w * h)
+    if count is None:
+        count = spp * w * h
+    dpos = np.zeros((scene.V, 3), np.float64)
+    c = scene._cam(cam)
+    L.oracle_boundary_adj(scene.h, _p(c, ctypes.c_float), w, h, seed, max_bounces, _boundary_kinds()[kind], spp,
+                          _p(dLdI, ctypes.c_float), first, count, _p(dpos, ctypes.c_double), threads or default_threads())
+    return dpos

@@ -926,43 +926,23 @@ static F3 emitted(const Scene& s, int emitter, F3 ng, F3 w) {
 }
 
 // ---------------------------------------------------------------------------
-// One path (PathTracer::Radiance + EvalRadianceDirect), scalar or AD.
+// Bounce loop (PathTracer.cpp:55-73 + EvalRadianceDirect,
+// Integrator.cpp:178-303) from a surface point `it` reached along `raydir`
+// with throughput `beta`.  Random numbers of bounce b come from blocks
+// block0 + 2b (light) and block0 + 2b + 1 (BSDF) of (seed, domain, index).
+// per_bounce (optional): float value added by each executed bounce.
+// Returns the number of executed bounces.
 // ---------------------------------------------------------------------------
 template <class S>
-static V3<S> radiance(Access<S>& A, const Cam& C, uint32_t seed, int B, uint64_t path) {
+static int bounces(Access<S>& A, Its<S> it, V3<S> beta, V3<S> raydir, uint32_t seed, uint32_t domain,
+                   uint64_t index, uint32_t block0, int B, V3<S>& contrib, F3* per_bounce) {
   const Scene& s = A.s;
   const V3<S> zero{S(0.0f), S(0.0f), S(0.0f)};
-  const uint64_t WH = (uint64_t)C.W * C.H;
-  const uint32_t pix = (uint32_t)(path % WH);
-  const uint32_t px = pix % C.W, py = pix / C.W;
-  R4 rc = rnd4(seed, kDomInterior, path, 0u);
-  F3 d0 = camera_ray_dir(C, (float)px + rc.a, (float)py + rc.b);
-  Hit h = intersect(s, C.eye, d0, SHADOW_EPSILON_, 1e32f, false, false);
-  if (h.tri < 0) return zero;
-  Its<S> it = post_intersect(A, h.tri, h.u, h.v);
-  V3<S> eye = cast3<S>(C.eye);
-  // Camera::EvalFilter (box): G / detach(G), G = |cos_x| / (cos_y^3 d^2); times J
-  S beta0;
-  {
-    V3<S> cd = it.p - eye;
-    S dist = length(cd);
-    cd = cd / dist;
-    S cosy = dot(cd, cast3<S>(C.dir));
-    S cosx = dot(-cd, it.ng);
-    S G = abs_(cosx) / (cosy * cosy * cosy * (dist * dist));
-    S ar = area_of(A, h.tri);
-    beta0 = (G / detach(G)) * (ar / detach(ar));
-  }
-  V3<S> beta{beta0, beta0, beta0};
-  V3<S> raydir = normalize(it.p - eye);
-  V3<S> contrib = zero;
-  if (it.emitter >= 0) {
-    V3<S> e0 = beta * cast3<S>(emitted(s, it.emitter, val3(it.ng), val3(-raydir)));
-    if (finite3(e0)) contrib = contrib + e0;
-  }
+  int executed = 0;
   for (int b = 0; b < B; ++b) {
-    R4 rl = rnd4(seed, kDomInterior, path, 1u + 2u * (uint32_t)b);
-    R4 rb = rnd4(seed, kDomInterior, path, 2u + 2u * (uint32_t)b);
+    R4 rl = rnd4(seed, domain, index, block0 + 2u * (uint32_t)b);
+    R4 rb = rnd4(seed, domain, index, block0 + 2u * (uint32_t)b + 1u);
+    ++executed;
     V3<S> ret = zero;
     V3<S> wo = -raydir;
     Its<float> itf;
@@ -997,7 +977,10 @@ static V3<S> radiance(Access<S>& A, const Cam& C, uint32_t seed, int B, uint64_t
     Hit h2 = intersect(s, itf.p, wib, SHADOW_EPSILON_, 1e32f, false, false);
     bool ok = h2.tri >= 0 && pdfb > 0.0f && dir_valid(s, itf, wib);
     if (!ok) {
-      if (finite3(ret)) contrib = contrib + ret;
+      if (finite3(ret)) {
+        contrib = contrib + ret;
+        if (per_bounce) per_bounce[b] = val3(ret);
+      }
       break;
     }
     Its<S> it2 = post_intersect(A, h2.tri, h2.u, h2.v);
@@ -1018,12 +1001,641 @@ static V3<S> radiance(Access<S>& A, const Cam& C, uint32_t seed, int B, uint64_t
       v2 = v2 * S(mis2);
       ret = ret + v2;
     }
-    if (finite3(ret)) contrib = contrib + ret;
+    if (finite3(ret)) {
+      contrib = contrib + ret;
+      if (per_bounce) per_bounce[b] = val3(ret);
+    }
     it = it2;
     beta = beta2;
     raydir = nd;
   }
+  return executed;
+}
+
+// ---------------------------------------------------------------------------
+// One path (PathTracer::Radiance), scalar or AD.
+// ---------------------------------------------------------------------------
+template <class S>
+static V3<S> radiance(Access<S>& A, const Cam& C, uint32_t seed, int B, uint64_t path) {
+  const Scene& s = A.s;
+  const V3<S> zero{S(0.0f), S(0.0f), S(0.0f)};
+  const uint64_t WH = (uint64_t)C.W * C.H;
+  const uint32_t pix = (uint32_t)(path % WH);
+  const uint32_t px = pix % C.W, py = pix / C.W;
+  R4 rc = rnd4(seed, kDomInterior, path, 0u);
+  F3 d0 = camera_ray_dir(C, (float)px + rc.a, (float)py + rc.b);
+  Hit h = intersect(s, C.eye, d0, SHADOW_EPSILON_, 1e32f, false, false);
+  if (h.tri < 0) return zero;
+  Its<S> it = post_intersect(A, h.tri, h.u, h.v);
+  V3<S> eye = cast3<S>(C.eye);
+  // Camera::EvalFilter (box): G / detach(G), G = |cos_x| / (cos_y^3 d^2); times J
+  S beta0;
+  {
+    V3<S> cd = it.p - eye;
+    S dist = length(cd);
+    cd = cd / dist;
+    S cosy = dot(cd, cast3<S>(C.dir));
+    S cosx = dot(-cd, it.ng);
+    S G = abs_(cosx) / (cosy * cosy * cosy * (dist * dist));
+    S ar = area_of(A, h.tri);
+    beta0 = (G / detach(G)) * (ar / detach(ar));
+  }
+  V3<S> beta{beta0, beta0, beta0};
+  V3<S> raydir = normalize(it.p - eye);
+  V3<S> contrib = zero;
+  if (it.emitter >= 0) {
+    V3<S> e0 = beta * cast3<S>(emitted(s, it.emitter, val3(it.ng), val3(-raydir)));
+    if (finite3(e0)) contrib = contrib + e0;
+  }
+  bounces(A, it, beta, raydir, seed, kDomInterior, path, 1u, B, contrib, nullptr);
   return contrib;
+}
+
+// ===========================================================================
+// Boundary (edge) terms.  Each sample contributes only a gradient:
+//   term = detach(K) * xDotN,  xDotN = n . u2,
+//   u2 = barycentric point (detached vertices) of RayIntersectAD(dir, org, tri)
+// (BoundaryPrimary.cpp:131-163, BoundaryPixel.cpp:108-148, Edge.cpp:265-287,
+// BoundaryIndirect.cpp:117-131).  K (radiance, importance, baseVal / pdf) is
+// evaluated in float; xDotN on the tape.
+// ===========================================================================
+constexpr uint32_t kDomPrimaryEdge = 0x22u;
+constexpr uint32_t kDomPixelEdge = 0x33u;
+constexpr uint32_t kDomDirectEdge = 0x44u;
+constexpr uint32_t kDomIndirectEdge = 0x55u;
+constexpr uint32_t kSensorBlock0 = 64u;
+
+struct EdgeRec {
+  uint32_t v0, v1, v2;
+  int f0, f1;
+  F3 p0, e1, n0, n1, p2;
+  bool boundary;
+};
+
+// ObjMesh::processEdges over the flattened index buffer (meshes do not share
+// vertices, and later meshes have larger vertex ids, so the global key order
+// equals the per-mesh order concatenated).
+static std::vector<EdgeRec> all_edges(const Scene& s) {
+  std::map<std::pair<uint32_t, uint32_t>, std::vector<int>> m;
+  for (int t = 0; t < s.T; ++t)
+    for (int j = 0; j < 3; ++j) {
+      uint32_t a = s.idx[3 * t + j], b = s.idx[3 * t + (j + 1) % 3], c = s.idx[3 * t + (j + 2) % 3];
+      auto key = a < b ? std::make_pair(a, b) : std::make_pair(b, a);
+      auto it = m.find(key);
+      if (it == m.end()) it = m.emplace(key, std::vector<int>{(int)c}).first;
+      it->second.push_back(t);
+    }
+  std::vector<EdgeRec> out;
+  for (auto& kv : m) {
+    EdgeRec e;
+    e.v0 = kv.first.first;
+    e.v1 = kv.first.second;
+    e.v2 = (uint32_t)kv.second[0];
+    e.f0 = kv.second[1];
+    e.f1 = kv.second.size() == 3 ? kv.second[2] : -1;
+    e.boundary = e.f1 < 0;
+    e.p0 = P(s, e.v0);
+    e.e1 = P(s, e.v1) - e.p0;
+    e.p2 = P(s, e.v2);
+    e.n0 = s.fn[e.f0];
+    e.n1 = e.boundary ? F3{0, 0, 0} : s.fn[e.f1];
+    out.push_back(e);
+  }
+  return out;
+}
+
+// ConstructSecEdgeList (Edge.cpp:42-88)
+static std::vector<EdgeRec> sec_edges(const Scene& s) {
+  std::vector<EdgeRec> out;
+  for (const EdgeRec& e : all_edges(s)) {
+    bool keep = e.boundary;
+    if (!e.boundary) {
+      bool v0 = dot(e.n0, e.n1) < 1.0f - 1e-6f;
+      bool v1 = dot(e.n1, normalize(e.p2 - e.p0)) < -EDGE_EPSILON_;
+      keep = v0 && v1;
+    }
+    if (keep) out.push_back(e);
+  }
+  return out;
+}
+
+// ConstructPrimEdgeList (Edge.cpp:313-348)
+static std::vector<EdgeRec> prim_edges(const Scene& s, const Cam& c) {
+  std::vector<EdgeRec> out;
+  for (const EdgeRec& e : all_edges(s)) {
+    F3 d = c.eye - e.p0;
+    bool ff0 = dot(d, e.n0) > 0.0f, ff1 = dot(d, e.n1) > 0.0f;
+    if (e.boundary || ff0 != ff1) out.push_back(e);
+  }
+  return out;
+}
+
+// Distribution1D over edge lengths (Distribution.cpp:45-58, Distribution.cu:37-48)
+struct Dist1 {
+  std::vector<float> f, cdf;
+  float integral = 0.0f;
+  explicit Dist1(const std::vector<float>& fn) : f(fn), cdf(fn.size()) {
+    float inv = 1.0f / (float)fn.size(), run = 0.0f;
+    for (size_t i = 0; i < fn.size(); ++i) { run += fn[i]; cdf[i] = run * inv; }
+    integral = cdf.back();
+    if (integral > 0.0f)
+      for (float& c : cdf) c = c / integral;
+  }
+  int sample(float u, float* pmf) const {
+    int i = cdf_find(cdf, u);
+    *pmf = f[i] / (integral * (float)f.size());
+    return i;
+  }
+  float reuse(float u, int i) const {
+    float lo = i > 0 ? cdf[i - 1] : 0.0f;
+    return (u - lo) / (cdf[i] - lo);
+  }
+};
+
+static std::vector<float> edge_lengths(const std::vector<EdgeRec>& E) {
+  std::vector<float> l(E.size());
+  for (size_t i = 0; i < E.size(); ++i) l[i] = length(E[i].e1);
+  return l;
+}
+
+// Camera::sampleDirect (Camera.cpp:367-384) with the in-front test.
+struct Sensor {
+  float qx, qy, value;
+  bool valid;
+  int pixel;
+};
+static Sensor sensor(const Cam& c, F3 p) {
+  Sensor r;
+  F3 d = p - c.eye;
+  F3 l{dot(d, c.right), dot(d, c.up), dot(d, c.dir)};
+  float sx = l.x / (l.z * c.ratio * c.tan_half);
+  float sy = l.y / (l.z * c.tan_half);
+  r.qx = (1.0f - sx) * 0.5f;
+  r.qy = (1.0f - sy) * 0.5f;
+  r.valid = l.z > 0.0f && r.qx >= 0.0f && r.qx < 1.0f && r.qy >= 0.0f && r.qy < 1.0f;
+  r.pixel = (int)floorf(r.qy * (float)c.H) * c.W + (int)floorf(r.qx * (float)c.W);
+  float d2 = len2(d);
+  F3 dn = d / sqrtf(d2);
+  float ct = dot(c.dir, dn);
+  float ic = 1.0f / ct;
+  r.value = 1.0f / d2 * ic * ic * ic * c.inv_area;
+  return r;
+}
+
+// BSDF::Eval with the shading-normal correction (BSDF.cpp:172-195), float.
+static F3 bsdf_eval_corr(const Scene& s, const Its<float>& it, F3 wo, F3 wi) {
+  Access<float> A(s);
+  F3 f = bsdf_eval(A, it, wo, wi);
+  F3 lo = to_local(it, wo), li = to_local(it, wi);
+  float corr = fabsf(lo.z * dot(it.ng, wi) / (li.z * dot(it.ng, wo)));
+  return f * corr;
+}
+
+// EvalImportance (Integrator.cpp:75-176) for one sensor-side vertex: the
+// importance contribution (pixel, value) and the next vertex.
+struct ImpStep {
+  bool has_contrib;
+  int pixel;
+  F3 value;
+  bool next;
+  Its<float> it;
+  F3 dir, thr;
+};
+static ImpStep importance_step(const Scene& s, const Cam& c, const Its<float>& it, F3 rdir, F3 thr, R4 rnd) {
+  ImpStep r;
+  r.has_contrib = false;
+  r.next = false;
+  r.value = F3{0, 0, 0};
+  F3 wo = -rdir;
+  Sensor sd = sensor(c, it.p);
+  if (sd.valid) {
+    F3 cd = camera_ray_dir(c, sd.qx * (float)c.W, sd.qy * (float)c.H);
+    float tmax = length(it.p - c.eye) - SHADOW_EPSILON_;
+    bool vis = intersect(s, c.eye, cd, SHADOW_EPSILON_, tmax, true, false).tri < 0;
+    if (vis) {
+      F3 t = sd.value * thr;
+      F3 bv{0, 0, 0};
+      if (dir_valid(s, it, wo)) {
+        F3 wi = normalize(c.eye - it.p);
+        float wiok = dir_valid(s, it, wi) ? 1.0f : 0.0f;
+        bv = bsdf_eval_corr(s, it, wo, wi) * fabsf(dot(it.ns, wi)) * wiok;
+      }
+      F3 v = t * bv;
+      if (finite3(v)) {
+        r.has_contrib = true;
+        r.pixel = sd.pixel;
+        r.value = v;
+      }
+    }
+  }
+  if (dir_valid(s, it, wo)) {
+    float pdf;
+    F3 wi = bsdf_sample(s, it, wo, rnd.a, rnd.b, &pdf);
+    F3 bv = bsdf_eval_corr(s, it, wo, wi) * fabsf(dot(it.ns, wi));
+    Hit h = intersect(s, it.p, wi, SHADOW_EPSILON_, 1e32f, false, false);
+    if (h.tri >= 0 && pdf > 0.0f && dir_valid(s, it, wi)) {
+      Access<float> A(s);
+      r.next = true;
+      r.it = post_intersect(A, h.tri, h.u, h.v);
+      r.dir = wi;
+      r.thr = thr * bv / pdf;
+    }
+  }
+  return r;
+}
+
+// RayIntersectAD (Utils.cpp:288-297): Moller-Trumbore u, v.
+template <class S>
+static void ray_intersect_ad(V3<S> dir, V3<S> org, V3<S> p0, V3<S> e1, V3<S> e2, S* u, S* v) {
+  V3<S> h = cross(dir, e2);
+  S f = S(1.0f) / dot(e1, h);
+  V3<S> s0 = org - p0;
+  *u = f * dot(s0, h);
+  V3<S> q = cross(s0, e1);
+  *v = f * dot(dir, q);
+}
+
+// xDotN = n . ((1-u-v) P0d + u P1d + v P2d) for the triangle `tri`,
+// (u, v) from the ray (org -> through) against the differentiable triangle.
+template <class S>
+static S x_dot_n(Access<S>& A, int tri, V3<S> org, V3<S> through, F3 n) {
+  const Scene& s = A.s;
+  V3<S> P0 = A.pos(s.idx[3 * tri]), P1 = A.pos(s.idx[3 * tri + 1]), P2 = A.pos(s.idx[3 * tri + 2]);
+  S u, v;
+  ray_intersect_ad(normalize(through - org), org, P0, P1 - P0, P2 - P0, &u, &v);
+  S w = S(1.0f) - u - v;
+  V3<S> u2 = (w * detach3(P0) + u * detach3(P1)) + v * detach3(P2);
+  return dot(cast3<S>(n), u2);
+}
+
+// baseVal of the primary / pixel boundary (BoundaryPrimary.cpp:117-129)
+static float base_val_camera(F3 itp, F3 ng, F3 raydir, F3 p_edge, F3 eye, F3 edge, F3 edge2, F3* n_out,
+                             bool revert) {
+  float dist = length(itp - eye);
+  float dist1 = length(p_edge - eye);
+  float cos2 = fabsf(dot(ng, -raydir));
+  F3 e = cross(edge, raydir);
+  float sinphi = length(e);
+  F3 proj = normalize(cross(e, ng));
+  float sinphi2 = length(cross(raydir, proj));
+  F3 n = normalize(cross(ng, proj));
+  bool sign0 = dot(e, edge2) > 0.0f, sign1 = dot(e, n) > 0.0f;
+  float bv = (dist / dist1) * (sinphi / sinphi2) * cos2;
+  bv = bv * (sinphi > EPSILON_ ? 1.0f : 0.0f) * (sinphi2 > EPSILON_ ? 1.0f : 0.0f);
+  float sg = (sign0 == sign1) ? 1.0f : -1.0f;
+  bv = bv * (revert ? -sg : sg);
+  *n_out = n;
+  return bv;
+}
+
+// baseVal of the secondary terms (Edge.cpp:245-257, BoundaryIndirect.cpp:100-112)
+static float base_val_secondary(F3 its_p, F3 far_p, F3 far_n, F3 dir_to_near, F3 p_edge, F3 edge, F3 edge2,
+                                F3* n_out) {
+  // dir_to_near: ray direction from the edge point toward the near-side
+  // surface (its_p); the far side is the emitter side.
+  float dist = length(far_p - its_p);
+  float cos2 = fabsf(dot(far_n, dir_to_near));
+  F3 e = cross(edge, -dir_to_near);
+  float sinphi = length(e);
+  F3 proj = normalize(cross(e, far_n));
+  float sinphi2 = length(cross(-dir_to_near, proj));
+  float itsT = length(its_p - p_edge);
+  F3 n = normalize(cross(far_n, proj));
+  bool sign0 = dot(e, edge2) > 0.0f, sign1 = dot(e, n) > 0.0f;
+  float bv = (itsT / dist) * (sinphi / sinphi2) * cos2;
+  bv = bv * (sinphi > EPSILON_ ? 1.0f : 0.0f) * (sinphi2 > EPSILON_ ? 1.0f : 0.0f);
+  bv = bv * ((sign0 == sign1) ? 1.0f : -1.0f);
+  *n_out = n;
+  return bv;
+}
+
+// Radiance along a camera boundary ray: EvalRadianceEmitted + bounces, all
+// detached, throughput = sensor importance (GenerateBoundaryRays).
+static F3 boundary_radiance(const Scene& s, const Its<float>& it, F3 raydir, float sensor_val, uint32_t seed,
+                            uint32_t domain, uint64_t index, int B) {
+  Access<float> A(s);
+  F3 beta{sensor_val, sensor_val, sensor_val};
+  F3 rad{0, 0, 0};
+  if (it.emitter >= 0) {
+    F3 e = beta * emitted(s, it.emitter, it.ng, -raydir);
+    if (finite3(e)) rad = rad + e;
+  }
+  bounces(A, it, beta, raydir, seed, domain, index, 1u, B, rad, nullptr);
+  return rad;
+}
+
+// One boundary sample: weight seeds on the tape.  Returns the scalar weight
+// w = sum over splats of <dLdI[pixel], K> / spp and the xDotN Var (or sets
+// *valid = false).
+struct BSample {
+  bool valid = false;
+  int tri = -1;        // triangle of the RayIntersectAD
+  F3 org, through, n;  // values (org / through may be differentiable, see kind)
+  double w = 0.0;
+  // differentiable org / through: edge (v0, v1, t) and / or material point (tri, u, v)
+  bool through_edge = false;
+  uint32_t ev0 = 0, ev1 = 0;
+  float et = 0.0f;
+  bool org_material = false;
+  int otri = -1;
+  float ou = 0.0f, ov = 0.0f;
+};
+
+static BSample primary_sample(const Scene& s, const Cam& c, const std::vector<EdgeRec>& E, const Dist1& D,
+                              uint32_t seed, int B, int spp, uint64_t i, const float* dLdI) {
+  BSample r;
+  R4 rnd = rnd4(seed, kDomPrimaryEdge, i, 0u);
+  float pmf;
+  int ei = D.sample(rnd.a, &pmf);
+  const EdgeRec& e = E[ei];
+  float pdf = pmf / D.f[ei];
+  F3 p = e.p0 + rnd.b * e.e1;
+  F3 edge = normalize(e.e1);
+  F3 edge2 = normalize(e.p2 - e.p0);
+  F3 dir = normalize(p - c.eye);
+  Hit h = intersect(s, p, dir, SHADOW_EPSILON_, 1e32f, false, false);
+  if (h.tri < 0) return r;
+  F3 hitP = p + h.t * dir;
+  Sensor sd = sensor(c, hitP);
+  if (!sd.valid) return r;
+  F3 rd = camera_ray_dir(c, sd.qx * (float)c.W, sd.qy * (float)c.H);
+  float tmax = length(p - c.eye) - SHADOW_EPSILON_;
+  if (intersect(s, c.eye, rd, SHADOW_EPSILON_, tmax, true, false).tri >= 0) return r;
+  Access<float> A(s);
+  Its<float> it = post_intersect(A, h.tri, h.u, h.v);
+  F3 n;
+  float bv = base_val_camera(it.p, it.ng, rd, p, c.eye, edge, edge2, &n, false);
+  F3 rad = boundary_radiance(s, it, rd, sd.value, seed, kDomPrimaryEdge, i, B);
+  F3 K = rad * bv / pdf;
+  if (!(length(K) > 0.0f) || !finite3(K)) return r;
+  const uint64_t WH = (uint64_t)c.W * c.H;
+  r.w = ((double)dLdI[sd.pixel] * K.x + (double)dLdI[WH + sd.pixel] * K.y + (double)dLdI[2 * WH + sd.pixel] * K.z) /
+        (double)spp;
+  r.valid = true;
+  r.tri = h.tri;
+  r.org = c.eye;
+  r.through = p;
+  r.n = n;
+  r.through_edge = true;
+  r.ev0 = e.v0;
+  r.ev1 = e.v1;
+  r.et = rnd.b;
+  return r;
+}
+
+static BSample pixel_sample(const Scene& s, const Cam& c, uint32_t seed, int B, int spp, uint64_t i, uint64_t N,
+                            const float* dLdI) {
+  BSample r;
+  const uint64_t per_anti = N / 4;
+  const uint64_t j = i % per_anti;
+  const uint64_t anti = i / per_anti;
+  float rr = rnd4(seed, kDomPixelEdge, j, 0u).a;
+  float rv;
+  if (anti == 0) rv = rr;
+  else if (anti == 1) rv = rr < 0.5f ? rr + 0.5f : rr - 0.5f;
+  else if (anti == 2) rv = rr < 0.75f ? 0.75f - rr : 1.75f - rr;
+  else rv = rr < 0.25f ? 0.25f - rr : 1.25f - rr;
+  float r4 = rv * 4.0f;
+  int side = (int)floorf(r4);
+  float ox, oy;
+  if (side == 0) { ox = 0.0f; oy = r4; }
+  else if (side == 1) { ox = r4 - 1.0f; oy = 1.0f; }
+  else if (side == 2) { ox = 1.0f; oy = 3.0f - r4; }
+  else { ox = 4.0f - r4; oy = 0.0f; }
+  const uint64_t WH = (uint64_t)c.W * c.H;
+  const int pix = (int)(i % WH);
+  float rx = (float)(pix % c.W) + ox, ry = (float)(pix / c.W) + oy;
+  // RasterToWorld: camera point (ax rx + bx, ay ry + by, 1)
+  F3 p0 = ((c.eye + (c.ax * rx + c.bx) * c.right) + (c.ay * ry + c.by) * c.up) + c.dir;
+  // edge / visible directions of the pixel side (raster vectors through the
+  // linear part of RasterToWorld)
+  float plx = 2.0f * c.ratio * c.tan_half, ply = 2.0f * c.tan_half;
+  F3 vdir = normalize(c.ay * c.up), hdir = normalize(c.ax * c.right);
+  F3 edge = (side == 0 || side == 2) ? vdir : hdir;
+  F3 edge2 = side == 0 ? hdir : (side == 1 ? -vdir : (side == 2 ? -hdir : vdir));
+  float pdf = (side == 0 || side == 2) ? 0.25f / (ply * (float)c.W) : 0.25f / (plx * (float)c.H);
+  F3 dir = normalize(p0 - c.eye);
+  Hit h = intersect(s, p0, dir, SHADOW_EPSILON_, 1e32f, false, false);
+  if (h.tri < 0) return r;
+  F3 hitP = p0 + h.t * dir;
+  Sensor sd = sensor(c, hitP);
+  F3 rd = camera_ray_dir(c, sd.qx * (float)c.W, sd.qy * (float)c.H);
+  Access<float> A(s);
+  Its<float> it = post_intersect(A, h.tri, h.u, h.v);
+  F3 n;
+  float bv = base_val_camera(it.p, it.ng, rd, p0, c.eye, edge, edge2, &n, true);
+  F3 rad = boundary_radiance(s, it, rd, sd.value, seed, kDomPixelEdge, j, B);
+  F3 K = rad * bv / pdf;
+  if (!finite3(K)) return r;
+  r.w = ((double)dLdI[pix] * K.x + (double)dLdI[WH + pix] * K.y + (double)dLdI[2 * WH + pix] * K.z) / (double)spp;
+  r.valid = r.w != 0.0;
+  r.tri = h.tri;
+  r.org = c.eye;
+  r.through = p0;
+  r.n = n;
+  return r;
+}
+
+static BSample direct_sample(const Scene& s, const Cam& c, const std::vector<EdgeRec>& E, const Dist1& D,
+                             uint32_t seed, int B, int spp, uint64_t i, const float* dLdI) {
+  BSample r;
+  R4 rnd = rnd4(seed, kDomDirectEdge, i, 0u);
+  float pmf;
+  int ei = D.sample(rnd.a, &pmf);
+  const EdgeRec& e = E[ei];
+  float pdf_e = pmf / D.f[ei];
+  float t = D.reuse(rnd.a, ei);
+  F3 p = e.p0 + t * e.e1;
+  F3 edge = normalize(e.e1);
+  F3 edge2 = normalize(e.p2 - e.p0);
+  Access<float> A(s);
+  LightS<float> ls = light_sample(A, rnd.b, rnd.c);
+  F3 ev = ls.p - p;
+  float d2 = len2(ev), dist = sqrtf(d2);
+  F3 en = ev / dist;
+  float cosT = dot(ls.n, -en);
+  float c0 = dot(en, e.n0), c1 = dot(en, e.n1);
+  bool valid0 = fabsf(c0) > EDGE_EPSILON_;
+  bool valid1 = (c0 > EDGE_EPSILON_ && c1 < -EDGE_EPSILON_) || (c0 < -EDGE_EPSILON_ && c1 > EDGE_EPSILON_);
+  bool right = (e.boundary && valid0) || (!e.boundary && valid1);
+  if (!(right && cosT > EPSILON_ && dist > SHADOW_EPSILON_)) return r;
+  float pdf = pdf_e * ls.pdf * d2 / cosT * 1.0f;
+  // EvalBoundarySegmentDirect (Edge.cpp:201-309)
+  F3 rdir = normalize(p - ls.p);
+  Hit h0 = intersect(s, p, -rdir, SHADOW_EPSILON_, 1e32f, false, false);
+  Hit h = intersect(s, p, rdir, SHADOW_EPSILON_, 1e32f, false, false);
+  if (h0.tri < 0 || h.tri < 0) return r;
+  F3 hitP = p + h0.t * (-rdir);
+  if (!(length(hitP - ls.p) < SHADOW_EPSILON_)) return r;
+  F3 Le{0, 0, 0};
+  if (s.tri_emitter[h0.tri] >= 0) Le = emitted(s, s.tri_emitter[h0.tri], s.fn[h0.tri], rdir);
+  Its<float> it = post_intersect(A, h.tri, h.u, h.v);
+  F3 n;
+  float bv = base_val_secondary(it.p, ls.p, ls.n, rdir, p, edge, edge2, &n);
+  F3 K0 = Le * bv / pdf;
+  // sensor side (EvalImportance for B bounces, throughput 1)
+  const uint64_t WH = (uint64_t)c.W * c.H;
+  double w = 0.0;
+  Its<float> cur = it;
+  F3 cdir = rdir, thr{1, 1, 1};
+  for (int b = 0; b < B; ++b) {
+    ImpStep st = importance_step(s, c, cur, cdir, thr, rnd4(seed, kDomDirectEdge, i, kSensorBlock0 + (uint32_t)b));
+    if (st.has_contrib) {
+      F3 v = K0 * st.value;
+      if (length(v) > 0.0f)
+        w += (double)dLdI[st.pixel] * v.x + (double)dLdI[WH + st.pixel] * v.y + (double)dLdI[2 * WH + st.pixel] * v.z;
+    }
+    if (!st.next) break;
+    cur = st.it;
+    cdir = st.dir;
+    thr = st.thr;
+  }
+  if (w == 0.0 || !std::isfinite(w)) return r;
+  r.valid = true;
+  r.w = w / (double)spp;
+  r.tri = h0.tri;
+  r.org = it.p;
+  r.through = p;
+  r.n = n;
+  r.through_edge = true;
+  r.ev0 = e.v0;
+  r.ev1 = e.v1;
+  r.et = t;
+  r.org_material = true;
+  r.otri = h.tri;
+  r.ou = h.u;
+  r.ov = h.v;
+  return r;
+}
+
+static BSample indirect_sample(const Scene& s, const Cam& c, const std::vector<EdgeRec>& E, const Dist1& D,
+                               uint32_t seed, int B, int spp, uint64_t i, const float* dLdI) {
+  BSample r;
+  R4 rnd = rnd4(seed, kDomIndirectEdge, i, 0u);
+  float pmf;
+  int ei = D.sample(rnd.a, &pmf);
+  const EdgeRec& e = E[ei];
+  float pdf_e = pmf / D.f[ei];
+  float t = D.reuse(rnd.a, ei);
+  F3 p = e.p0 + t * e.e1;
+  F3 edge = normalize(e.e1);
+  F3 edge2 = normalize(e.p2 - e.p0);
+  // SampleBoundarySegmentIndirect (Edge.cpp:146-199)
+  F3 dir;
+  float pdf_d;
+  if (e.boundary) {
+    F3 nrm = cross(e.n0, edge);
+    Its<float> fr;
+    coordinate_system(nrm, &fr.t, &fr.b);
+    float z = std::min(1.0f - 2.0f * rnd.b, 1.0f - EDGE_EPSILON_);
+    float rr = sqrtf(std::max(0.0f, 1.0f - z * z));
+    float sp, cp;
+    f_sincos(2.0f * PI * rnd.c, &sp, &cp);
+    dir = ((rr * cp) * fr.t + (rr * sp) * fr.b) + z * nrm;
+    pdf_d = 0.25f / PI;
+  } else {
+    float cosd = dot(e.n0, e.n1);
+    float phi0 = acosf(std::min(1.0f, std::max(-1.0f, cosd)));
+    pdf_d = 0.25f / phi0;
+    F3 z = normalize(e.n0 + e.n1);
+    F3 y = normalize(cross(e.n0, z));
+    F3 x = cross(y, z);
+    float phi = (rnd.b - 0.5f) * phi0;
+    phi = std::min(std::max(phi, -0.5f * phi0 + EDGE_EPSILON_), 0.5f * phi0 - EDGE_EPSILON_);
+    float sp, cp;
+    f_sincos(phi, &sp, &cp);
+    F3 x1 = x * cp + z * sp;
+    bool m0 = rnd.c > 0.5f;
+    float bb = 4.0f * rnd.c - (m0 ? 3.0f : 1.0f);
+    float aa = sqrtf(1.0f - bb * bb) * (m0 ? -1.0f : 1.0f);
+    dir = x1 * aa + y * bb;
+  }
+  float pdf = pdf_e * pdf_d;
+  Hit he = intersect(s, p, -dir, SHADOW_EPSILON_, 1e32f, false, false);
+  Hit hs = intersect(s, p, dir, SHADOW_EPSILON_, 1e32f, false, false);
+  if (he.tri < 0 || hs.tri < 0) return r;
+  Access<float> A(s);
+  Its<float> ite = post_intersect(A, he.tri, he.u, he.v);
+  Its<float> its = post_intersect(A, hs.tri, hs.u, hs.v);
+  F3 n;
+  float bv = base_val_secondary(its.p, ite.p, ite.ng, dir, p, edge, edge2, &n);
+  float v0 = bv / pdf;
+  // emitter side: cumulative radiance after 1..B-1 bounces from ite
+  std::vector<F3> per(std::max(B - 1, 1), F3{0, 0, 0});
+  F3 acc{0, 0, 0};
+  if (B > 1) bounces(A, ite, F3{1, 1, 1}, -dir, seed, kDomIndirectEdge, i, 1u, B - 1, acc, per.data());
+  std::vector<F3> cum(std::max(B - 1, 1));
+  F3 run{0, 0, 0};
+  for (int k = 0; k < B - 1; ++k) { run = run + per[k]; cum[k] = run; }
+  // sensor side
+  const uint64_t WH = (uint64_t)c.W * c.H;
+  double w = 0.0;
+  Its<float> cur = its;
+  F3 cdir = dir, thr{1, 1, 1};
+  for (int b = 0; b < B - 1; ++b) {
+    ImpStep st = importance_step(s, c, cur, cdir, thr, rnd4(seed, kDomIndirectEdge, i, kSensorBlock0 + (uint32_t)b));
+    int ri = B - b - 2;
+    if (st.has_contrib && ri >= 0 && ri < B - 1) {
+      F3 v = (v0 * cum[ri]) * st.value;
+      if (length(v) > 0.0f)
+        w += (double)dLdI[st.pixel] * v.x + (double)dLdI[WH + st.pixel] * v.y + (double)dLdI[2 * WH + st.pixel] * v.z;
+    }
+    if (!st.next) break;
+    cur = st.it;
+    cdir = st.dir;
+    thr = st.thr;
+  }
+  if (w == 0.0 || !std::isfinite(w)) return r;
+  r.valid = true;
+  r.w = w / (double)spp;
+  r.tri = he.tri;
+  r.org = its.p;
+  r.through = p;
+  r.n = n;
+  r.through_edge = true;
+  r.ev0 = e.v0;
+  r.ev1 = e.v1;
+  r.et = t;
+  r.org_material = true;
+  r.otri = hs.tri;
+  r.ou = hs.u;
+  r.ov = hs.v;
+  return r;
+}
+
+// Backpropagate one boundary sample's weight through xDotN into dpos.
+static void boundary_backprop(const Scene& s, const BSample& b, Tape& tape, double* dpos) {
+  tape.clear();
+  Access<Var> A(s);
+  V3<Var> through = cast3<Var>(b.through);
+  if (b.through_edge) {
+    V3<Var> q0 = A.pos(b.ev0), q1 = A.pos(b.ev1);
+    through = q0 + Var(b.et) * (q1 - q0);
+  }
+  V3<Var> org = cast3<Var>(b.org);
+  if (b.org_material) {
+    float w = 1.0f - b.ou - b.ov;
+    org = (Var(w) * A.pos(s.idx[3 * b.otri]) + Var(b.ou) * A.pos(s.idx[3 * b.otri + 1])) +
+          Var(b.ov) * A.pos(s.idx[3 * b.otri + 2]);
+  }
+  Var x = x_dot_n(A, b.tri, org, through, b.n);
+  if (x.i < 0) return;
+  size_t nn = tape.nodes.size();
+  tape.adj.assign(nn, 0.0);
+  tape.adj[x.i] = b.w;
+  for (size_t k = nn; k-- > 0;) {
+    double a = tape.adj[k];
+    if (a == 0.0) continue;
+    const Node& nd = tape.nodes[k];
+    if (nd.a >= 0) tape.adj[nd.a] += a * nd.da;
+    if (nd.b >= 0) tape.adj[nd.b] += a * nd.db;
+  }
+  for (auto& kv : A.vcache) {
+    double* d = &dpos[3 * (size_t)kv.first];
+    d[0] += tape.adj[kv.second.x.i];
+    d[1] += tape.adj[kv.second.y.i];
+    d[2] += tape.adj[kv.second.z.i];
+  }
 }
 
 static void parallel_for(uint64_t n, int threads, const std::function<void(uint64_t, uint64_t, int)>& f) {
@@ -1205,6 +1817,40 @@ void oracle_path_twin(void* sp, const float* cam, int w, int h, uint32_t seed, i
     out[3 * sgn + 1] = r.y.s;
     out[3 * sgn + 2] = r.z.s;
   }
+}
+
+// Boundary gradients: kind 0 primary edges, 1 pixel boundary, 2 secondary
+// direct, 3 secondary indirect.  dpos [V][3] += d/dV of the boundary terms
+// for samples [first, first + count) of spp * W * H.
+void oracle_boundary_adj(void* sp, const float* cam, int w, int h, uint32_t seed, int max_bounces, int kind, int spp,
+                         const float* dLdI, uint64_t first, uint64_t count, double* dpos, int threads) {
+  const Scene& s = *(Scene*)sp;
+  Cam c = make_camera(cam, cam + 3, cam + 6, w, h, cam[9]);
+  std::vector<EdgeRec> E;
+  if (kind == 0) E = prim_edges(s, c);
+  else if (kind >= 2) E = sec_edges(s);
+  if (kind != 1 && E.empty()) return;
+  std::unique_ptr<Dist1> D;
+  if (kind != 1) D.reset(new Dist1(edge_lengths(E)));
+  const uint64_t N = (uint64_t)spp * w * h;
+  int nt = std::max(1, threads);
+  std::vector<std::vector<double>> tp(nt, std::vector<double>(3 * (size_t)s.V, 0.0));
+  parallel_for(count, threads, [&](uint64_t a, uint64_t b, int tid) {
+    Tape tape;
+    g_tape = &tape;
+    for (uint64_t k = a; k < b; ++k) {
+      uint64_t i = first + k;
+      BSample bs;
+      if (kind == 0) bs = primary_sample(s, c, E, *D, seed, max_bounces, spp, i, dLdI);
+      else if (kind == 1) bs = pixel_sample(s, c, seed, max_bounces, spp, i, N, dLdI);
+      else if (kind == 2) bs = direct_sample(s, c, E, *D, seed, max_bounces, spp, i, dLdI);
+      else bs = indirect_sample(s, c, E, *D, seed, max_bounces, spp, i, dLdI);
+      if (bs.valid) boundary_backprop(s, bs, tape, tp[tid].data());
+    }
+    g_tape = nullptr;
+  });
+  for (int t = 0; t < nt; ++t)
+    for (size_t k = 0; k < tp[t].size(); ++k) dpos[k] += tp[t][k];
 }
 
 // Reverse mode: dpos [V][3] += d/dV sum_paths <dLdI[pix], path(pix)/spp>,

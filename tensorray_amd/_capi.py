@@ -71,6 +71,8 @@ SIGNATURES = [
     ("trc_film_from_paths", ctypes.c_int32, [vp, vp, ctypes.c_int32, ctypes.c_int32, vp, vp]),
     ("trc_render_stats", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options),
                                           ctypes.POINTER(ctypes.c_uint64), vp]),
+    ("trc_render_boundary", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options),
+                                             ctypes.c_int32, vp, vp, vp]),
     ("trc_render_adj", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options), vp, vp, vp,
                                         vp]),
 ]
@@ -265,6 +267,13 @@ class DeviceScene:
     def render_adj(self, cam, opt, dldi_ptr, dpos_ptr, dmat_ptr=None, stream=None):
         self.ctx.check(self.ctx.lib.trc_render_adj(self.h, ctypes.byref(cam), ctypes.byref(opt), dldi_ptr, dpos_ptr,
                                                    dmat_ptr, _stream_ptr(stream)), "trc_render_adj")
+
+    BOUNDARY_KINDS = {"primary": 0, "pixel": 1, "direct": 2, "indirect": 3}
+
+    def render_boundary(self, cam, opt, kind, dldi_ptr, dpos_ptr, stream=None):
+        k = self.BOUNDARY_KINDS[kind] if isinstance(kind, str) else int(kind)
+        self.ctx.check(self.ctx.lib.trc_render_boundary(self.h, ctypes.byref(cam), ctypes.byref(opt), k, dldi_ptr,
+                                                        dpos_ptr, _stream_ptr(stream)), "trc_render_boundary")
 
     def close(self):
         if self.h is not None and self.h.value:

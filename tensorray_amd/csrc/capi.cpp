@@ -91,6 +91,8 @@ void reupload(const T* dst, const std::vector<T>& v, hipStream_t st) {
 void free_scene_bufs(trc_scene* s) {
   for (auto& b : s->bufs) hipFree(b.ptr);
   s->bufs.clear();
+  if (s->prim_buf.ptr) hipFree(s->prim_buf.ptr);
+  s->prim_buf = DevBuf{};
   s->device_bytes = 0;
   s->dev_texs = nullptr;
 }
@@ -121,6 +123,11 @@ void upload_scene(trc_scene* s, hipStream_t st) {
   d.light_cdf = upload(s, h.light_cdf, st);
   d.light_f = upload(s, h.light_f, st);
   d.emit_rad = upload(s, h.emit_rad, st);
+  d.sec.e = upload(s, h.sec_edges, st);
+  d.sec.len = upload(s, h.sec_len, st);
+  d.sec.cdf = upload(s, h.sec_cdf, st);
+  d.sec.n = (int32_t)h.sec_edges.size();
+  d.sec.integral = h.sec_integral;
   d.n_light = (int32_t)h.light_tri.size();
   d.light_integral = h.light_integral;
   d.n_tris = h.num_triangles();
@@ -135,6 +142,14 @@ CameraDev to_cam(const trc_camera* c) {
   if (c->width <= 0 || c->height <= 0) throw Fail(TRC_ERR_INVALID, "camera size must be positive");
   if (!(c->fov > 0.0f && c->fov < 180.0f)) throw Fail(TRC_ERR_INVALID, "camera fov out of range");
   return make_camera(c->pos, c->target, c->up, c->width, c->height, c->fov);
+}
+
+void check_render_common(const trc_scene* s, const trc_options* o) {
+  if (!s->configured) throw Fail(TRC_ERR_STATE, "scene is not configured");
+  if (!o) throw Fail(TRC_ERR_INVALID, "options are NULL");
+  if (o->max_bounces < 0 || o->max_bounces > kMaxBounces)
+    throw Fail(TRC_ERR_INVALID, "max_bounces must be in [0, " + std::to_string(kMaxBounces) + "]");
+  if (s->dev.n_light <= 0) throw Fail(TRC_ERR_INVALID, "scene has no area emitter");
 }
 
 void check_render(const trc_scene* s, const trc_options* o) {
@@ -310,9 +325,18 @@ trc_status trc_scene_set_vertices(trc_scene* s, const float* pos, int32_t nv, in
     std::vector<float> tmp(3 * (size_t)nv);
     hip_check(hipMemcpyAsync(tmp.data(), pos, tmp.size() * 4, hipMemcpyDefault, st), "hipMemcpyAsync");
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    const size_t n_sec_before = s->host.sec_edges.size();
     s->host.update_positions(tmp.data());
     SceneDev& d = s->dev;
-    if (refit) {
+    if (refit && s->host.sec_edges.size() != n_sec_before) {
+      // the secondary edge list changed size: re-upload everything
+      s->host.refit_bvh();
+      upload_scene(s, st);
+    } else if (refit) {
+      reupload(d.sec.e, s->host.sec_edges, st);
+      reupload(d.sec.len, s->host.sec_len, st);
+      reupload(d.sec.cdf, s->host.sec_cdf, st);
+      d.sec.integral = s->host.sec_integral;
       s->host.refit_bvh();
       reupload(d.pos, s->host.pos, st);
       reupload(d.vnrm, s->host.vnrm, st);
@@ -465,6 +489,52 @@ trc_status trc_render_stats(trc_scene* s, const trc_camera* cam, const trc_optio
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     hipFree(d);
     hip_check(e, "k_render_stats");
+  });
+}
+
+trc_status trc_render_boundary(trc_scene* s, const trc_camera* cam, const trc_options* opt, int32_t kind,
+                               const float* dLdI, float* dpos, trc_stream stream) {
+  if (!s) return TRC_ERR_INVALID;
+  return guarded(s->ctx, [&] {
+    check_render_common(s, opt);
+    CameraDev C = to_cam(cam);
+    if (!dLdI || !dpos) throw Fail(TRC_ERR_INVALID, "dLdI / dpos is NULL");
+    if (kind < 0 || kind > 3) throw Fail(TRC_ERR_INVALID, "unknown boundary kind");
+    const int spp = kind == 0 ? opt->sppe : kind == 1 ? opt->sppe0 : kind == 2 ? opt->sppse0 : opt->sppse1;
+    if (spp < 0) throw Fail(TRC_ERR_INVALID, "negative boundary spp");
+    if (spp == 0) return;
+    const long long N = (long long)spp * C.width * C.height;
+    if (kind == 1 && N % 4 != 0) throw Fail(TRC_ERR_INVALID, "pixel-boundary samples (sppe0*W*H) must be a multiple of 4");
+    hipStream_t st = (hipStream_t)stream;
+    EdgeSet prim{};
+    if (kind == 0) {
+      select_prim_edges(s->host, C.eye, s->prim_edges);
+      if (s->prim_edges.empty()) return;
+      prim.integral = edge_distribution(s->host, s->prim_edges, s->prim_len, s->prim_cdf);
+      prim.n = (int32_t)s->prim_edges.size();
+      const size_t eb = s->prim_edges.size() * sizeof(EdgeDev), fb = s->prim_edges.size() * sizeof(float);
+      const size_t need = eb + 2 * fb;
+      if (s->prim_buf.bytes < need) {
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        if (s->prim_buf.ptr) hipFree(s->prim_buf.ptr);
+        s->prim_buf.ptr = nullptr;
+        s->prim_buf.bytes = 0;
+        hip_check(hipMalloc(&s->prim_buf.ptr, need), "hipMalloc(primary edges)");
+        s->prim_buf.bytes = need;
+      }
+      char* base = (char*)s->prim_buf.ptr;
+      hip_check(hipMemcpyAsync(base, s->prim_edges.data(), eb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+      hip_check(hipMemcpyAsync(base + eb, s->prim_len.data(), fb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+      hip_check(hipMemcpyAsync(base + eb + fb, s->prim_cdf.data(), fb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+      prim.e = (const EdgeDev*)base;
+      prim.len = (const float*)(base + eb);
+      prim.cdf = (const float*)(base + eb + fb);
+    }
+    if (kind >= 2 && s->dev.sec.n == 0) return;
+    LaunchCfg L = make_cfg(s->ctx, s, st, N);
+    hip_check(launch_boundary(kind, s->dev, C, opt->seed, opt->max_bounces, spp, L, prim, dLdI, dpos, st),
+              "k_boundary");
+    if (kind == 0) hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
   });
 }
 

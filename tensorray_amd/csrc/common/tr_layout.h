@@ -92,6 +92,25 @@ struct alignas(16) Material {
   float pad4;
 };
 
+// Mesh edge for the boundary terms (ObjMesh::processEdges semantics): end
+// vertices v0 < v1, the opposite vertex v2 of face f0, the second face f1
+// (-1 on a boundary edge).  Geometry is gathered from `pos` / face normals.
+struct alignas(16) EdgeDev {
+  uint32_t v0, v1, v2;
+  int32_t f0, f1;
+  uint32_t pad[3];
+};
+
+// A sampled edge list with its length distribution (Distribution1D over
+// |p1 - p0|, Edge.cpp:90-106 / 350-367).
+struct EdgeSet {
+  const EdgeDev* e;
+  const float* len;
+  const float* cdf;
+  int32_t n;
+  float integral;
+};
+
 // Texture descriptor: FP32 RGB texels, row-major [h][w][3], nearest lookup
 // (ImageTexture::Eval, BSDF.cpp:72-98).
 struct TexDesc {
@@ -130,6 +149,7 @@ struct SceneDev {
   const float* light_cdf;     // normalized CDF over emitter triangles
   const float* light_f;       // luminance * area per emitter triangle
   const float* emit_rad;      // [n_emitters][3]
+  EdgeSet sec;                // secondary edges (ConstructSecEdgeList, Edge.cpp:42-88)
   int32_t n_light;
   float light_integral;       // sum(f) / n_light (Distribution1D::mIntegralVal)
   int32_t n_tris;

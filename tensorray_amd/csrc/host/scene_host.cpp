@@ -175,6 +175,61 @@ void SceneHost::update_positions(const float* new_pos) {
     if (light_integral > 0.0f)
       for (int32_t i = 0; i < NL; ++i) light_cdf[i] = light_cdf[i] / light_integral;
   }
+  // secondary edges depend on the face normals
+  select_sec_edges(*this, sec_edges);
+  sec_integral = edge_distribution(*this, sec_edges, sec_len, sec_cdf);
+}
+
+namespace {
+inline F3 face_n(const SceneHost& s, int32_t f) {
+  if (f < 0) return F3{0.0f, 0.0f, 0.0f};
+  return F3{s.shade[f].n[0], s.shade[f].n[1], s.shade[f].n[2]};
+}
+}  // namespace
+
+void select_sec_edges(const SceneHost& s, std::vector<EdgeDev>& out) {
+  out.clear();
+  for (const EdgeHost& e : s.edges) {
+    bool keep = e.f1 < 0;
+    if (!keep) {
+      F3 n0 = face_n(s, e.f0), n1 = face_n(s, e.f1);
+      F3 p0 = vget(s.pos, e.v0), p2 = vget(s.pos, e.v2);
+      bool noncoplanar = dot(n0, n1) < 1.0f - 1e-6f;
+      bool convex = dot(n1, normalize(p2 - p0)) < -kEdgeEpsilon;
+      keep = noncoplanar && convex;
+    }
+    if (keep) out.push_back(EdgeDev{e.v0, e.v1, e.v2, e.f0, e.f1, {0, 0, 0}});
+  }
+}
+
+void select_prim_edges(const SceneHost& s, const float eye[3], std::vector<EdgeDev>& out) {
+  out.clear();
+  const F3 c{eye[0], eye[1], eye[2]};
+  for (const EdgeHost& e : s.edges) {
+    F3 d = c - vget(s.pos, e.v0);
+    bool ff0 = dot(d, face_n(s, e.f0)) > 0.0f;
+    bool ff1 = dot(d, face_n(s, e.f1)) > 0.0f;
+    if (e.f1 < 0 || ff0 != ff1) out.push_back(EdgeDev{e.v0, e.v1, e.v2, e.f0, e.f1, {0, 0, 0}});
+  }
+}
+
+float edge_distribution(const SceneHost& s, const std::vector<EdgeDev>& e, std::vector<float>& len,
+                        std::vector<float>& cdf) {
+  const size_t n = e.size();
+  len.resize(n);
+  cdf.resize(n);
+  if (n == 0) return 0.0f;
+  for (size_t i = 0; i < n; ++i) len[i] = length(vget(s.pos, e[i].v1) - vget(s.pos, e[i].v0));
+  const float inv = 1.0f / (float)n;
+  float run = 0.0f;
+  for (size_t i = 0; i < n; ++i) {
+    run += len[i];
+    cdf[i] = run * inv;
+  }
+  float integral = cdf[n - 1];
+  if (integral > 0.0f)
+    for (size_t i = 0; i < n; ++i) cdf[i] = cdf[i] / integral;
+  return integral;
 }
 
 void SceneHost::build_bvh() { bvh_build(pos, idx, nodes, tris, tri_order, bvh_stats); }

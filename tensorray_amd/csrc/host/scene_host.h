@@ -67,6 +67,10 @@ struct SceneHost {
   // edges (global vertex / triangle ids)
   std::vector<EdgeHost> edges;
   std::vector<uint32_t> mesh_edge_offset;
+  // secondary (scene-wide) edge list and its length distribution
+  std::vector<EdgeDev> sec_edges;
+  std::vector<float> sec_len, sec_cdf;
+  float sec_integral = 0.0f;
   // BVH
   std::vector<Bvh4Node> nodes;
   std::vector<TriRecord> tris;
@@ -90,6 +94,15 @@ struct SceneHost {
 // Camera reduction (pos, target, up, w, h, fov in degrees).
 CameraDev make_camera(const float pos[3], const float target[3], const float up[3], int w, int h,
                       float fov_deg);
+
+// Edge filters of the boundary terms.
+// Secondary: boundary edges, or non-coplanar convex ones (Edge.cpp:42-88).
+void select_sec_edges(const SceneHost& s, std::vector<EdgeDev>& out);
+// Primary: boundary edges or silhouettes seen from `eye` (Edge.cpp:313-348).
+void select_prim_edges(const SceneHost& s, const float eye[3], std::vector<EdgeDev>& out);
+// Distribution1D over edge lengths: len, normalized cdf, integral.
+float edge_distribution(const SceneHost& s, const std::vector<EdgeDev>& e, std::vector<float>& len,
+                        std::vector<float>& cdf);
 
 // Mesh edge list with the reference's ordering (std::map keyed by (min v,
 // max v), lexicographic).  Returns edges with local vertex / face ids.

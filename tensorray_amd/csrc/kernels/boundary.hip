@@ -1,0 +1,534 @@
+// Boundary (edge) terms of the differentiable integrator, one sample per
+// thread iteration.  Each term has zero value and the gradient
+//
+//   d/dtheta [ detach(K) * xDotN ],   xDotN = n . u2,
+//   u2 = (1-u-v) P0d + u P1d + v P2d, (u, v) = RayIntersectAD(dir, org, P0, P1-P0, P2-P0)
+//
+// where K carries the detached radiance / importance / baseVal / pdf of the
+// sample and (dir, org, P) may all depend on vertex positions.  The weight
+// w = <dL/dI[pixel], K> / spp is folded in before the VJP, so each sample
+// scatters a handful of vertex-gradient atomics.
+//
+// Reference: BoundaryPrimary.cpp:36-181, BoundaryPixel.cpp:36-159,
+// BoundaryDirect.cpp:38-143, BoundaryIndirect.cpp:38-192, Edge.cpp:90-409,
+// Camera.cpp:345-467, EvalImportance (Integrator.cpp:75-176).
+#include <hip/hip_runtime.h>
+
+#include "../trc_internal.h"
+#include "adjoint.h"
+
+namespace trc {
+
+constexpr uint32_t kSensorBlock0 = 64u;
+
+__device__ inline TraceCtx make_trace_ctx_b(uint32_t* lds, const LaunchCfg& L) {
+  TraceCtx t;
+  t.lds = lds + threadIdx.x;
+  t.lds_stride = kBlock;
+  t.lds_depth = kStackLds;
+  const int gtid = blockIdx.x * kBlock + threadIdx.x;
+  t.spill = L.spill + gtid;
+  t.spill_stride = L.grid * kBlock;
+  t.cap = L.cap;
+  t.err = L.err;
+  t.n_nodes = t.n_tris = t.n_queries = 0u;
+  return t;
+}
+
+__device__ inline F3 cam_eye(const CameraDev& c) { return F3{c.eye[0], c.eye[1], c.eye[2]}; }
+
+// Edge sampling with the length distribution; returns the edge index and
+// the pdf per unit length.
+__device__ inline int32_t sample_edge(const EdgeSet& E, float u, float* pdf) {
+  int32_t i = cdf_search(E.cdf, E.n, u);
+  float pmf = E.len[i] / (E.integral * (float)E.n);
+  *pdf = pmf / E.len[i];
+  return i;
+}
+
+struct EdgeGeo {
+  F3 p0, e1, n0, n1, p2;
+  bool boundary;
+};
+__device__ inline EdgeGeo edge_geo(const SceneDev& S, const EdgeDev& e) {
+  EdgeGeo g;
+  g.p0 = load3(S.pos, e.v0);
+  g.e1 = load3(S.pos, e.v1) - g.p0;
+  g.p2 = load3(S.pos, e.v2);
+  const TriShade& s0 = S.shade[e.f0];
+  g.n0 = F3{s0.n[0], s0.n[1], s0.n[2]};
+  g.boundary = e.f1 < 0;
+  if (g.boundary) {
+    g.n1 = F3{0.0f, 0.0f, 0.0f};
+  } else {
+    const TriShade& s1 = S.shade[e.f1];
+    g.n1 = F3{s1.n[0], s1.n[1], s1.n[2]};
+  }
+  return g;
+}
+
+__device__ inline F3 nan0(F3 v) { return finite3(v) ? v : F3{0.0f, 0.0f, 0.0f}; }
+
+// Radiance along a camera boundary ray: EvalRadianceEmitted + bounces with
+// throughput = sensor importance (GenerateBoundaryRays), all detached.
+__device__ F3 boundary_radiance(const SceneDev& S, TraceCtx& tc, const Its& its, F3 raydir, float sensor_val,
+                                uint32_t seed, uint32_t domain, uint64_t index, int B) {
+  F3 beta{sensor_val, sensor_val, sensor_val};
+  F3 rad{0.0f, 0.0f, 0.0f};
+  if (its.emitter >= 0) {
+    F3 e = beta * emitted(S, its.emitter, its.ng, -raydir);
+    if (finite3(e)) rad = rad + e;
+  }
+  path_bounces<false, false>(S, tc, its, beta, raydir, seed, domain, index, 1u, B, rad, nullptr, nullptr);
+  return rad;
+}
+
+// BSDF::Eval with the shading-normal correction (BSDF.cpp:172-195).
+__device__ inline F3 bsdf_eval_corrected(const SceneDev& S, const Its& s, F3 wo, F3 wi) {
+  F3 f = bsdf_eval(S, s, wo, wi);
+  F3 lo = to_local(s, wo), li = to_local(s, wi);
+  float corr = fabsf(lo.z * dot(s.ng, wi) / (li.z * dot(s.ng, wo)));
+  return f * corr;
+}
+
+// One vertex of EvalImportance (Integrator.cpp:75-176): the contribution to
+// the pixel seen through the camera and the next (BSDF-sampled) vertex.
+struct ImpStep {
+  bool contrib;
+  int32_t pixel;
+  F3 value;
+  bool next;
+};
+__device__ ImpStep importance_step(const SceneDev& S, const CameraDev& C, TraceCtx& tc, Its& its, F3& rdir, F3& thr,
+                                   Rnd4 rnd) {
+  ImpStep r;
+  r.contrib = false;
+  r.next = false;
+  r.pixel = 0;
+  r.value = F3{0.0f, 0.0f, 0.0f};
+  const F3 eye = cam_eye(C);
+  F3 wo = -rdir;
+  SensorSample sd = sensor_sample(C, its.p);
+  if (sd.valid) {
+    F3 cd = camera_dir(C, sd.qx * (float)C.width, sd.qy * (float)C.height);
+    float tmax = length(its.p - eye) - kShadowEpsilon;
+    bool vis = trace_ray<true>(S, tc, eye, cd, kShadowEpsilon, tmax).tri < 0;
+    if (vis) {
+      F3 t = sd.value * thr;
+      F3 bv{0.0f, 0.0f, 0.0f};
+      if (bsdf_dir_valid(S, its, wo)) {
+        F3 wi = normalize(eye - its.p);
+        float wiok = bsdf_dir_valid(S, its, wi) ? 1.0f : 0.0f;
+        bv = bsdf_eval_corrected(S, its, wo, wi) * fabsf(dot(its.ns, wi)) * wiok;
+      }
+      F3 v = t * bv;
+      if (finite3(v)) {
+        r.contrib = true;
+        r.pixel = sd.pixel;
+        r.value = v;
+      }
+    }
+  }
+  if (bsdf_dir_valid(S, its, wo)) {
+    float pdf;
+    F3 wi = bsdf_sample(S, its, wo, rnd.a, rnd.b, &pdf);
+    F3 bv = bsdf_eval_corrected(S, its, wo, wi) * fabsf(dot(its.ns, wi));
+    Hit h = trace_ray<false>(S, tc, its.p, wi, kShadowEpsilon, 1e32f);
+    if (h.tri >= 0 && pdf > 0.0f && bsdf_dir_valid(S, its, wi)) {
+      r.next = true;
+      thr = thr * bv / pdf;
+      its = post_intersect(S, h.tri, h.u, h.v);
+      rdir = wi;
+    }
+  }
+  return r;
+}
+
+// baseVal of the camera-side terms (BoundaryPrimary.cpp:117-129,
+// BoundaryPixel.cpp:95-107; the pixel term reverts the sign).
+__device__ inline float base_val_camera(F3 itp, F3 ng, F3 raydir, F3 p_edge, F3 eye, F3 edge, F3 edge2, F3* n_out,
+                                        bool revert) {
+  float dist = length(itp - eye);
+  float dist1 = length(p_edge - eye);
+  float cos2 = fabsf(dot(ng, -raydir));
+  F3 e = cross(edge, raydir);
+  float sinphi = length(e);
+  F3 proj = normalize(cross(e, ng));
+  float sinphi2 = length(cross(raydir, proj));
+  F3 n = normalize(cross(ng, proj));
+  bool sign0 = dot(e, edge2) > 0.0f, sign1 = dot(e, n) > 0.0f;
+  float bv = (dist / dist1) * (sinphi / sinphi2) * cos2;
+  bv = bv * (sinphi > kEpsilon ? 1.0f : 0.0f) * (sinphi2 > kEpsilon ? 1.0f : 0.0f);
+  float sg = (sign0 == sign1) ? 1.0f : -1.0f;
+  *n_out = n;
+  return bv * (revert ? -sg : sg);
+}
+
+// baseVal of the secondary terms (Edge.cpp:245-257, BoundaryIndirect.cpp:100-112).
+__device__ inline float base_val_secondary(F3 its_p, F3 far_p, F3 far_n, F3 dir_to_near, F3 p_edge, F3 edge, F3 edge2,
+                                           F3* n_out) {
+  float dist = length(far_p - its_p);
+  float cos2 = fabsf(dot(far_n, dir_to_near));
+  F3 e = cross(edge, -dir_to_near);
+  float sinphi = length(e);
+  F3 proj = normalize(cross(e, far_n));
+  float sinphi2 = length(cross(-dir_to_near, proj));
+  float itsT = length(its_p - p_edge);
+  F3 n = normalize(cross(far_n, proj));
+  bool sign0 = dot(e, edge2) > 0.0f, sign1 = dot(e, n) > 0.0f;
+  float bv = (itsT / dist) * (sinphi / sinphi2) * cos2;
+  bv = bv * (sinphi > kEpsilon ? 1.0f : 0.0f) * (sinphi2 > kEpsilon ? 1.0f : 0.0f);
+  *n_out = n;
+  return bv * ((sign0 == sign1) ? 1.0f : -1.0f);
+}
+
+// VJP of w * xDotN with respect to the triangle vertices P0..P2 (of the
+// RayIntersectAD), the ray origin and the point the ray passes through.
+// xDotN = n . ((1-u-v) P0d + u P1d + v P2d).
+struct XdnAdj {
+  F3 P0, P1, P2, org, through;
+};
+__device__ inline XdnAdj x_dot_n_vjp(F3 P0, F3 P1, F3 P2, F3 org, F3 through, F3 n, float w) {
+  XdnAdj a;
+  F3 dr = through - org;
+  float dl = length(dr);
+  F3 dir = dr / dl;
+  F3 e1 = P1 - P0, e2 = P2 - P0;
+  F3 h = cross(dir, e2);
+  float aa = dot(e1, h);
+  float f = 1.0f / aa;
+  F3 s0 = org - P0;
+  float su = dot(s0, h);
+  F3 q = cross(s0, e1);
+  float sv = dot(dir, q);
+  float adj_u = w * dot(n, P1 - P0);
+  float adj_v = w * dot(n, P2 - P0);
+  float adj_f = adj_u * su + adj_v * sv;
+  float adj_su = adj_u * f, adj_sv = adj_v * f;
+  float adj_a = -adj_f * f * f;
+  F3 adj_s0 = adj_su * h;
+  F3 adj_h = adj_su * s0 + adj_a * e1;
+  F3 adj_dir = adj_sv * q;
+  F3 adj_q = adj_sv * dir;
+  adj_s0 += cross(e1, adj_q);
+  F3 adj_e1 = cross(adj_q, s0) + adj_a * h;
+  adj_dir += cross(e2, adj_h);
+  F3 adj_e2 = cross(adj_h, dir);
+  a.org = adj_s0;
+  a.P0 = -(adj_s0 + adj_e1 + adj_e2);
+  a.P1 = adj_e1;
+  a.P2 = adj_e2;
+  F3 adj_dr = normalize_vjp(dir, dl, adj_dir);
+  a.through = adj_dr;
+  a.org -= adj_dr;
+  return a;
+}
+
+__device__ inline void vtx_add(const SceneDev& S, uint32_t v, int32_t tri, F3 g, float* dpos) {
+  if (!(S.shade[tri].flags & kTriDiff)) return;
+  if (!finite3(g)) return;
+  if (g.x != 0.0f) atomicAdd(&dpos[3 * v], g.x);
+  if (g.y != 0.0f) atomicAdd(&dpos[3 * v + 1], g.y);
+  if (g.z != 0.0f) atomicAdd(&dpos[3 * v + 2], g.z);
+}
+
+__device__ inline F3 dLdI_at(const float* dLdI, uint32_t WH, int32_t pix) {
+  return F3{dLdI[pix], dLdI[WH + pix], dLdI[2 * WH + pix]};
+}
+
+// Scatter of one sample: triangle of the RayIntersectAD (tri), edge point
+// (through = q0 + t (q1 - q0) when the edge is given) and material-form
+// origin (otri, ou, ov) when given.
+__device__ void scatter_xdn(const SceneDev& S, int32_t tri, F3 org, F3 through, F3 n, float w, const EdgeDev* edge,
+                            float et, int32_t ef0, int32_t otri, float ou, float ov, float* dpos) {
+  F3 P0, P1, P2;
+  tri_verts(S, tri, P0, P1, P2);
+  XdnAdj a = x_dot_n_vjp(P0, P1, P2, org, through, n, w);
+  vtx_add(S, S.idx[3 * tri], tri, a.P0, dpos);
+  vtx_add(S, S.idx[3 * tri + 1], tri, a.P1, dpos);
+  vtx_add(S, S.idx[3 * tri + 2], tri, a.P2, dpos);
+  if (edge) {
+    vtx_add(S, edge->v0, ef0, (1.0f - et) * a.through, dpos);
+    vtx_add(S, edge->v1, ef0, et * a.through, dpos);
+  }
+  if (otri >= 0) {
+    float ow = 1.0f - ou - ov;
+    vtx_add(S, S.idx[3 * otri], otri, ow * a.org, dpos);
+    vtx_add(S, S.idx[3 * otri + 1], otri, ou * a.org, dpos);
+    vtx_add(S, S.idx[3 * otri + 2], otri, ov * a.org, dpos);
+  }
+}
+
+struct BoundaryParams {
+  uint32_t seed;
+  int32_t max_bounces;
+  int32_t spp;
+  uint32_t pad;
+};
+
+// ---------------------------------------------------------------------------
+// Primary edges (BoundaryPrimary.cpp:36-181)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_boundary_primary(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
+                                                             EdgeSet E, uint64_t N, const float* dLdI, float* dpos) {
+  __shared__ uint32_t lds_stack[kStackLds * kBlock];
+  TraceCtx tc = make_trace_ctx_b(lds_stack, L);
+  const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
+  const F3 eye = cam_eye(C);
+  const uint64_t stride = (uint64_t)L.grid * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
+    Rnd4 rnd = rng4(P.seed, kRngPrimaryEdge, i, 0u);
+    float pdf;
+    int32_t ei = sample_edge(E, rnd.a, &pdf);
+    const EdgeDev ed = E.e[ei];
+    EdgeGeo g = edge_geo(S, ed);
+    F3 p = g.p0 + rnd.b * g.e1;
+    F3 edge = normalize(g.e1);
+    F3 edge2 = normalize(g.p2 - g.p0);
+    F3 dir = normalize(p - eye);
+    Hit h = trace_ray<false>(S, tc, p, dir, kShadowEpsilon, 1e32f);
+    if (h.tri < 0) continue;
+    F3 hitP = p + h.t * dir;
+    SensorSample sd = sensor_sample(C, hitP);
+    if (!sd.valid) continue;
+    F3 rd = camera_dir(C, sd.qx * (float)C.width, sd.qy * (float)C.height);
+    float tmax = length(p - eye) - kShadowEpsilon;
+    if (trace_ray<true>(S, tc, eye, rd, kShadowEpsilon, tmax).tri >= 0) continue;
+    Its its = post_intersect(S, h.tri, h.u, h.v);
+    F3 n;
+    float bv = base_val_camera(its.p, its.ng, rd, p, eye, edge, edge2, &n, false);
+    F3 rad = boundary_radiance(S, tc, its, rd, sd.value, P.seed, kRngPrimaryEdge, i, P.max_bounces);
+    F3 K = rad * bv / pdf;
+    if (!(length(K) > 0.0f) || !finite3(K)) continue;
+    float w = dot(dLdI_at(dLdI, WH, sd.pixel), K) / (float)P.spp;
+    if (w == 0.0f) continue;
+    scatter_xdn(S, h.tri, eye, p, n, w, &ed, rnd.b, ed.f0, -1, 0.0f, 0.0f, dpos);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pixel boundary of the box filter (BoundaryPixel.cpp:36-159, Edge.cpp:377-409,
+// Camera.cpp:433-467).  4 antithetic positions per random number.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_boundary_pixel(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
+                                                           uint64_t N, const float* dLdI, float* dpos) {
+  __shared__ uint32_t lds_stack[kStackLds * kBlock];
+  TraceCtx tc = make_trace_ctx_b(lds_stack, L);
+  const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
+  const F3 eye = cam_eye(C);
+  const F3 R{C.right[0], C.right[1], C.right[2]}, U{C.up[0], C.up[1], C.up[2]}, D{C.dir[0], C.dir[1], C.dir[2]};
+  const float plx = 2.0f * C.ratio * C.tan_half, ply = 2.0f * C.tan_half;
+  const F3 vdir = normalize(C.ay * U), hdir = normalize(C.ax * R);
+  const uint64_t per_anti = N / 4;
+  const uint64_t stride = (uint64_t)L.grid * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
+    const uint64_t j = i % per_anti, anti = i / per_anti;
+    float rr = rng4(P.seed, kRngPixelEdge, j, 0u).a;
+    float rv;
+    if (anti == 0) rv = rr;
+    else if (anti == 1) rv = rr < 0.5f ? rr + 0.5f : rr - 0.5f;
+    else if (anti == 2) rv = rr < 0.75f ? 0.75f - rr : 1.75f - rr;
+    else rv = rr < 0.25f ? 0.25f - rr : 1.25f - rr;
+    float r4 = rv * 4.0f;
+    int side = (int)floorf(r4);
+    float ox, oy;
+    if (side == 0) { ox = 0.0f; oy = r4; }
+    else if (side == 1) { ox = r4 - 1.0f; oy = 1.0f; }
+    else if (side == 2) { ox = 1.0f; oy = 3.0f - r4; }
+    else { ox = 4.0f - r4; oy = 0.0f; }
+    const uint32_t pix = (uint32_t)(i % WH);
+    float rx = (float)(pix % (uint32_t)C.width) + ox, ry = (float)(pix / (uint32_t)C.width) + oy;
+    F3 p0 = ((eye + (C.ax * rx + C.bx) * R) + (C.ay * ry + C.by) * U) + D;
+    F3 edge = (side == 0 || side == 2) ? vdir : hdir;
+    F3 edge2 = side == 0 ? hdir : (side == 1 ? -vdir : (side == 2 ? -hdir : vdir));
+    float pdf = (side == 0 || side == 2) ? 0.25f / (ply * (float)C.width) : 0.25f / (plx * (float)C.height);
+    F3 dir = normalize(p0 - eye);
+    Hit h = trace_ray<false>(S, tc, p0, dir, kShadowEpsilon, 1e32f);
+    if (h.tri < 0) continue;
+    F3 hitP = p0 + h.t * dir;
+    SensorSample sd = sensor_sample(C, hitP);
+    F3 rd = camera_dir(C, sd.qx * (float)C.width, sd.qy * (float)C.height);
+    Its its = post_intersect(S, h.tri, h.u, h.v);
+    F3 n;
+    float bv = base_val_camera(its.p, its.ng, rd, p0, eye, edge, edge2, &n, true);
+    F3 rad = boundary_radiance(S, tc, its, rd, sd.value, P.seed, kRngPixelEdge, j, P.max_bounces);
+    F3 K = rad * bv / pdf;
+    if (!finite3(K)) continue;
+    float w = dot(dLdI_at(dLdI, WH, (int32_t)pix), K) / (float)P.spp;
+    if (w == 0.0f) continue;
+    scatter_xdn(S, h.tri, eye, p0, n, w, nullptr, 0.0f, -1, -1, 0.0f, 0.0f, dpos);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Secondary edges, direct illumination (BoundaryDirect.cpp:38-143,
+// Edge.cpp:108-144 and 201-309)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_boundary_direct(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
+                                                            uint64_t N, const float* dLdI, float* dpos) {
+  __shared__ uint32_t lds_stack[kStackLds * kBlock];
+  TraceCtx tc = make_trace_ctx_b(lds_stack, L);
+  const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
+  const EdgeSet& E = S.sec;
+  const uint64_t stride = (uint64_t)L.grid * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
+    Rnd4 rnd = rng4(P.seed, kRngDirectEdge, i, 0u);
+    float pdf_e;
+    int32_t ei = sample_edge(E, rnd.a, &pdf_e);
+    const EdgeDev ed = E.e[ei];
+    EdgeGeo g = edge_geo(S, ed);
+    float t = cdf_reuse(E.cdf, ei, rnd.a);
+    F3 p = g.p0 + t * g.e1;
+    F3 edge = normalize(g.e1);
+    F3 edge2 = normalize(g.p2 - g.p0);
+    LightSample ls = light_sample(S, rnd.b, rnd.c);
+    F3 ev = ls.p - p;
+    float d2 = len2(ev), dist = sqrtf(d2);
+    F3 en = ev / dist;
+    float cosT = dot(ls.n, -en);
+    float c0 = dot(en, g.n0), c1 = dot(en, g.n1);
+    bool valid0 = fabsf(c0) > kEdgeEpsilon;
+    bool valid1 = (c0 > kEdgeEpsilon && c1 < -kEdgeEpsilon) || (c0 < -kEdgeEpsilon && c1 > kEdgeEpsilon);
+    bool right = (g.boundary && valid0) || (!g.boundary && valid1);
+    if (!(right && cosT > kEpsilon && dist > kShadowEpsilon)) continue;
+    float pdf = pdf_e * ls.pdf * d2 / cosT * 1.0f;
+    F3 rdir = normalize(p - ls.p);
+    Hit h0 = trace_ray<false>(S, tc, p, -rdir, kShadowEpsilon, 1e32f);
+    if (h0.tri < 0) continue;
+    Hit h = trace_ray<false>(S, tc, p, rdir, kShadowEpsilon, 1e32f);
+    if (h.tri < 0) continue;
+    F3 hitP = p + h0.t * (-rdir);
+    if (!(length(hitP - ls.p) < kShadowEpsilon)) continue;
+    F3 Le{0.0f, 0.0f, 0.0f};
+    const TriShade& sh0 = S.shade[h0.tri];
+    if (sh0.emitter >= 0) Le = emitted(S, sh0.emitter, F3{sh0.n[0], sh0.n[1], sh0.n[2]}, rdir);
+    Its its = post_intersect(S, h.tri, h.u, h.v);
+    F3 n;
+    float bv = base_val_secondary(its.p, ls.p, ls.n, rdir, p, edge, edge2, &n);
+    F3 K0 = Le * bv / pdf;
+    // sensor side: EvalImportance for B bounces, throughput 1
+    float w = 0.0f;
+    Its cur = its;
+    F3 cdir = rdir, thr{1.0f, 1.0f, 1.0f};
+    for (int b = 0; b < P.max_bounces; ++b) {
+      ImpStep st = importance_step(S, C, tc, cur, cdir, thr, rng4(P.seed, kRngDirectEdge, i, kSensorBlock0 + (uint32_t)b));
+      if (st.contrib) {
+        F3 v = K0 * st.value;
+        if (length(v) > 0.0f) w += dot(dLdI_at(dLdI, WH, st.pixel), v);
+      }
+      if (!st.next) break;
+    }
+    w = w / (float)P.spp;
+    if (w == 0.0f || !isfinite(w)) continue;
+    scatter_xdn(S, h0.tri, its.p, p, n, w, &ed, t, ed.f0, h.tri, h.u, h.v, dpos);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Secondary edges, indirect illumination (BoundaryIndirect.cpp:38-192,
+// Edge.cpp:146-199)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_boundary_indirect(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
+                                                              uint64_t N, const float* dLdI, float* dpos) {
+  __shared__ uint32_t lds_stack[kStackLds * kBlock];
+  TraceCtx tc = make_trace_ctx_b(lds_stack, L);
+  const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
+  const EdgeSet& E = S.sec;
+  const int B = P.max_bounces;
+  const uint64_t stride = (uint64_t)L.grid * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
+    Rnd4 rnd = rng4(P.seed, kRngIndirectEdge, i, 0u);
+    float pdf_e;
+    int32_t ei = sample_edge(E, rnd.a, &pdf_e);
+    const EdgeDev ed = E.e[ei];
+    EdgeGeo g = edge_geo(S, ed);
+    float t = cdf_reuse(E.cdf, ei, rnd.a);
+    F3 p = g.p0 + t * g.e1;
+    F3 edge = normalize(g.e1);
+    F3 edge2 = normalize(g.p2 - g.p0);
+    F3 dir;
+    float pdf_d;
+    if (g.boundary) {
+      F3 nrm = cross(g.n0, edge);
+      F3 ft, fb;
+      coordinate_system(nrm, &ft, &fb);
+      float z = minf(1.0f - 2.0f * rnd.b, 1.0f - kEdgeEpsilon);
+      float rr = sqrtf(maxf(0.0f, 1.0f - z * z));
+      float sp, cp;
+      tr_sincos(2.0f * kPi * rnd.c, &sp, &cp);
+      dir = ((rr * cp) * ft + (rr * sp) * fb) + z * nrm;
+      pdf_d = 0.25f / kPi;
+    } else {
+      float phi0 = tr_acos(dot(g.n0, g.n1));
+      pdf_d = 0.25f / phi0;
+      F3 z = normalize(g.n0 + g.n1);
+      F3 y = normalize(cross(g.n0, z));
+      F3 x = cross(y, z);
+      float phi = (rnd.b - 0.5f) * phi0;
+      phi = minf(maxf(phi, -0.5f * phi0 + kEdgeEpsilon), 0.5f * phi0 - kEdgeEpsilon);
+      float sp, cp;
+      tr_sincos(phi, &sp, &cp);
+      F3 x1 = x * cp + z * sp;
+      bool m0 = rnd.c > 0.5f;
+      float bb = 4.0f * rnd.c - (m0 ? 3.0f : 1.0f);
+      float aa = sqrtf(1.0f - bb * bb) * (m0 ? -1.0f : 1.0f);
+      dir = x1 * aa + y * bb;
+    }
+    float pdf = pdf_e * pdf_d;
+    Hit he = trace_ray<false>(S, tc, p, -dir, kShadowEpsilon, 1e32f);
+    if (he.tri < 0) continue;
+    Hit hs = trace_ray<false>(S, tc, p, dir, kShadowEpsilon, 1e32f);
+    if (hs.tri < 0) continue;
+    Its ite = post_intersect(S, he.tri, he.u, he.v);
+    Its its = post_intersect(S, hs.tri, hs.u, hs.v);
+    F3 n;
+    float bv = base_val_secondary(its.p, ite.p, ite.ng, dir, p, edge, edge2, &n);
+    float v0 = bv / pdf;
+    // emitter side: cumulative radiance after 1..B-1 bounces
+    F3 cum[kMaxBounces];
+    F3 per[kMaxBounces];
+    for (int k = 0; k < kMaxBounces; ++k) per[k] = F3{0.0f, 0.0f, 0.0f};
+    F3 acc{0.0f, 0.0f, 0.0f};
+    if (B > 1)
+      path_bounces<false, true>(S, tc, ite, F3{1.0f, 1.0f, 1.0f}, -dir, P.seed, kRngIndirectEdge, i, 1u, B - 1, acc,
+                                nullptr, per);
+    F3 run{0.0f, 0.0f, 0.0f};
+    for (int k = 0; k < B - 1; ++k) {
+      run = run + per[k];
+      cum[k] = run;
+    }
+    // sensor side
+    float w = 0.0f;
+    Its cur = its;
+    F3 cdir = dir, thr{1.0f, 1.0f, 1.0f};
+    for (int b = 0; b < B - 1; ++b) {
+      ImpStep st = importance_step(S, C, tc, cur, cdir, thr, rng4(P.seed, kRngIndirectEdge, i, kSensorBlock0 + (uint32_t)b));
+      int ri = B - b - 2;
+      if (st.contrib && ri >= 0) {
+        F3 v = (v0 * cum[ri]) * st.value;
+        if (length(v) > 0.0f) w += dot(dLdI_at(dLdI, WH, st.pixel), v);
+      }
+      if (!st.next) break;
+    }
+    w = w / (float)P.spp;
+    if (w == 0.0f || !isfinite(w)) continue;
+    scatter_xdn(S, he.tri, its.p, p, n, w, &ed, t, ed.f0, hs.tri, hs.u, hs.v, dpos);
+  }
+}
+
+hipError_t launch_boundary(int kind, const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
+                           const LaunchCfg& L, const EdgeSet& prim, const float* dLdI, float* dpos, hipStream_t st) {
+  BoundaryParams P{seed, max_bounces, spp, 0u};
+  uint64_t N = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
+  if (kind == 0)
+    hipLaunchKernelGGL(k_boundary_primary, dim3(L.grid), dim3(kBlock), 0, st, S, C, P, L, prim, N, dLdI, dpos);
+  else if (kind == 1)
+    hipLaunchKernelGGL(k_boundary_pixel, dim3(L.grid), dim3(kBlock), 0, st, S, C, P, L, N, dLdI, dpos);
+  else if (kind == 2)
+    hipLaunchKernelGGL(k_boundary_direct, dim3(L.grid), dim3(kBlock), 0, st, S, C, P, L, N, dLdI, dpos);
+  else
+    hipLaunchKernelGGL(k_boundary_indirect, dim3(L.grid), dim3(kBlock), 0, st, S, C, P, L, N, dLdI, dpos);
+  return hipGetLastError();
+}
+
+}  // namespace trc

@@ -39,6 +39,107 @@ struct PathRec {
 
 TR_HD F3 nan_to_zero(F3 v) { return finite3(v) ? v : F3{0.0f, 0.0f, 0.0f}; }
 
+// Bounce loop (PathTracer.cpp:55-73, EvalRadianceDirect Integrator.cpp:178-303)
+// from surface point `its` reached along `raydir` with throughput `beta`.
+// Bounce b draws blocks block0 + 2b (light) and block0 + 2b + 1 (BSDF) of
+// (seed, domain, index).  kRecord fills the replay record (interior
+// adjoint); kPerBounce stores each executed bounce's contribution.
+template <bool kRecord, bool kPerBounce>
+__device__ void path_bounces(const SceneDev& S, TraceCtx& tc, Its its, F3 beta, F3 raydir, uint32_t seed,
+                             uint32_t domain, uint64_t index, uint32_t block0, int B, F3& contrib, PathRec* R,
+                             F3* per_bounce) {
+  const F3 zero{0.0f, 0.0f, 0.0f};
+  for (int b = 0; b < B; ++b) {
+    Rnd4 rl = rng4(seed, domain, index, block0 + 2u * (uint32_t)b);
+    Rnd4 rb = rng4(seed, domain, index, block0 + 2u * (uint32_t)b + 1u);
+    F3 ret = zero;
+    F3 wo = -raydir;
+    // ---- next-event estimation ----
+    LightSample ls = light_sample(S, rl.a, rl.b);
+    F3 neec = zero;
+    {
+      F3 wi = ls.p - its.p;
+      float d2 = len2(wi);
+      float dist = sqrtf(d2);
+      wi = wi / dist;
+      float dsh = dot(its.ns, wi), dgeo = dot(its.ng, wi);
+      bool stable = dist > kShadowEpsilon && fabsf(dsh) > kEdgeEpsilon;
+      bool noleak = (dsh * dgeo) > 0.0f;
+      bool visible = false;
+      if (stable && noleak)
+        visible = trace_ray<true>(S, tc, its.p, wi, kShadowEpsilon, dist - kShadowEpsilon).tri < 0;
+      float valid = (stable && noleak && visible) ? 1.0f : 0.0f;
+      float Gl = fabsf(dot(ls.n, -wi)) / d2;
+      F3 f = bsdf_eval(S, its, wo, wi) * fabsf(dsh);
+      F3 Le = emitted(S, ls.emitter, ls.n, -wi);
+      float JL = S.shade[ls.tri].area / S.shade[ls.tri].area;
+      F3 val = valid * beta * f * JL * Gl * Le / ls.pdf;
+      float pb = bsdf_pdf(S, its, wo, wi) * Gl;
+      float mis = (ls.pdf * ls.pdf) / (ls.pdf * ls.pdf + pb * pb);
+      val = val * mis;
+      ret = ret + val;
+      if (kRecord) neec = valid * Le / ls.pdf * mis;
+    }
+    if (kRecord) {
+      R->ltri[b] = ls.tri; R->lbu[b] = ls.bu; R->lbv[b] = ls.bv;
+      R->neec[b] = neec;
+      R->emc[b] = zero;
+    }
+    // ---- BSDF sampling ----
+    float pdfb;
+    F3 wib = bsdf_sample(S, its, wo, rb.a, rb.b, &pdfb);
+    Hit h2 = trace_ray<false>(S, tc, its.p, wib, kShadowEpsilon, 1e32f);
+    bool ok = h2.tri >= 0 && pdfb > 0.0f && bsdf_dir_valid(S, its, wib);
+    if (!ok) {
+      if (finite3(ret)) {
+        contrib = contrib + ret;
+        if (kPerBounce) per_bounce[b] = ret;
+      } else if (kRecord) {
+        R->neec[b] = zero;
+      }
+      break;
+    }
+    Its its2 = post_intersect(S, h2.tri, h2.u, h2.v);
+    F3 dd = its2.p - its.p;
+    float dd2 = len2(dd);
+    F3 nd = dd / sqrtf(dd2);
+    float G2 = fabsf(dot(its2.ng, -nd)) / dd2;
+    F3 f2 = bsdf_eval(S, its, wo, nd) * fabsf(dot(its.ns, nd));
+    float ppdf = pdfb * G2;
+    float J2 = S.shade[its2.tri].area / S.shade[its2.tri].area;
+    F3 beta2 = beta * f2 * G2 * J2 / ppdf;
+    bool gpos = G2 > 0.0f;
+    if (!gpos) beta2 = zero;
+    F3 emc = zero;
+    if (its2.emitter >= 0) {
+      F3 Le2 = emitted(S, its2.emitter, its2.ng, -nd);
+      F3 val2 = beta2 * Le2;
+      float pn = light_pdf_tri(S, its2.tri);
+      float mis2 = (ppdf * ppdf) / (ppdf * ppdf + pn * pn);
+      val2 = val2 * mis2;
+      ret = ret + val2;
+      emc = Le2 * mis2;
+    }
+    bool fin = finite3(ret);
+    if (fin) {
+      contrib = contrib + ret;
+      if (kPerBounce) per_bounce[b] = ret;
+    }
+    if (kRecord) {
+      if (!fin) { R->neec[b] = zero; emc = zero; }
+      R->emc[b] = emc;
+      R->ppdf[b] = ppdf;
+      if (gpos) R->tflag |= 1u << b;
+      R->K = b + 1;
+      R->tri[b + 1] = its2.tri; R->bu[b + 1] = h2.u; R->bv[b + 1] = h2.v;
+      R->beta[b + 1] = beta2;
+    }
+    its = its2;
+    beta = beta2;
+    raydir = nd;
+  }
+}
+
 template <bool kRecord>
 __device__ F3 path_forward(const SceneDev& S, const CameraDev& C, const PathParams& P, uint64_t path,
                            TraceCtx& tc, PathRec* R) {
@@ -82,89 +183,8 @@ __device__ F3 path_forward(const SceneDev& S, const CameraDev& C, const PathPara
     R->beta[0] = beta;
     R->e0c = e0c;
   }
-  const int B = P.max_bounces;
-  for (int b = 0; b < B; ++b) {
-    Rnd4 rl = rng4(P.seed, kRngInterior, path, 1u + 2u * (uint32_t)b);
-    Rnd4 rb = rng4(P.seed, kRngInterior, path, 2u + 2u * (uint32_t)b);
-    F3 ret = zero;
-    F3 wo = -raydir;
-    // ---- next-event estimation ----
-    LightSample ls = light_sample(S, rl.a, rl.b);
-    F3 neec = zero;
-    {
-      F3 wi = ls.p - its.p;
-      float d2 = len2(wi);
-      float dist = sqrtf(d2);
-      wi = wi / dist;
-      float dsh = dot(its.ns, wi), dgeo = dot(its.ng, wi);
-      bool stable = dist > kShadowEpsilon && fabsf(dsh) > kEdgeEpsilon;
-      bool noleak = (dsh * dgeo) > 0.0f;
-      bool visible = false;
-      if (stable && noleak)
-        visible = trace_ray<true>(S, tc, its.p, wi, kShadowEpsilon, dist - kShadowEpsilon).tri < 0;
-      float valid = (stable && noleak && visible) ? 1.0f : 0.0f;
-      float Gl = fabsf(dot(ls.n, -wi)) / d2;
-      F3 f = bsdf_eval(S, its, wo, wi) * fabsf(dsh);
-      F3 Le = emitted(S, ls.emitter, ls.n, -wi);
-      float JL = S.shade[ls.tri].area / S.shade[ls.tri].area;
-      F3 val = valid * beta * f * JL * Gl * Le / ls.pdf;
-      float pb = bsdf_pdf(S, its, wo, wi) * Gl;
-      float mis = (ls.pdf * ls.pdf) / (ls.pdf * ls.pdf + pb * pb);
-      val = val * mis;
-      ret = ret + val;
-      neec = valid * Le / ls.pdf * mis;
-    }
-    if (kRecord) {
-      R->ltri[b] = ls.tri; R->lbu[b] = ls.bu; R->lbv[b] = ls.bv;
-      R->neec[b] = neec;
-      R->emc[b] = zero;
-    }
-    // ---- BSDF sampling ----
-    float pdfb;
-    F3 wib = bsdf_sample(S, its, wo, rb.a, rb.b, &pdfb);
-    Hit h2 = trace_ray<false>(S, tc, its.p, wib, kShadowEpsilon, 1e32f);
-    bool ok = h2.tri >= 0 && pdfb > 0.0f && bsdf_dir_valid(S, its, wib);
-    if (!ok) {
-      if (finite3(ret)) contrib = contrib + ret;
-      else if (kRecord) R->neec[b] = zero;
-      break;
-    }
-    Its its2 = post_intersect(S, h2.tri, h2.u, h2.v);
-    F3 dd = its2.p - its.p;
-    float dd2 = len2(dd);
-    F3 nd = dd / sqrtf(dd2);
-    float G2 = fabsf(dot(its2.ng, -nd)) / dd2;
-    F3 f2 = bsdf_eval(S, its, wo, nd) * fabsf(dot(its.ns, nd));
-    float ppdf = pdfb * G2;
-    float J2 = S.shade[its2.tri].area / S.shade[its2.tri].area;
-    F3 beta2 = beta * f2 * G2 * J2 / ppdf;
-    bool gpos = G2 > 0.0f;
-    if (!gpos) beta2 = zero;
-    F3 emc = zero;
-    if (its2.emitter >= 0) {
-      F3 Le2 = emitted(S, its2.emitter, its2.ng, -nd);
-      F3 val2 = beta2 * Le2;
-      float pn = light_pdf_tri(S, its2.tri);
-      float mis2 = (ppdf * ppdf) / (ppdf * ppdf + pn * pn);
-      val2 = val2 * mis2;
-      ret = ret + val2;
-      emc = Le2 * mis2;
-    }
-    bool fin = finite3(ret);
-    if (fin) contrib = contrib + ret;
-    if (kRecord) {
-      if (!fin) { R->neec[b] = zero; emc = zero; }
-      R->emc[b] = emc;
-      R->ppdf[b] = ppdf;
-      if (gpos) R->tflag |= 1u << b;
-      R->K = b + 1;
-      R->tri[b + 1] = its2.tri; R->bu[b + 1] = h2.u; R->bv[b + 1] = h2.v;
-      R->beta[b + 1] = beta2;
-    }
-    its = its2;
-    beta = beta2;
-    raydir = nd;
-  }
+  path_bounces<kRecord, false>(S, tc, its, beta, raydir, P.seed, kRngInterior, path, 1u, P.max_bounces, contrib,
+                               R, nullptr);
   return contrib;
 }
 

@@ -34,6 +34,11 @@ struct trc_scene {
   trc::SceneDev dev{};
   trc::TexDesc* dev_texs = nullptr;
   int64_t device_bytes = 0;
+  // primary-edge list of the last camera (host staging kept alive until the
+  // next call, device copy in prim_buf)
+  std::vector<trc::EdgeDev> prim_edges;
+  std::vector<float> prim_len, prim_cdf;
+  DevBuf prim_buf;
 };
 
 namespace trc {
@@ -56,6 +61,9 @@ hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t s
 hipError_t launch_render_stats(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                                const LaunchCfg& L, unsigned long long* counters, hipStream_t st);
 hipError_t launch_film_reduce(const float* paths, int spp, int wh, float* film, hipStream_t st);
+// kind: 0 primary edges, 1 pixel boundary, 2 secondary direct, 3 secondary indirect
+hipError_t launch_boundary(int kind, const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
+                           const LaunchCfg& L, const EdgeSet& prim, const float* dLdI, float* dpos, hipStream_t st);
 hipError_t launch_render_adj(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                              const LaunchCfg& L, const float* dLdI, float* dpos, float* dmat, hipStream_t st);
 

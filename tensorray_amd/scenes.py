@@ -278,6 +278,46 @@ def sphere_scene(subdiv=2, amplitude=0.0, noise_seed=1, material=None, diff=True
     return spec
 
 
+def _quad(p0, p1, p2, p3):
+    """Quad p0 p1 p2 p3 (counter-clockwise seen from the side its normal
+    points to): triangles (0, 1, 2), (0, 2, 3)."""
+    return np.array([p0, p1, p2, p3], np.float32), np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+
+
+BOX_CAMERA = dict(pos=(0.0, 0.0, 3.4), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=45.0)
+
+
+def box_scene(subdiv=2, amplitude=0.0, noise_seed=1, material=None):
+    """Closed room [-1.5, 1.5]^3 (open towards +z) with a 0.8 x 0.8 area light
+    under the ceiling, coloured side walls and an icosphere of radius 0.6.
+    Inter-reflections make every boundary term (incl. the indirect secondary
+    edges) non-zero.  Use with BOX_CAMERA."""
+    spec = SceneSpec()
+    spec.materials = [material or Material(MAT_DIFFUSE, albedo=(0.6, 0.6, 0.6)),
+                      Material(MAT_DIFFUSE, albedo=(0.7, 0.7, 0.7)),
+                      Material(MAT_DIFFUSE, albedo=(0.7, 0.2, 0.2)),
+                      Material(MAT_DIFFUSE, albedo=(0.2, 0.7, 0.2)),
+                      Material(MAT_DIFFUSE, albedo=(0.0, 0.0, 0.0))]
+    pos, idx = icosphere(subdiv, radius=0.6, center=(0.0, -0.6, 0.0))
+    if amplitude:
+        pos = displace(pos, amplitude, noise_seed, center=(0.0, -0.6, 0.0))
+    spec.meshes.append(Mesh(pos, idx, material=0, flags=MESH_DIFF))
+    a = 1.5
+    walls = [
+        (_quad((-a, -a, -a), (-a, -a, a), (a, -a, a), (a, -a, -a)), 1),   # floor, +y
+        (_quad((-a, a, -a), (a, a, -a), (a, a, a), (-a, a, a)), 1),       # ceiling, -y
+        (_quad((-a, -a, -a), (a, -a, -a), (a, a, -a), (-a, a, -a)), 1),   # back, +z
+        (_quad((-a, -a, -a), (-a, a, -a), (-a, a, a), (-a, -a, a)), 2),   # left, +x
+        (_quad((a, -a, -a), (a, -a, a), (a, a, a), (a, a, -a)), 3),       # right, -x
+    ]
+    for (p, i), m in walls:
+        spec.meshes.append(Mesh(p, i, material=m))
+    l = 0.4
+    lp, li = _quad((-l, a - 0.01, -l), (l, a - 0.01, -l), (l, a - 0.01, l), (-l, a - 0.01, l))
+    spec.meshes.append(Mesh(lp, li, material=4, emit=(12.0, 12.0, 12.0)))
+    return spec
+
+
 def config1_scene():
     """64x64, 16 spp, depth 2: icosphere subdiv 2 (320 tris), Lambert 0.5."""
     return sphere_scene(subdiv=2, amplitude=0.0)

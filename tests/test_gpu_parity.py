@@ -202,3 +202,35 @@ def test_invalid_options_rejected(gpu_ctx):
     with pytest.raises(_capi.TRCError, match="spp"):
         ds.render_fwd(cam, _capi.make_options(max_bounces=1, spp=0), film.data_ptr())
     ds.close()
+
+
+BOUNDARY_CASES = {
+    # kind: (scene factory, camera, W, H, spp, B)
+    "primary": (lambda: scenes.sphere_scene(subdiv=2), scenes.CAMERA_DEFAULT, 48, 48, 4, 1),
+    "pixel": (lambda: scenes.sphere_scene(subdiv=2), scenes.CAMERA_DEFAULT, 48, 48, 4, 1),
+    "direct": (lambda: scenes.sphere_scene(subdiv=2), scenes.CAMERA_DEFAULT, 48, 48, 8, 2),
+    "indirect": (lambda: scenes.box_scene(subdiv=2), scenes.BOX_CAMERA, 32, 32, 16, 3),
+}
+
+
+@pytest.mark.parametrize("kind", list(BOUNDARY_CASES))
+def test_boundary_gradients_match_oracle(gpu_ctx, oracle_mod, kind):
+    torch = _torch()
+    make, cam_d, W, H, spp, B = BOUNDARY_CASES[kind]
+    spec = make()
+    ds = _capi.DeviceScene(gpu_ctx, spec)
+    info = ds.info()
+    cam = _capi.make_camera(W, H, **cam_d)
+    opt = _capi.make_options(seed=31, max_bounces=B, spp=1, sppe=spp, sppe0=spp, sppse0=spp, sppse1=spp)
+    dl = np.random.default_rng(2).uniform(0.2, 1.0, (3, W * H)).astype(np.float32)
+    dpos = torch.zeros(info.num_vertices, 3, device="cuda")
+    ds.render_boundary(cam, opt, kind, torch.from_numpy(dl).cuda().data_ptr(), dpos.data_ptr())
+    torch.cuda.synchronize()
+    S = oracle_mod.Scene(spec)
+    ref = oracle_mod.boundary_adj(S, cam_d, W, H, 31, B, kind, spp, dl)
+    got = dpos.cpu().numpy().astype(np.float64)
+    mask = spec.diff_vertex_mask()
+    assert np.abs(ref[mask]).max() > 0, "oracle boundary term is zero on this scene"
+    ok, e = _grad_close(got[mask], ref[mask])
+    assert ok, e
+    ds.close()
