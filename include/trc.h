@@ -63,6 +63,15 @@ typedef struct trc_options {
   int32_t sppse1;        /* secondary (indirect) edge samples per pixel */
   int32_t sppe0;         /* pixel-boundary samples per pixel */
   int32_t quiet;
+  /* Guiding of the secondary direct term (Integrator.h:72-76, AQ_distrb.h):
+   * one leaf per secondary edge over [cdf_{i-1}, cdf_i] x [0,1]^2, weight =
+   * mean |baseVal| over g_direct_spp samples, mixed eps + (1 - eps) w. */
+  int32_t g_direct;          /* 0 = off */
+  int32_t g_direct_depth;    /* accepted and ignored: the reference's refinement loop is disabled */
+  int32_t g_direct_max_size; /* leaf capacity; more secondary edges than this is an error (<= 0: no limit) */
+  int32_t g_direct_spp;      /* guide samples per leaf, 1..4096 */
+  float g_direct_thold;      /* accepted and ignored, as in the reference */
+  float g_eps;               /* uniform mix in [0, 1]; the reference uses 0.01 */
 } trc_options;
 
 typedef struct trc_scene_info {

@@ -214,15 +214,20 @@ def _boundary_kinds():
     return {"primary": 0, "pixel": 1, "direct": 2, "indirect": 3}
 
 
-def boundary_adj(scene, cam, w, h, seed, max_bounces, kind, spp, dLdI, first=0, count=None, threads=None):
+def boundary_adj(scene, cam, w, h, seed, max_bounces, kind, spp, dLdI, first=0, count=None, threads=None,
+                 guide=None):
     """Gradient [V,3] (float64) of one boundary estimator (primary, pixel,
-    direct, indirect) for samples [first, first + count) of spp * w * h."""
+    direct, indirect) for samples [first, first + count) of spp * w * h.
+    guide = (g_spp, g_eps) enables guiding of the direct term."""
     L = lib()
     if not hasattr(L, "_bnd_decl"):
         P = ctypes.POINTER
         L.oracle_boundary_adj.argtypes = [ctypes.c_void_p, P(ctypes.c_float), ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                           ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_float), ctypes.c_uint64,
-                                          ctypes.c_uint64, P(ctypes.c_double), ctypes.c_int]
+                                          ctypes.c_uint64, P(ctypes.c_double), ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_float]
+        L.oracle_guide.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_float, P(ctypes.c_float),
+                                   P(ctypes.c_float), P(ctypes.c_float), P(ctypes.c_int)]
         L._bnd_decl = True
     dLdI = np.ascontiguousarray(dLdI, np.float32).reshape(3, w * h)
     if count is None:
@@ -230,5 +235,23 @@ def boundary_adj(scene, cam, w, h, seed, max_bounces, kind, spp, dLdI, first=0, 
     dpos = np.zeros((scene.V, 3), np.float64)
     c = scene._cam(cam)
     L.oracle_boundary_adj(scene.h, _p(c, ctypes.c_float), w, h, seed, max_bounces, _boundary_kinds()[kind], spp,
-                          _p(dLdI, ctypes.c_float), first, count, _p(dpos, ctypes.c_double), threads or default_threads())
+                          _p(dLdI, ctypes.c_float), first, count, _p(dpos, ctypes.c_double), threads or default_threads(),
+                          1 if guide else 0, int(guide[0]) if guide else 0, float(guide[1]) if guide else 0.0)
     return dpos
+
+
+def guide_weights(scene, seed, g_spp, g_eps):
+    """Guiding leaves of the direct boundary term: (x0, x1, poly) per
+    secondary edge, poly after the eps mix."""
+    L = lib()
+    n = ctypes.c_int(0)
+    L.oracle_guide.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_float,
+                               ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                               ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int)]
+    L.oracle_guide(scene.h, seed, g_spp, g_eps, None, None, None, ctypes.byref(n))
+    x0 = np.zeros(n.value, np.float32)
+    x1 = np.zeros(n.value, np.float32)
+    poly = np.zeros(n.value, np.float32)
+    L.oracle_guide(scene.h, seed, g_spp, g_eps, _p(x0, ctypes.c_float), _p(x1, ctypes.c_float),
+                   _p(poly, ctypes.c_float), ctypes.byref(n))
+    return x0, x1, poly

@@ -1436,10 +1436,95 @@ static BSample pixel_sample(const Scene& s, const Cam& c, uint32_t seed, int B, 
   return r;
 }
 
+// ---------------------------------------------------------------------------
+// Guiding of the direct boundary term (AQ_distrb.h:83-197, NEE.cu:70-89):
+// one leaf per secondary edge, x in [cdf_{i-1}, cdf_i] of the edge-length
+// CDF, y, z in [0, 1]; weight = mean |baseVal| over gspp samples (guiding
+// mode of Sample/EvalBoundarySegmentDirect: only the hit / same-point tests
+// apply), normalized, mixed eps + (1 - eps) w; leaves sampled with
+// probability proportional to weight * volume.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kDomGuide = 0x66u;
+
+static void parallel_for(uint64_t n, int threads, const std::function<void(uint64_t, uint64_t, int)>& f);
+
+struct Guide {
+  std::vector<float> x0, x1, poly;
+  std::unique_ptr<Dist1> dist;
+  float fsum = 0.0f;
+};
+
+static float guide_value(const Scene& s, const std::vector<EdgeRec>& E, const Dist1& D, float x, float y, float z) {
+  float pmf;
+  int ei = D.sample(x, &pmf);
+  const EdgeRec& e = E[ei];
+  float t = D.reuse(x, ei);
+  F3 p = e.p0 + t * e.e1;
+  F3 edge = normalize(e.e1);
+  F3 edge2 = normalize(e.p2 - e.p0);
+  Access<float> A(s);
+  LightS<float> ls = light_sample(A, y, z);
+  F3 rdir = normalize(p - ls.p);
+  Hit h0 = intersect(s, p, -rdir, SHADOW_EPSILON_, 1e32f, false, false);
+  Hit h = intersect(s, p, rdir, SHADOW_EPSILON_, 1e32f, false, false);
+  if (h0.tri < 0 || h.tri < 0) return 0.0f;
+  F3 hitP = p + h0.t * (-rdir);
+  if (!(length(hitP - ls.p) < SHADOW_EPSILON_)) return 0.0f;
+  Its<float> it = post_intersect(A, h.tri, h.u, h.v);
+  F3 n;
+  float bv = base_val_secondary(it.p, ls.p, ls.n, rdir, p, edge, edge2, &n);
+  float v = fabsf(bv);
+  return std::isfinite(v) ? v : 0.0f;
+}
+
+static std::unique_ptr<Guide> build_guide(const Scene& s, const std::vector<EdgeRec>& E, const Dist1& D,
+                                          uint32_t seed, int gspp, float eps, int threads) {
+  std::unique_ptr<Guide> g(new Guide());
+  const size_t n = E.size();
+  g->x0.resize(n);
+  g->x1.resize(n);
+  g->poly.assign(n, 0.0f);
+  for (size_t i = 0; i < n; ++i) {
+    g->x0[i] = i > 0 ? D.cdf[i - 1] : 0.0f;
+    g->x1[i] = D.cdf[i];
+  }
+  parallel_for(n, threads, [&](uint64_t a, uint64_t b, int) {
+    for (uint64_t i = a; i < b; ++i) {
+      float acc = 0.0f;
+      for (int k = 0; k < gspp; ++k) {
+        R4 r = rnd4(seed, kDomGuide, i * (uint64_t)gspp + (uint64_t)k, 0u);
+        float x = g->x0[i] + r.a * (g->x1[i] - g->x0[i]);
+        acc = acc + guide_value(s, E, D, x, r.b, r.c) / (float)gspp;
+      }
+      g->poly[i] = acc;
+    }
+  });
+  float sum = 0.0f;
+  for (size_t i = 0; i < n; ++i) sum += g->poly[i];
+  std::vector<float> f(n);
+  for (size_t i = 0; i < n; ++i) {
+    float w = sum > 0.0f ? g->poly[i] / sum : 0.0f;
+    g->poly[i] = eps + (1.0f - eps) * w;
+    f[i] = g->poly[i] * ((g->x1[i] - g->x0[i]) * 1.0f * 1.0f);
+  }
+  g->fsum = 0.0f;
+  for (size_t i = 0; i < n; ++i) g->fsum += f[i];
+  g->dist.reset(new Dist1(f));
+  return g;
+}
+
 static BSample direct_sample(const Scene& s, const Cam& c, const std::vector<EdgeRec>& E, const Dist1& D,
-                             uint32_t seed, int B, int spp, uint64_t i, const float* dLdI) {
+                             const Guide* G, uint32_t seed, int B, int spp, uint64_t i, const float* dLdI) {
   BSample r;
   R4 rnd = rnd4(seed, kDomDirectEdge, i, 0u);
+  float pdf_b = 1.0f;
+  if (G) {
+    // AdaptiveQuadratureDistribution::sample (AQ_distrb.h:182-197)
+    float lp;
+    int leaf = G->dist->sample(rnd.d, &lp);
+    rnd.a = G->x0[leaf] + rnd.a * (G->x1[leaf] - G->x0[leaf]);
+    pdf_b = G->poly[leaf] / G->fsum;
+  }
   float pmf;
   int ei = D.sample(rnd.a, &pmf);
   const EdgeRec& e = E[ei];
@@ -1459,7 +1544,7 @@ static BSample direct_sample(const Scene& s, const Cam& c, const std::vector<Edg
   bool valid1 = (c0 > EDGE_EPSILON_ && c1 < -EDGE_EPSILON_) || (c0 < -EDGE_EPSILON_ && c1 > EDGE_EPSILON_);
   bool right = (e.boundary && valid0) || (!e.boundary && valid1);
   if (!(right && cosT > EPSILON_ && dist > SHADOW_EPSILON_)) return r;
-  float pdf = pdf_e * ls.pdf * d2 / cosT * 1.0f;
+  float pdf = pdf_e * ls.pdf * d2 / cosT * pdf_b;
   // EvalBoundarySegmentDirect (Edge.cpp:201-309)
   F3 rdir = normalize(p - ls.p);
   Hit h0 = intersect(s, p, -rdir, SHADOW_EPSILON_, 1e32f, false, false);
@@ -1823,7 +1908,8 @@ void oracle_path_twin(void* sp, const float* cam, int w, int h, uint32_t seed, i
 // direct, 3 secondary indirect.  dpos [V][3] += d/dV of the boundary terms
 // for samples [first, first + count) of spp * W * H.
 void oracle_boundary_adj(void* sp, const float* cam, int w, int h, uint32_t seed, int max_bounces, int kind, int spp,
-                         const float* dLdI, uint64_t first, uint64_t count, double* dpos, int threads) {
+                         const float* dLdI, uint64_t first, uint64_t count, double* dpos, int threads, int g_direct,
+                         int g_spp, float g_eps) {
   const Scene& s = *(Scene*)sp;
   Cam c = make_camera(cam, cam + 3, cam + 6, w, h, cam[9]);
   std::vector<EdgeRec> E;
@@ -1832,6 +1918,8 @@ void oracle_boundary_adj(void* sp, const float* cam, int w, int h, uint32_t seed
   if (kind != 1 && E.empty()) return;
   std::unique_ptr<Dist1> D;
   if (kind != 1) D.reset(new Dist1(edge_lengths(E)));
+  std::unique_ptr<Guide> G;
+  if (kind == 2 && g_direct) G = build_guide(s, E, *D, seed, g_spp, g_eps, threads);
   const uint64_t N = (uint64_t)spp * w * h;
   int nt = std::max(1, threads);
   std::vector<std::vector<double>> tp(nt, std::vector<double>(3 * (size_t)s.V, 0.0));
@@ -1843,7 +1931,7 @@ void oracle_boundary_adj(void* sp, const float* cam, int w, int h, uint32_t seed
       BSample bs;
       if (kind == 0) bs = primary_sample(s, c, E, *D, seed, max_bounces, spp, i, dLdI);
       else if (kind == 1) bs = pixel_sample(s, c, seed, max_bounces, spp, i, N, dLdI);
-      else if (kind == 2) bs = direct_sample(s, c, E, *D, seed, max_bounces, spp, i, dLdI);
+      else if (kind == 2) bs = direct_sample(s, c, E, *D, G.get(), seed, max_bounces, spp, i, dLdI);
       else bs = indirect_sample(s, c, E, *D, seed, max_bounces, spp, i, dLdI);
       if (bs.valid) boundary_backprop(s, bs, tape, tp[tid].data());
     }
@@ -1851,6 +1939,22 @@ void oracle_boundary_adj(void* sp, const float* cam, int w, int h, uint32_t seed
   });
   for (int t = 0; t < nt; ++t)
     for (size_t k = 0; k < tp[t].size(); ++k) dpos[k] += tp[t][k];
+}
+
+// Guiding weights of the direct boundary term: poly [E_sec] (after the eps
+// mix) and the leaf x ranges.
+void oracle_guide(void* sp, uint32_t seed, int g_spp, float g_eps, float* x0, float* x1, float* poly, int* n_out) {
+  const Scene& s = *(Scene*)sp;
+  std::vector<EdgeRec> E = sec_edges(s);
+  *n_out = (int)E.size();
+  if (E.empty()) return;
+  Dist1 D(edge_lengths(E));
+  std::unique_ptr<Guide> G = build_guide(s, E, D, seed, g_spp, g_eps, 8);
+  for (size_t i = 0; i < E.size(); ++i) {
+    if (x0) x0[i] = G->x0[i];
+    if (x1) x1[i] = G->x1[i];
+    if (poly) poly[i] = G->poly[i];
+  }
 }
 
 // Reverse mode: dpos [V][3] += d/dV sum_paths <dLdI[pix], path(pix)/spp>,

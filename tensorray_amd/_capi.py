@@ -31,7 +31,9 @@ class trc_camera(ctypes.Structure):
 class trc_options(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint32), ("max_bounces", ctypes.c_int32), ("spp", ctypes.c_int32),
                 ("sppe", ctypes.c_int32), ("sppse0", ctypes.c_int32), ("sppse1", ctypes.c_int32),
-                ("sppe0", ctypes.c_int32), ("quiet", ctypes.c_int32)]
+                ("sppe0", ctypes.c_int32), ("quiet", ctypes.c_int32), ("g_direct", ctypes.c_int32),
+                ("g_direct_depth", ctypes.c_int32), ("g_direct_max_size", ctypes.c_int32),
+                ("g_direct_spp", ctypes.c_int32), ("g_direct_thold", ctypes.c_float), ("g_eps", ctypes.c_float)]
 
 
 class trc_scene_info(ctypes.Structure):
@@ -150,8 +152,16 @@ def make_camera(width, height, pos=(0.0, 1.0, 3.0), target=(0.0, 0.0, 0.0), up=(
     return c
 
 
-def make_options(seed=1234, max_bounces=1, spp=1, sppe=0, sppse0=0, sppse1=0, sppe0=0, quiet=1):
+def make_options(seed=1234, max_bounces=1, spp=1, sppe=0, sppse0=0, sppse1=0, sppe0=0, quiet=1, g_direct=False,
+                 g_direct_depth=0, g_direct_max_size=100000, g_direct_spp=16, g_direct_thold=0.01, g_eps=0.01):
+    """RenderOptions with the reference's defaults (Integrator.h:47-78)."""
     o = trc_options()
+    o.g_direct = 1 if g_direct else 0
+    o.g_direct_depth = g_direct_depth
+    o.g_direct_max_size = g_direct_max_size
+    o.g_direct_spp = g_direct_spp
+    o.g_direct_thold = g_direct_thold
+    o.g_eps = g_eps
     o.seed = seed
     o.max_bounces = max_bounces
     o.spp = spp

@@ -93,6 +93,8 @@ void free_scene_bufs(trc_scene* s) {
   s->bufs.clear();
   if (s->prim_buf.ptr) hipFree(s->prim_buf.ptr);
   s->prim_buf = DevBuf{};
+  if (s->guide_buf.ptr) hipFree(s->guide_buf.ptr);
+  s->guide_buf = DevBuf{};
   s->device_bytes = 0;
   s->dev_texs = nullptr;
 }
@@ -531,8 +533,39 @@ trc_status trc_render_boundary(trc_scene* s, const trc_camera* cam, const trc_op
       prim.cdf = (const float*)(base + eb + fb);
     }
     if (kind >= 2 && s->dev.sec.n == 0) return;
+    GuideDev guide{};
+    if (kind == 2 && opt->g_direct) {
+      const int n = s->dev.sec.n;
+      if (opt->g_direct_spp < 1 || opt->g_direct_spp > 4096)
+        throw Fail(TRC_ERR_INVALID, "g_direct_spp must be in [1, 4096]");
+      if (!(opt->g_eps >= 0.0f && opt->g_eps <= 1.0f)) throw Fail(TRC_ERR_INVALID, "g_eps must be in [0, 1]");
+      if (opt->g_direct_max_size > 0 && n > opt->g_direct_max_size)
+        throw Fail(TRC_ERR_INVALID, "guiding: " + std::to_string(n) + " secondary edges exceed g_direct_max_size " +
+                                        std::to_string(opt->g_direct_max_size));
+      const size_t nv = (size_t)n * (size_t)opt->g_direct_spp;
+      const size_t need = (nv + 2 * (size_t)n + 1) * sizeof(float);
+      if (s->guide_buf.bytes < need) {
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        if (s->guide_buf.ptr) hipFree(s->guide_buf.ptr);
+        s->guide_buf.ptr = nullptr;
+        s->guide_buf.bytes = 0;
+        hip_check(hipMalloc(&s->guide_buf.ptr, need), "hipMalloc(guide)");
+        s->guide_buf.bytes = need;
+      }
+      float* base = (float*)s->guide_buf.ptr;
+      guide.vals = base;
+      guide.poly = base + nv;
+      guide.cdf = base + nv + n;
+      guide.fsum = base + nv + 2 * (size_t)n;
+      guide.n = n;
+      guide.spp = opt->g_direct_spp;
+      guide.eps = opt->g_eps;
+      guide.enabled = 1;
+      LaunchCfg Lg = make_cfg(s->ctx, s, st, (long long)nv);
+      hip_check(launch_guide_build(s->dev, Lg, opt->seed, guide, st), "k_guide");
+    }
     LaunchCfg L = make_cfg(s->ctx, s, st, N);
-    hip_check(launch_boundary(kind, s->dev, C, opt->seed, opt->max_bounces, spp, L, prim, dLdI, dpos, st),
+    hip_check(launch_boundary(kind, s->dev, C, opt->seed, opt->max_bounces, spp, L, prim, guide, dLdI, dpos, st),
               "k_boundary");
     if (kind == 0) hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
   });

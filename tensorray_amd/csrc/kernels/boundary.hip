@@ -364,8 +364,113 @@ __global__ void __launch_bounds__(kBlock) k_boundary_pixel(SceneDev S, CameraDev
 // Secondary edges, direct illumination (BoundaryDirect.cpp:38-143,
 // Edge.cpp:108-144 and 201-309)
 // ---------------------------------------------------------------------------
+// Guide evaluation (AdaptiveQuadratureDistribution::setup and
+// GuideDirectBoundary_Helper, AQ_distrb.h:63-76 and 100-175): sample k of
+// leaf i draws x uniformly in the leaf's CDF interval and (y, z) in [0,1]^2
+// and records |baseVal| / spp of the guiding-mode evaluation, where only
+// the two hit tests and the same-point test of Edge.cpp:201-262 apply.
+__global__ void __launch_bounds__(kBlock) k_guide_eval(SceneDev S, LaunchCfg L, uint32_t seed, GuideDev G) {
+  __shared__ uint32_t lds_stack[kStackLds * kBlock];
+  TraceCtx tc = make_trace_ctx_b(lds_stack, L);
+  const EdgeSet& E = S.sec;
+  const uint64_t N = (uint64_t)G.n * (uint64_t)G.spp;
+  const uint64_t stride = (uint64_t)L.grid * kBlock;
+  for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < N; j += stride) {
+    const int32_t leaf = (int32_t)(j / (uint64_t)G.spp);
+    Rnd4 r = rng4(seed, kRngGuide, j, 0u);
+    const float x0 = leaf > 0 ? E.cdf[leaf - 1] : 0.0f, x1 = E.cdf[leaf];
+    const float x = x0 + r.a * (x1 - x0);
+    float pdf_e;
+    int32_t ei = sample_edge(E, x, &pdf_e);
+    const EdgeDev ed = E.e[ei];
+    EdgeGeo g = edge_geo(S, ed);
+    float t = cdf_reuse(E.cdf, ei, x);
+    F3 p = g.p0 + t * g.e1;
+    F3 edge = normalize(g.e1);
+    F3 edge2 = normalize(g.p2 - g.p0);
+    LightSample ls = light_sample(S, r.b, r.c);
+    F3 rdir = normalize(p - ls.p);
+    float v = 0.0f;
+    Hit h0 = trace_ray<false>(S, tc, p, -rdir, kShadowEpsilon, 1e32f);
+    if (h0.tri >= 0) {
+      Hit h = trace_ray<false>(S, tc, p, rdir, kShadowEpsilon, 1e32f);
+      F3 hitP = p + h0.t * (-rdir);
+      if (h.tri >= 0 && length(hitP - ls.p) < kShadowEpsilon) {
+        Its its = post_intersect(S, h.tri, h.u, h.v);
+        F3 n;
+        v = fabsf(base_val_secondary(its.p, ls.p, ls.n, rdir, p, edge, edge2, &n));
+        if (!isfinite(v)) v = 0.0f;
+      }
+    }
+    G.vals[j] = v / (float)G.spp;
+  }
+}
+
+// Guide normalization (AQ_distrb.h:166-180): poly_i = sum_k vals in sample
+// order, w = poly / sum(poly), poly = eps + (1 - eps) w, f = poly * volume,
+// then the Distribution1D CDF of f (Distribution.cpp:45-60).  The sums run
+// sequentially in index order on thread 0 over LDS-staged chunks, so the
+// result does not depend on the launch shape.
+constexpr int kGuideBlock = 1024;
+constexpr int kGuideChunk = 8192;
+__global__ void __launch_bounds__(kGuideBlock) k_guide_finish(const float* sec_cdf, GuideDev G) {
+  __shared__ float chunk[kGuideChunk];
+  __shared__ float total;
+  const int n = G.n;
+  for (int i = threadIdx.x; i < n; i += kGuideBlock) {
+    float acc = 0.0f;
+    const float* v = G.vals + (size_t)i * (size_t)G.spp;
+    for (int k = 0; k < G.spp; ++k) acc = acc + v[k];
+    G.poly[i] = acc;
+  }
+  __syncthreads();
+  float run = 0.0f;  // thread 0 only
+  for (int c0 = 0; c0 < n; c0 += kGuideChunk) {
+    const int m = min(kGuideChunk, n - c0);
+    for (int i = threadIdx.x; i < m; i += kGuideBlock) chunk[i] = G.poly[c0 + i];
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int i = 0; i < m; ++i) run += chunk[i];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) total = run;
+  __syncthreads();
+  const float sum = total;
+  for (int i = threadIdx.x; i < n; i += kGuideBlock) {
+    const float w = sum > 0.0f ? G.poly[i] / sum : 0.0f;
+    const float poly = G.eps + (1.0f - G.eps) * w;
+    const float x0 = i > 0 ? sec_cdf[i - 1] : 0.0f;
+    G.poly[i] = poly;
+    G.cdf[i] = poly * ((sec_cdf[i] - x0) * 1.0f * 1.0f);
+  }
+  __syncthreads();
+  const float inv = 1.0f / (float)n;
+  run = 0.0f;
+  for (int c0 = 0; c0 < n; c0 += kGuideChunk) {
+    const int m = min(kGuideChunk, n - c0);
+    for (int i = threadIdx.x; i < m; i += kGuideBlock) chunk[i] = G.cdf[c0 + i];
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int i = 0; i < m; ++i) {
+        run += chunk[i];
+        chunk[i] = run * inv;
+      }
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += kGuideBlock) G.cdf[c0 + i] = chunk[i];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    total = G.cdf[n - 1];
+    G.fsum[0] = run;
+  }
+  __syncthreads();
+  const float integral = total;
+  if (integral > 0.0f)
+    for (int i = threadIdx.x; i < n; i += kGuideBlock) G.cdf[i] = G.cdf[i] / integral;
+}
+
 __global__ void __launch_bounds__(kBlock) k_boundary_direct(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
-                                                            uint64_t N, const float* dLdI, float* dpos) {
+                                                            GuideDev G, uint64_t N, const float* dLdI, float* dpos) {
   __shared__ uint32_t lds_stack[kStackLds * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
   const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
@@ -373,6 +478,14 @@ __global__ void __launch_bounds__(kBlock) k_boundary_direct(SceneDev S, CameraDe
   const uint64_t stride = (uint64_t)L.grid * kBlock;
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
     Rnd4 rnd = rng4(P.seed, kRngDirectEdge, i, 0u);
+    float pdf_b = 1.0f;
+    if (G.enabled) {
+      // AdaptiveQuadratureDistribution::sample (AQ_distrb.h:182-197)
+      const int32_t leaf = cdf_search(G.cdf, G.n, rnd.d);
+      const float x0 = leaf > 0 ? E.cdf[leaf - 1] : 0.0f, x1 = E.cdf[leaf];
+      rnd.a = x0 + rnd.a * (x1 - x0);
+      pdf_b = G.poly[leaf] / G.fsum[0];
+    }
     float pdf_e;
     int32_t ei = sample_edge(E, rnd.a, &pdf_e);
     const EdgeDev ed = E.e[ei];
@@ -391,7 +504,7 @@ __global__ void __launch_bounds__(kBlock) k_boundary_direct(SceneDev S, CameraDe
     bool valid1 = (c0 > kEdgeEpsilon && c1 < -kEdgeEpsilon) || (c0 < -kEdgeEpsilon && c1 > kEdgeEpsilon);
     bool right = (g.boundary && valid0) || (!g.boundary && valid1);
     if (!(right && cosT > kEpsilon && dist > kShadowEpsilon)) continue;
-    float pdf = pdf_e * ls.pdf * d2 / cosT * 1.0f;
+    float pdf = pdf_e * ls.pdf * d2 / cosT * pdf_b;
     F3 rdir = normalize(p - ls.p);
     Hit h0 = trace_ray<false>(S, tc, p, -rdir, kShadowEpsilon, 1e32f);
     if (h0.tri < 0) continue;
@@ -516,8 +629,18 @@ __global__ void __launch_bounds__(kBlock) k_boundary_indirect(SceneDev S, Camera
   }
 }
 
+hipError_t launch_guide_build(const SceneDev& S, const LaunchCfg& L, uint32_t seed, const GuideDev& g,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(k_guide_eval, dim3(L.grid), dim3(kBlock), 0, st, S, L, seed, g);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_guide_finish, dim3(1), dim3(kGuideBlock), 0, st, S.sec.cdf, g);
+  return hipGetLastError();
+}
+
 hipError_t launch_boundary(int kind, const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
-                           const LaunchCfg& L, const EdgeSet& prim, const float* dLdI, float* dpos, hipStream_t st) {
+                           const LaunchCfg& L, const EdgeSet& prim, const GuideDev& guide, const float* dLdI,
+                           float* dpos, hipStream_t st) {
   BoundaryParams P{seed, max_bounces, spp, 0u};
   uint64_t N = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
   if (kind == 0)
@@ -525,7 +648,7 @@ hipError_t launch_boundary(int kind, const SceneDev& S, const CameraDev& C, uint
   else if (kind == 1)
     hipLaunchKernelGGL(k_boundary_pixel, dim3(L.grid), dim3(kBlock), 0, st, S, C, P, L, N, dLdI, dpos);
   else if (kind == 2)
-    hipLaunchKernelGGL(k_boundary_direct, dim3(L.grid), dim3(kBlock), 0, st, S, C, P, L, N, dLdI, dpos);
+    hipLaunchKernelGGL(k_boundary_direct, dim3(L.grid), dim3(kBlock), 0, st, S, C, P, L, guide, N, dLdI, dpos);
   else
     hipLaunchKernelGGL(k_boundary_indirect, dim3(L.grid), dim3(kBlock), 0, st, S, C, P, L, N, dLdI, dpos);
   return hipGetLastError();

@@ -39,6 +39,9 @@ struct trc_scene {
   std::vector<trc::EdgeDev> prim_edges;
   std::vector<float> prim_len, prim_cdf;
   DevBuf prim_buf;
+  // guiding of the direct boundary term: per-sample values, leaf weights,
+  // leaf CDF and the weight integral
+  DevBuf guide_buf;
 };
 
 namespace trc {
@@ -61,9 +64,24 @@ hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t s
 hipError_t launch_render_stats(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                                const LaunchCfg& L, unsigned long long* counters, hipStream_t st);
 hipError_t launch_film_reduce(const float* paths, int spp, int wh, float* film, hipStream_t st);
+// Guide of the direct boundary term: one leaf per secondary edge i over
+// x in [sec.cdf[i-1], sec.cdf[i]], y, z in [0, 1].
+struct GuideDev {
+  float* vals;  // [n * spp] per-sample |baseVal| / spp
+  float* poly;  // [n] leaf weights after the eps mix
+  float* cdf;   // [n] leaf CDF (weight * volume)
+  float* fsum;  // [1] sum of weight * volume
+  int32_t n, spp;
+  float eps;
+  int32_t enabled;
+};
+// Builds the guide on `st` (two kernels); g.enabled is ignored.
+hipError_t launch_guide_build(const SceneDev& S, const LaunchCfg& L, uint32_t seed, const GuideDev& g,
+                              hipStream_t st);
 // kind: 0 primary edges, 1 pixel boundary, 2 secondary direct, 3 secondary indirect
 hipError_t launch_boundary(int kind, const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
-                           const LaunchCfg& L, const EdgeSet& prim, const float* dLdI, float* dpos, hipStream_t st);
+                           const LaunchCfg& L, const EdgeSet& prim, const GuideDev& guide, const float* dLdI,
+                           float* dpos, hipStream_t st);
 hipError_t launch_render_adj(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                              const LaunchCfg& L, const float* dLdI, float* dpos, float* dmat, hipStream_t st);
 

@@ -234,3 +234,36 @@ def test_boundary_gradients_match_oracle(gpu_ctx, oracle_mod, kind):
     ok, e = _grad_close(got[mask], ref[mask])
     assert ok, e
     ds.close()
+
+
+def test_guided_direct_matches_oracle(gpu_ctx, oracle_mod):
+    """Guided sampling of the direct boundary term (AQ_distrb.h): guide build
+    and guided samples against the oracle."""
+    torch = _torch()
+    spec = scenes.sphere_scene(subdiv=2)
+    W = H = 48
+    ds = _capi.DeviceScene(gpu_ctx, spec)
+    info = ds.info()
+    cam_d = scenes.CAMERA_DEFAULT
+    cam = _capi.make_camera(W, H, **cam_d)
+    opt = _capi.make_options(seed=31, max_bounces=2, spp=1, sppse0=8, g_direct=True, g_direct_spp=8, g_eps=0.1)
+    dl = np.random.default_rng(2).uniform(0.2, 1.0, (3, W * H)).astype(np.float32)
+    dpos = torch.zeros(info.num_vertices, 3, device="cuda")
+    ds.render_boundary(cam, opt, "direct", torch.from_numpy(dl).cuda().data_ptr(), dpos.data_ptr())
+    torch.cuda.synchronize()
+    S = oracle_mod.Scene(spec)
+    ref = oracle_mod.boundary_adj(S, cam_d, W, H, 31, 2, "direct", 8, dl, guide=(8, 0.1))
+    unguided = oracle_mod.boundary_adj(S, cam_d, W, H, 31, 2, "direct", 8, dl)
+    mask = spec.diff_vertex_mask()
+    got = dpos.cpu().numpy().astype(np.float64)
+    assert not np.allclose(ref[mask], unguided[mask])  # guiding changes the samples
+    ok, e = _grad_close(got[mask], ref[mask])
+    assert ok, e
+    # guide limits
+    small = _capi.make_options(seed=31, max_bounces=2, spp=1, sppse0=1, g_direct=True, g_direct_max_size=10)
+    with pytest.raises(_capi.TRCError, match="g_direct_max_size"):
+        ds.render_boundary(cam, small, "direct", torch.from_numpy(dl).cuda().data_ptr(), dpos.data_ptr())
+    bad = _capi.make_options(seed=31, max_bounces=2, spp=1, sppse0=1, g_direct=True, g_eps=1.5)
+    with pytest.raises(_capi.TRCError, match="g_eps"):
+        ds.render_boundary(cam, bad, "direct", torch.from_numpy(dl).cuda().data_ptr(), dpos.data_ptr())
+    ds.close()
