@@ -16,7 +16,8 @@
  * planes [3][H*W] of float, pixel index = y * W + x.
  * Vertex gradients: [V][3] float, V = total vertex count of the scene, in
  * shape order.  Material gradients: [M][8] float per BSDF, slots
- * {roughness, albedo/kd r, g, b, specular/F0 r, g, b, unused}.
+ * {roughness, albedo/kd r, g, b, specular/F0 r, g, b, unused}; for a rough
+ * conductor {roughness, eta r, g, b, k r, g, b, unused}.
  */
 #ifndef TRC_H
 #define TRC_H
@@ -42,6 +43,7 @@ typedef void* trc_stream;
 /* Material types (BSDF classes of the reference). */
 #define TRC_MAT_DIFFUSE 0      /* Diffuse.cpp:37-85 */
 #define TRC_MAT_MICROFACET 1   /* Microfacet.cpp:37-165 */
+#define TRC_MAT_ROUGHCONDUCTOR 2 /* Roughconductor.cpp:37-121; albedo = eta, spec = k, no texture */
 
 /* Mesh flags. */
 #define TRC_MESH_SMOOTH 1      /* smooth shading from averaged face normals (Primitive.cpp:323-340) */

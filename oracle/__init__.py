@@ -101,6 +101,28 @@ def exp2(x):
     return y
 
 
+def ggx_d(z, alpha):
+    """BSDF::GGX_D at half vectors with cos(theta_h) = z."""
+    z = np.ascontiguousarray(z, np.float32)
+    y = np.zeros_like(z)
+    L = lib()
+    L.oracle_ggx_d.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_float,
+                               ctypes.POINTER(ctypes.c_float)]
+    L.oracle_ggx_d(_p(z, ctypes.c_float), len(z), alpha, _p(y, ctypes.c_float))
+    return y
+
+
+def fresnel_conductor(ci, eta, k):
+    """BSDF::FresnelConductor for one channel."""
+    ci = np.ascontiguousarray(ci, np.float32)
+    y = np.zeros_like(ci)
+    L = lib()
+    L.oracle_fresnel_conductor.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_float,
+                                           ctypes.c_float, ctypes.POINTER(ctypes.c_float)]
+    L.oracle_fresnel_conductor(_p(ci, ctypes.c_float), len(ci), eta, k, _p(y, ctypes.c_float))
+    return y
+
+
 def camera_params(cam, w, h):
     """cam = dict(pos, target, up, fov).  Returns 19 floats: eye, right, up,
     dir, ax, bx, ay, by, tan_half, ratio, inv_area."""

@@ -799,6 +799,48 @@ static F3 VNDF_sample(F3 v, float a, float s0, float s1) {
 }
 static F3 reflect_(F3 i, F3 n) { return i + (2.0f * dot(-i, n)) * n; }
 
+// ---- GGX and the conductor Fresnel term, templated for AD (BSDF.cpp:232-323) ----
+template <class S> static S tan2_t(V3<S> v) {
+  S c2 = v.z * v.z;
+  S om = S(1.0f) - c2;
+  if (!(val(om) > 0.0f)) om = S(0.0f);
+  return om / c2;
+}
+template <class S> static S safe_sqrt_t(S x) { return val(x) > 0.0f ? sqrt_(x) : S(0.0f); }
+template <class S> static S GGX_D_t(V3<S> wh, S a) {
+  S t2 = tan2_t(wh);
+  bool ok = val(t2) < 3.402823466e38f;
+  S c2 = wh.z * wh.z;
+  S e = S(1.0f) / (a * a) * t2;
+  S root = a * c2 * (S(1.0f) + e);
+  S D = S(INV_PI) / (root * root);
+  return ok ? D : S(0.0f);
+}
+template <class S> static S Smith_G_t(V3<S> v, V3<S> wh, S a) {
+  S t2 = tan2_t(wh);
+  if (!(val(t2) < 3.402823466e38f)) return S(0.0f);
+  if (val(t2) == 0.0f) return (val(dot(v, wh)) * val(v.z)) <= 0.0f ? S(0.0f) : S(1.0f);
+  if ((val(dot(v, wh)) * val(v.z)) <= 0.0f) return S(0.0f);
+  S root = (a * a) * t2;
+  return S(2.0f) / (S(1.0f) + sqrt_(S(1.0f) + root * root));
+}
+// BSDF::FresnelConductor (BSDF.cpp:304-323), one channel.
+template <class S> static S fresnel_conductor(S ci, S eta, S k) {
+  S ci2 = ci * ci;
+  S si2 = S(1.0f) - ci2;
+  S si4 = si2 * si2;
+  S eta2 = eta * eta, k2 = k * k;
+  S temp1 = eta2 - k2 - si2;
+  S a2pb2 = safe_sqrt_t(temp1 * temp1 + k2 * eta2 * S(4.0f));
+  S a = safe_sqrt_t((a2pb2 + temp1) * S(0.5f));
+  S term1 = a2pb2 + ci2;
+  S term2 = a * (S(2.0f) * ci);
+  S Rs2 = (term1 - term2) / (term1 + term2);
+  S term3 = a2pb2 * ci2 + si4, term4 = term2 * si2;
+  S Rp2 = Rs2 * (term3 - term4) / (term3 + term4);
+  return S(0.5f) * (Rp2 + Rs2);
+}
+
 // BSDF value (BSDF::Eval without correction), templated for AD.
 template <class S>
 static V3<S> bsdf_eval(Access<S>& A, const Its<S>& it, V3<S> wo_w, V3<S> wi_w) {
@@ -808,6 +850,23 @@ static V3<S> bsdf_eval(Access<S>& A, const Its<S>& it, V3<S> wo_w, V3<S> wi_w) {
     bool refl = val(wo.z) > 0.0f && (val(wo.z) * val(wi.z)) > 0.0f;
     S inner = S(INV_PI) * S(refl ? 1.0f : 0.0f);
     return albedo_at(A, it) * inner;
+  }
+  if (m.type == 2) {
+    // RoughConductor::EvalInner (Roughconductor.cpp:37-57); the albedo and
+    // spec slots hold eta and k.
+    V3<S> zero{S(0.0f), S(0.0f), S(0.0f)};
+    if (!(val(wo.z) > 0.0f && val(wi.z) > 0.0f)) return zero;
+    V3<S> wh = normalize(wo + wi);
+    S r = A.rough(it.mat);
+    S D = GGX_D_t(wh, r);
+    if (val(D) == 0.0f) return zero;
+    S G = Smith_G_t(wo, wh, r) * Smith_G_t(wi, wh, r);
+    S coso = dot(wo, wh);
+    V3<S> eta = A.albedo(it.mat), k = A.spec(it.mat);
+    V3<S> Fr{fresnel_conductor(coso, eta.x, k.x), fresnel_conductor(coso, eta.y, k.y),
+             fresnel_conductor(coso, eta.z, k.z)};
+    S den = S(4.0f) * abs_(wi.z) * abs_(wo.z);
+    return Fr * D * G / den;
   }
   // Microfacet::EvalInner (Microfacet.cpp:37-73)
   S nv = wo.z, nl = wi.z;
@@ -1761,6 +1820,16 @@ void oracle_sincos(const float* x, int n, float* s, float* c) {
 }
 void oracle_exp2(const float* x, int n, float* y) {
   for (int i = 0; i < n; ++i) y[i] = f_exp2(x[i]);
+}
+// GGX_D at the half vector (sqrt(1 - z^2), 0, z) and roughness a.
+void oracle_ggx_d(const float* z, int n, float a, float* out) {
+  for (int i = 0; i < n; ++i) {
+    F3 wh{sqrtf(std::max(0.0f, 1.0f - z[i] * z[i])), 0.0f, z[i]};
+    out[i] = GGX_D_t<float>(wh, a);
+  }
+}
+void oracle_fresnel_conductor(const float* ci, int n, float eta, float k, float* out) {
+  for (int i = 0; i < n; ++i) out[i] = fresnel_conductor<float>(ci[i], eta, k);
 }
 
 // materials: [nmat][12] = type, tex, roughness, unused, albedo rgb, spec rgb, unused x2

@@ -95,6 +95,7 @@ void free_scene_bufs(trc_scene* s) {
   s->prim_buf = DevBuf{};
   if (s->guide_buf.ptr) hipFree(s->guide_buf.ptr);
   s->guide_buf = DevBuf{};
+  s->vn_adj = nullptr;
   s->device_bytes = 0;
   s->dev_texs = nullptr;
 }
@@ -135,6 +136,26 @@ void upload_scene(trc_scene* s, hipStream_t st) {
   d.n_tris = h.num_triangles();
   d.n_mats = (int32_t)h.materials.size();
   d.stack_need = h.bvh_stats.stack_need;
+  d.n_verts = h.num_vertices();
+  // smooth-shaded differentiable faces per vertex, in triangle order
+  std::vector<uint32_t> vf_off(h.num_vertices() + 1, 0u), vf_list;
+  const uint32_t T = (uint32_t)h.num_triangles();
+  auto smooth_diff = [&](uint32_t t) { return (h.shade[t].flags & (kTriSmooth | kTriDiff)) == (kTriSmooth | kTriDiff); };
+  for (uint32_t t = 0; t < T; ++t)
+    if (smooth_diff(t))
+      for (int j = 0; j < 3; ++j) vf_off[h.idx[3 * t + j] + 1]++;
+  for (size_t v = 0; v + 1 < vf_off.size(); ++v) vf_off[v + 1] += vf_off[v];
+  if (vf_off.back() > 0) {
+    vf_list.resize(vf_off.back());
+    std::vector<uint32_t> fill(vf_off.begin(), vf_off.end() - 1);
+    for (uint32_t t = 0; t < T; ++t)
+      if (smooth_diff(t))
+        for (int j = 0; j < 3; ++j) vf_list[fill[h.idx[3 * t + j]]++] = t;
+    d.vf_off = upload(s, vf_off, st);
+    d.vf_list = upload(s, vf_list, st);
+    std::vector<float> zeros(3 * (size_t)h.num_vertices(), 0.0f);
+    s->vn_adj = (float*)upload(s, zeros, st);
+  }
   s->dev = d;
   hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
 }
@@ -242,8 +263,12 @@ trc_status trc_scene_destroy(trc_scene* s) {
 int32_t trc_scene_add_material(trc_scene* s, int32_t type, const float albedo[3], const float spec[3],
                                float roughness, int32_t texture) {
   if (!s) return -1;
-  if (type != TRC_MAT_DIFFUSE && type != TRC_MAT_MICROFACET) {
+  if (type != TRC_MAT_DIFFUSE && type != TRC_MAT_MICROFACET && type != TRC_MAT_ROUGHCONDUCTOR) {
     s->ctx->last_error = "unknown material type";
+    return -1;
+  }
+  if (type == TRC_MAT_ROUGHCONDUCTOR && texture >= 0) {
+    s->ctx->last_error = "roughconductor takes no texture";
     return -1;
   }
   Material m{};
@@ -253,6 +278,8 @@ int32_t trc_scene_add_material(trc_scene* s, int32_t type, const float albedo[3]
   for (int k = 0; k < 3; ++k) {
     m.albedo[k] = albedo ? albedo[k] : 0.0f;
     m.spec[k] = spec ? spec[k] : 0.0f;
+    m.eta[k] = m.albedo[k];  // rough conductor: albedo slot = eta, spec slot = k
+    m.k[k] = m.spec[k];
   }
   s->host.materials.push_back(m);
   s->configured = false;
@@ -365,6 +392,8 @@ trc_status trc_scene_set_material(trc_scene* s, int32_t mi, const float albedo[3
     for (int k = 0; k < 3; ++k) {
       if (albedo) m.albedo[k] = albedo[k];
       if (spec) m.spec[k] = spec[k];
+      m.eta[k] = m.albedo[k];
+      m.k[k] = m.spec[k];
     }
     m.roughness = roughness;
     if (s->configured) {
@@ -580,7 +609,7 @@ trc_status trc_render_adj(trc_scene* s, const trc_camera* cam, const trc_options
     if (!dLdI || !dpos) throw Fail(TRC_ERR_INVALID, "dLdI / dpos is NULL");
     long long np = (long long)opt->spp * C.width * C.height;
     LaunchCfg L = make_cfg(s->ctx, s, (hipStream_t)stream, np);
-    hip_check(launch_render_adj(s->dev, C, opt->seed, opt->max_bounces, opt->spp, L, dLdI, dpos, dmat,
+    hip_check(launch_render_adj(s->dev, C, opt->seed, opt->max_bounces, opt->spp, L, dLdI, dpos, dmat, s->vn_adj,
                                 (hipStream_t)stream),
               "k_render_adj");
   });

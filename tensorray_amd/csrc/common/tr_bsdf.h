@@ -225,7 +225,41 @@ TR_HD F3 microfacet_inner(F3 wo, F3 wi, float r, F3 kd, F3 F0) {
   return active ? val : F3{0.0f, 0.0f, 0.0f};
 }
 
-// Microfacet::PdfInner (Microfacet.cpp:152-165)
+// BSDF::FresnelConductor (BSDF.cpp:304-323), one channel.
+TR_HD float fresnel_conductor(float ci, float eta, float k) {
+  float ci2 = ci * ci;
+  float si2 = 1.0f - ci2;
+  float si4 = si2 * si2;
+  float eta2 = eta * eta, k2 = k * k;
+  float temp1 = eta2 - k2 - si2;
+  float a2pb2 = safe_sqrt(temp1 * temp1 + k2 * eta2 * 4.0f);
+  float a = safe_sqrt((a2pb2 + temp1) * 0.5f);
+  float term1 = a2pb2 + ci2;
+  float term2 = a * (2.0f * ci);
+  float Rs2 = (term1 - term2) / (term1 + term2);
+  float term3 = a2pb2 * ci2 + si4, term4 = term2 * si2;
+  float Rp2 = Rs2 * (term3 - term4) / (term3 + term4);
+  return 0.5f * (Rp2 + Rs2);
+}
+
+// RoughConductor::EvalInner (Roughconductor.cpp:37-57): GGX with alpha =
+// roughness (no squaring, unlike Microfacet), conductor Fresnel.
+TR_HD F3 roughconductor_inner(F3 wo, F3 wi, float r, F3 eta, F3 k) {
+  const F3 zero{0.0f, 0.0f, 0.0f};
+  if (!(wo.z > 0.0f && wi.z > 0.0f)) return zero;
+  F3 wh = normalize(wo + wi);
+  float D = ggx_d(wh, r);
+  if (D == 0.0f) return zero;
+  float G = smith_g(wo, wh, r) * smith_g(wi, wh, r);
+  float coso = dot(wo, wh);
+  F3 Fr{fresnel_conductor(coso, eta.x, k.x), fresnel_conductor(coso, eta.y, k.y),
+        fresnel_conductor(coso, eta.z, k.z)};
+  float den = 4.0f * fabsf(wi.z) * fabsf(wo.z);
+  return Fr * D * G / den;
+}
+
+// Microfacet::PdfInner (Microfacet.cpp:152-165); RoughConductor::PdfInner
+// (Roughconductor.cpp:107-121) is the same expression.
 TR_HD float microfacet_pdf(F3 wo, F3 wi, float r) {
   F3 wh = normalize(wo + wi);
   float mp = ggx_vndf_pdf(wo, wh, r);
@@ -248,6 +282,8 @@ TR_HD F3 bsdf_eval(const SceneDev& S, const Its& s, F3 wo_w, F3 wi_w) {
   if (m.type == kMatDiffuse) {
     return mat_albedo(S, m, s) * diffuse_inner(wo, wi);
   }
+  if (m.type == kMatRoughConductor)
+    return roughconductor_inner(wo, wi, m.roughness, F3{m.eta[0], m.eta[1], m.eta[2]}, F3{m.k[0], m.k[1], m.k[2]});
   return microfacet_inner(wo, wi, m.roughness, mat_albedo(S, m, s), F3{m.spec[0], m.spec[1], m.spec[2]});
 }
 
@@ -262,8 +298,9 @@ TR_HD bool bsdf_dir_valid(const SceneDev& S, const Its& s, F3 w) {
   return S.mats[s.mat].type == kMatDiffuse ? diffuse_dir_valid(s, w) : base_dir_valid(s, w);
 }
 
-// SampleOnly (Diffuse.cpp:66-78, Microfacet.cpp:127-150): world direction
-// and its solid-angle pdf.
+// SampleOnly (Diffuse.cpp:66-78, Microfacet.cpp:127-150,
+// Roughconductor.cpp:83-105; the last two are the same VNDF draw): world
+// direction and its solid-angle pdf.
 TR_HD F3 bsdf_sample(const SceneDev& S, const Its& s, F3 wo_w, float s0, float s1, float* pdf) {
   const Material& m = S.mats[s.mat];
   F3 wo = to_local(s, wo_w);

@@ -155,7 +155,12 @@ struct SceneDev {
   int32_t n_tris;
   int32_t n_mats;
   int32_t stack_need;         // worst-case traversal stack depth of this BVH
-  int32_t pad;
+  int32_t n_verts;
+  // smooth-shaded faces incident to each vertex, in triangle order (CSR:
+  // faces of vertex v are vf_list[vf_off[v] .. vf_off[v+1])), for the
+  // adjoint of the vertex normals; null when no differentiable mesh is smooth
+  const uint32_t* vf_off;
+  const uint32_t* vf_list;
 };
 
 }  // namespace trc

@@ -21,7 +21,8 @@
 
 namespace trc {
 
-// Material parameter gradient slots: [roughness, albedo rgb, spec rgb, pad].
+// Material parameter gradient slots: [roughness, albedo rgb, spec rgb, pad];
+// for a rough conductor [roughness, eta rgb, k rgb, pad].
 constexpr int kMatGradStride = 8;
 constexpr int kMaxMatGrad = 32;
 
@@ -149,6 +150,121 @@ TR_HD void microfacet_inner_vjp(F3 wo, F3 wi, float r, F3 kd, F3 F0, F3 adj_f, F
   *adj_wi = F3{0.0f, 0.0f, adj_nl} + ahr;
 }
 
+// d tan^2(theta) / d z for tan^2 = max(1 - z^2, 0) / z^2.
+TR_HD float tan2_dz(float z) {
+  float c2 = z * z;
+  return (1.0f - c2) > 0.0f ? -2.0f * z / (c2 * c2) : 0.0f;
+}
+
+// VJP of ggx_d(wh, a) (value D) into wh.z and a.
+TR_HD void ggx_d_vjp(float z, float a, float D, float gD, float* g_z, float* g_a) {
+  float c2 = z * z;
+  float om = 1.0f - c2;
+  float t2 = (om > 0.0f ? om : 0.0f) / c2;
+  float e = 1.0f / (a * a) * t2;
+  float root = a * c2 * (1.0f + e);
+  float g_root = gD * (-2.0f * D / root);
+  float g_e = g_root * a * c2;
+  *g_a += g_root * c2 * (1.0f + e) + g_e * (-2.0f * t2 / (a * a * a));
+  float g_t2 = g_e / (a * a);
+  *g_z += g_root * a * (1.0f + e) * 2.0f * z + g_t2 * tan2_dz(z);
+}
+
+// VJP of smith_g(v, wh, a) into wh.z and a (v enters only through masks).
+TR_HD void smith_g_vjp(F3 v, F3 wh, float a, float gG, float* g_z, float* g_a) {
+  float t2 = tan_theta2(wh);
+  if (!(t2 < 3.402823466e38f) || t2 == 0.0f || (dot(v, wh) * v.z) <= 0.0f) return;
+  float root = (a * a) * t2;
+  float sq = sqrtf(1.0f + root * root);
+  float g_root = gG * (-2.0f / ((1.0f + sq) * (1.0f + sq))) * root / sq;
+  *g_a += g_root * 2.0f * a * t2;
+  *g_z += g_root * (a * a) * tan2_dz(wh.z);
+}
+
+// VJP of fresnel_conductor(ci, eta, k) (BSDF.cpp:304-323).
+TR_HD void fresnel_conductor_vjp(float ci, float eta, float k, float gF, float* g_ci, float* g_eta, float* g_k) {
+  float ci2 = ci * ci;
+  float si2 = 1.0f - ci2;
+  float si4 = si2 * si2;
+  float eta2 = eta * eta, k2 = k * k;
+  float temp1 = eta2 - k2 - si2;
+  float q = temp1 * temp1 + k2 * eta2 * 4.0f;
+  float a2pb2 = safe_sqrt(q);
+  float p = (a2pb2 + temp1) * 0.5f;
+  float a = safe_sqrt(p);
+  float term1 = a2pb2 + ci2;
+  float term2 = a * (2.0f * ci);
+  float U = term1 - term2, W = term1 + term2;
+  float Rs2 = U / W;
+  float term3 = a2pb2 * ci2 + si4, term4 = term2 * si2;
+  float X = term3 - term4, Y = term3 + term4;
+  float Rp2 = Rs2 * X / Y;
+  float g_Rp2 = 0.5f * gF, g_Rs2 = 0.5f * gF;
+  g_Rs2 += g_Rp2 * X / Y;
+  float g_X = g_Rp2 * Rs2 / Y, g_Y = -g_Rp2 * Rp2 / Y;
+  float g_term3 = g_X + g_Y, g_term4 = g_Y - g_X;
+  float g_U = g_Rs2 / W, g_W = -g_Rs2 * Rs2 / W;
+  float g_term1 = g_U + g_W, g_term2 = g_W - g_U;
+  g_term2 += g_term4 * si2;
+  float g_si2 = g_term4 * term2;
+  float g_a2pb2 = g_term3 * ci2 + g_term1;
+  float g_ci2 = g_term3 * a2pb2 + g_term1;
+  float g_si4 = g_term3;
+  float g_a = g_term2 * 2.0f * ci;
+  float gci = g_term2 * 2.0f * a;
+  float g_p = p > 0.0f ? g_a * 0.5f / a : 0.0f;
+  g_a2pb2 += 0.5f * g_p;
+  float g_temp1 = 0.5f * g_p;
+  float g_q = q > 0.0f ? g_a2pb2 * 0.5f / a2pb2 : 0.0f;
+  g_temp1 += g_q * 2.0f * temp1;
+  float g_k2 = g_q * 4.0f * eta2, g_eta2 = g_q * 4.0f * k2;
+  g_eta2 += g_temp1;
+  g_k2 -= g_temp1;
+  g_si2 -= g_temp1;
+  g_si2 += g_si4 * 2.0f * si2;
+  g_ci2 -= g_si2;
+  gci += g_ci2 * 2.0f * ci;
+  *g_ci += gci;
+  *g_eta = g_eta2 * 2.0f * eta;
+  *g_k = g_k2 * 2.0f * k;
+}
+
+// VJP of roughconductor_inner.
+TR_HD void roughconductor_inner_vjp(F3 wo, F3 wi, float r, F3 eta, F3 kk, F3 adj_f, F3* adj_wo, F3* adj_wi,
+                                    float* adj_r, F3* adj_eta, F3* adj_k) {
+  const F3 zero{0.0f, 0.0f, 0.0f};
+  *adj_wo = *adj_wi = *adj_eta = *adj_k = zero;
+  *adj_r = 0.0f;
+  if (!(wo.z > 0.0f && wi.z > 0.0f)) return;
+  F3 hr = wo + wi;
+  float hl = length(hr);
+  F3 wh = hr / hl;
+  float D = ggx_d(wh, r);
+  if (D == 0.0f) return;
+  float Go = smith_g(wo, wh, r), Gi = smith_g(wi, wh, r), G = Go * Gi;
+  float coso = dot(wo, wh);
+  float den = 4.0f * fabsf(wi.z) * fabsf(wo.z);
+  F3 Fr{fresnel_conductor(coso, eta.x, kk.x), fresnel_conductor(coso, eta.y, kk.y),
+        fresnel_conductor(coso, eta.z, kk.z)};
+  F3 gFr = adj_f * (D * G / den);
+  float sFr = dot(adj_f, Fr);
+  float gD = sFr * G / den, gG = sFr * D / den;
+  float gden = -sFr * D * G / (den * den);
+  float g_coso = 0.0f;
+  fresnel_conductor_vjp(coso, eta.x, kk.x, gFr.x, &g_coso, &adj_eta->x, &adj_k->x);
+  fresnel_conductor_vjp(coso, eta.y, kk.y, gFr.y, &g_coso, &adj_eta->y, &adj_k->y);
+  fresnel_conductor_vjp(coso, eta.z, kk.z, gFr.z, &g_coso, &adj_eta->z, &adj_k->z);
+  float g_whz = 0.0f, g_r = 0.0f;
+  ggx_d_vjp(wh.z, r, D, gD, &g_whz, &g_r);
+  smith_g_vjp(wo, wh, r, gG * Gi, &g_whz, &g_r);
+  smith_g_vjp(wi, wh, r, gG * Go, &g_whz, &g_r);
+  *adj_r = g_r;
+  F3 g_wh = F3{0.0f, 0.0f, g_whz} + g_coso * wo;
+  F3 g_hr = normalize_vjp(wh, hl, g_wh);
+  *adj_wo = F3{0.0f, 0.0f, gden * 4.0f * wi.z} + g_coso * wh + g_hr;
+  *adj_wi = F3{0.0f, 0.0f, gden * 4.0f * wo.z} + g_hr;
+}
+
 // Adjoint of a BSDF evaluation f(wo_world, wi_world) at shading point `s`
 // (frame t, b, ns).  Accumulates into adj_wo / adj_wi / frame adjoints and
 // the material slot.
@@ -172,18 +288,31 @@ __device__ inline void bsdf_eval_vjp(const SceneDev& S, const Its& s, F3 wo_w, F
     return;  // no directional dependence
   }
   if (!(wo.z > 0.0f && wi.z > 0.0f)) return;
-  F3 kd = mat_albedo(S, m, s);
-  F3 F0{m.spec[0], m.spec[1], m.spec[2]};
-  float ar;
-  F3 akd, aF0;
-  microfacet_inner_vjp(wo, wi, m.roughness, kd, F0, adj_f, &awo, &awi, &ar, &akd, &aF0);
-  if (s.mat < kMaxMatGrad) {
-    float* g = &mat_grad[s.mat * kMatGradStride];
-    atomicAdd(&g[0], ar);
-    if (m.tex < 0) {
-      atomicAdd(&g[1], akd.x); atomicAdd(&g[2], akd.y); atomicAdd(&g[3], akd.z);
+  if (m.type == kMatRoughConductor) {
+    float ar;
+    F3 aeta, ak;
+    roughconductor_inner_vjp(wo, wi, m.roughness, F3{m.eta[0], m.eta[1], m.eta[2]}, F3{m.k[0], m.k[1], m.k[2]},
+                             adj_f, &awo, &awi, &ar, &aeta, &ak);
+    if (s.mat < kMaxMatGrad) {
+      float* g = &mat_grad[s.mat * kMatGradStride];
+      atomicAdd(&g[0], ar);
+      atomicAdd(&g[1], aeta.x); atomicAdd(&g[2], aeta.y); atomicAdd(&g[3], aeta.z);
+      atomicAdd(&g[4], ak.x); atomicAdd(&g[5], ak.y); atomicAdd(&g[6], ak.z);
     }
-    atomicAdd(&g[4], aF0.x); atomicAdd(&g[5], aF0.y); atomicAdd(&g[6], aF0.z);
+  } else {
+    F3 kd = mat_albedo(S, m, s);
+    F3 F0{m.spec[0], m.spec[1], m.spec[2]};
+    float ar;
+    F3 akd, aF0;
+    microfacet_inner_vjp(wo, wi, m.roughness, kd, F0, adj_f, &awo, &awi, &ar, &akd, &aF0);
+    if (s.mat < kMaxMatGrad) {
+      float* g = &mat_grad[s.mat * kMatGradStride];
+      atomicAdd(&g[0], ar);
+      if (m.tex < 0) {
+        atomicAdd(&g[1], akd.x); atomicAdd(&g[2], akd.y); atomicAdd(&g[3], akd.z);
+      }
+      atomicAdd(&g[4], aF0.x); atomicAdd(&g[5], aF0.y); atomicAdd(&g[6], aF0.z);
+    }
   }
   // local = (v.t, v.b, v.ns)
   adj_wo_w += awo.x * s.t + awo.y * s.b + awo.z * s.ns;
@@ -219,7 +348,7 @@ __device__ inline void tri_verts(const SceneDev& S, int32_t tri, F3& p0, F3& p1,
 
 // Backward sweep over a recorded path.  g = dL/dI of the pixel times 1/spp.
 __device__ void path_backward(const SceneDev& S, const CameraDev& C, const PathParams& P, const PathRec& R,
-                              F3 g, float* dpos, float* mat_grad) {
+                              F3 g, float* dpos, float* mat_grad, float* adj_vn) {
   const F3 zero{0.0f, 0.0f, 0.0f};
   const int K = R.K;
   if (K < 0) return;
@@ -368,7 +497,21 @@ __device__ void path_backward(const SceneDev& S, const CameraDev& C, const PathP
     }
     // ---- shading frame -> normal -> triangle ----
     F3 adj_ns = coordsys_vjp(its.ns, fa.t, fa.b) + fa.ns;
-    adj_ng += adj_ns;  // flat shading: ns = ng
+    const uint32_t tflags = S.shade[ti].flags;
+    if (!(tflags & kTriSmooth)) {
+      adj_ng += adj_ns;  // flat shading: ns = ng
+    } else if (adj_vn && (tflags & kTriDiff)) {
+      // ns = w n0 + u n1 + v n2 (vertex normals, not renormalized)
+      const uint32_t k0 = S.idx[3 * ti], k1 = S.idx[3 * ti + 1], k2 = S.idx[3 * ti + 2];
+      const F3 c[3] = {w * adj_ns, u * adj_ns, v * adj_ns};
+      const uint32_t ks[3] = {k0, k1, k2};
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        if (c[j].x != 0.0f) atomicAdd(&adj_vn[3 * ks[j]], c[j].x);
+        if (c[j].y != 0.0f) atomicAdd(&adj_vn[3 * ks[j] + 1], c[j].y);
+        if (c[j].z != 0.0f) atomicAdd(&adj_vn[3 * ks[j] + 2], c[j].z);
+      }
+    }
     A_cur.a0 += w * adj_x; A_cur.a1 += u * adj_x; A_cur.a2 += v * adj_x;
     face_vjp(fg, adj_ng, adj_area, A_cur);
     if (i > 0) {
@@ -382,6 +525,35 @@ __device__ void path_backward(const SceneDev& S, const CameraDev& C, const PathP
     A_cur = A_prev;
   }
   tri_flush(S, tri_next, A_next, dpos);
+}
+
+// Adjoint of the smooth-shading vertex normals (TriangleMesh::
+// ComputeVertexNormal, Primitive.cpp:323-340): n_v = normalize(a / c),
+// a = sum of the c incident face normals.  One thread per vertex folds
+// adj_vn[v] into the face normals' vertices and clears adj_vn[v].
+__device__ inline void vertex_normal_backward(const SceneDev& S, uint32_t vtx, float* adj_vn, float* dpos) {
+  const uint32_t b = S.vf_off[vtx], e = S.vf_off[vtx + 1];
+  const F3 g{adj_vn[3 * vtx], adj_vn[3 * vtx + 1], adj_vn[3 * vtx + 2]};
+  if (b == e || (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f)) return;
+  adj_vn[3 * vtx] = adj_vn[3 * vtx + 1] = adj_vn[3 * vtx + 2] = 0.0f;
+  F3 a{0.0f, 0.0f, 0.0f};
+  for (uint32_t j = b; j < e; ++j) {
+    const TriShade& sh = S.shade[S.vf_list[j]];
+    a = a + F3{sh.n[0], sh.n[1], sh.n[2]};
+  }
+  const float c = (float)(e - b);
+  const F3 q = a / c;
+  const float l = length(q);
+  const F3 adj_a = normalize_vjp(q / l, l, g) / c;
+  for (uint32_t j = b; j < e; ++j) {
+    const int32_t t = (int32_t)S.vf_list[j];
+    F3 p0, p1, p2;
+    tri_verts(S, t, p0, p1, p2);
+    TriAdj ta;
+    tri_adj_zero(ta);
+    face_vjp(face_geom(p0, p1, p2), adj_a, 0.0f, ta);
+    tri_flush(S, t, ta, dpos);
+  }
 }
 
 }  // namespace trc

@@ -42,6 +42,9 @@ struct trc_scene {
   // guiding of the direct boundary term: per-sample values, leaf weights,
   // leaf CDF and the weight integral
   DevBuf guide_buf;
+  // adjoint of the vertex normals [V][3] (kept zero between calls), present
+  // when a differentiable mesh is smooth-shaded
+  float* vn_adj = nullptr;
 };
 
 namespace trc {
@@ -82,7 +85,10 @@ hipError_t launch_guide_build(const SceneDev& S, const LaunchCfg& L, uint32_t se
 hipError_t launch_boundary(int kind, const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                            const LaunchCfg& L, const EdgeSet& prim, const GuideDev& guide, const float* dLdI,
                            float* dpos, hipStream_t st);
+// vn_adj: [V][3] zeroed scratch for smooth-shaded differentiable meshes or
+// null; it is folded into dpos and cleared again by a second kernel.
 hipError_t launch_render_adj(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
-                             const LaunchCfg& L, const float* dLdI, float* dpos, float* dmat, hipStream_t st);
+                             const LaunchCfg& L, const float* dLdI, float* dpos, float* dmat, float* vn_adj,
+                             hipStream_t st);
 
 }  // namespace trc

@@ -19,6 +19,7 @@ import numpy as np
 
 MAT_DIFFUSE = 0
 MAT_MICROFACET = 1
+MAT_ROUGHCONDUCTOR = 2
 MESH_SMOOTH = 1
 MESH_DIFF = 2
 
@@ -57,8 +58,8 @@ class Mesh:
 @dataclass
 class Material:
     type: int = MAT_DIFFUSE
-    albedo: tuple = (0.5, 0.5, 0.5)     # diffuse reflectance / microfacet kd
-    spec: tuple = (0.04, 0.04, 0.04)    # microfacet F0
+    albedo: tuple = (0.5, 0.5, 0.5)     # diffuse reflectance / microfacet kd / conductor eta
+    spec: tuple = (0.04, 0.04, 0.04)    # microfacet F0 / conductor k
     roughness: float = 0.3
     texture: int = -1
 

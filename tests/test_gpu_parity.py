@@ -74,6 +74,9 @@ CASES = {
     "metric_small": (scenes.metric_scene, 96, 96, 4, 3),
     "textured": (lambda: scenes.sphere_scene(subdiv=3, amplitude=0.05, texture_res=64), 48, 48, 4, 2),
     "smooth": (lambda: scenes.sphere_scene(subdiv=2, smooth=True), 48, 48, 4, 2),
+    "conductor": (lambda: scenes.sphere_scene(subdiv=3, amplitude=0.05, material=scenes.Material(
+        scenes.MAT_ROUGHCONDUCTOR, albedo=(0.143, 0.374, 1.44), spec=(3.98, 2.38, 1.60), roughness=0.35)),
+        48, 48, 4, 3),
 }
 
 
@@ -108,7 +111,7 @@ def _grad_close(g, r, rel=1e-3):
     return ok.all(), err.max() / scale
 
 
-@pytest.mark.parametrize("case", ["config1", "metric_small", "textured"])
+@pytest.mark.parametrize("case", ["config1", "metric_small", "textured", "conductor", "smooth"])
 def test_adjoint_matches_oracle(gpu_ctx, oracle_mod, case):
     torch = _torch()
     make, W, H, spp, B = CASES[case]

@@ -267,10 +267,19 @@ def test_direct_lighting_matches_lambert_formula(oracle_mod):
 # material-form estimator: the oracle re-evaluates a path with one parameter
 # shifted by +-delta in a float64 twin lane while every branch, hit record and
 # detached factor keeps its unperturbed value.
-@pytest.mark.parametrize("kind", ["diffuse", "microfacet"])
+@pytest.mark.parametrize("kind", ["diffuse", "microfacet", "conductor", "smooth"])
 def test_adjoint_matches_finite_differences(oracle_mod, kind):
     if kind == "diffuse":
         spec, B = scenes.sphere_scene(subdiv=2), 2
+    elif kind == "smooth":
+        # smooth shading: the shading frame follows the averaged vertex normals
+        mat = scenes.Material(scenes.MAT_MICROFACET, albedo=(0.2, 0.3, 0.1), spec=(0.04, 0.05, 0.06), roughness=0.4)
+        spec, B = scenes.sphere_scene(subdiv=2, smooth=True, material=mat), 2
+    elif kind == "conductor":
+        # gold-like complex IOR in the albedo (eta) and spec (k) slots
+        mat = scenes.Material(scenes.MAT_ROUGHCONDUCTOR, albedo=(0.143, 0.374, 1.44), spec=(3.98, 2.38, 1.60),
+                              roughness=0.35)
+        spec, B = scenes.sphere_scene(subdiv=2, material=mat), 3
     else:
         mat = scenes.Material(scenes.MAT_MICROFACET, albedo=(0.2, 0.3, 0.1), spec=(0.04, 0.05, 0.06), roughness=0.4)
         spec, B = scenes.sphere_scene(subdiv=2, material=mat), 3
@@ -294,12 +303,16 @@ def test_adjoint_matches_finite_differences(oracle_mod, kind):
             ad = dpos[v, k]
             assert abs(fd - ad) <= 1e-3 * max(abs(ad), abs(fd)) + 1e-6, (int(pth), v, k, fd, ad)
             checked += 1
-        if kind == "microfacet":
+        if kind not in ("diffuse", "smooth"):
             fd = S.path_twin_fd(cam, W, H, seed, B, pth, "roughness", 0) @ gw.astype(np.float64)
             assert abs(fd - dmat[0, 0]) <= 1e-3 * max(abs(fd), abs(dmat[0, 0])) + 1e-6
             for c in range(3):
                 fd = S.path_twin_fd(cam, W, H, seed, B, pth, "spec", 0, c) @ gw.astype(np.float64)
                 assert abs(fd - dmat[0, 4 + c]) <= 1e-3 * max(abs(fd), abs(dmat[0, 4 + c])) + 1e-6
+        if kind == "conductor":
+            for c in range(3):
+                fd = S.path_twin_fd(cam, W, H, seed, B, pth, "albedo", 0, c) @ gw.astype(np.float64)
+                assert abs(fd - dmat[0, 1 + c]) <= 1e-3 * max(abs(fd), abs(dmat[0, 1 + c])) + 1e-6
     assert checked >= 60
 
 
@@ -392,3 +405,27 @@ def test_total_gradient_matches_finite_differences(oracle_mod):
     assert abs(total[2] - fd_z) < 0.1 * scale, (total, fd_z)
     # the interior term alone misses the edge contributions along y
     assert abs(di[:nv, 1].sum() - fd_y) > 0.15 * scale
+
+
+def test_ggx_and_conductor_fresnel_closed_forms(oracle_mod):
+    """GGX_D (alpha = roughness, as RoughConductor uses it) is a normalized
+    microfacet distribution: integral of D cos(theta_h) over the hemisphere
+    = 1.  FresnelConductor: normal incidence ((eta-1)^2 + k^2) / ((eta+1)^2 +
+    k^2); with k = 0 it is the unpolarized dielectric Fresnel reflectance;
+    grazing incidence reflects everything."""
+    for a in (0.1, 0.3, 0.7):
+        th = (np.arange(200000) + 0.5) / 200000 * (np.pi / 2)
+        D = oracle_mod.ggx_d(np.cos(th), a).astype(np.float64)
+        integral = (D * np.cos(th) * np.sin(th)).sum() * 2 * np.pi * (np.pi / 2 / 200000)
+        assert abs(integral - 1.0) < 2e-3, (a, integral)
+    for eta, k in ((0.143, 3.98), (1.5, 0.0), (0.27, 2.78), (2.0, 1.0)):
+        f0 = oracle_mod.fresnel_conductor([1.0], eta, k)[0]
+        assert abs(f0 - ((eta - 1) ** 2 + k ** 2) / ((eta + 1) ** 2 + k ** 2)) < 1e-5
+        assert abs(oracle_mod.fresnel_conductor([1e-4], eta, k)[0] - 1.0) < 2e-3
+    ci = np.linspace(0.05, 1.0, 64)
+    eta = 1.5
+    ct = np.sqrt(1 - (1 - ci ** 2) / eta ** 2)
+    rs = ((ci - eta * ct) / (ci + eta * ct)) ** 2
+    rp = ((eta * ci - ct) / (eta * ci + ct)) ** 2
+    got = oracle_mod.fresnel_conductor(ci, eta, 0.0)
+    assert np.abs(got - 0.5 * (rs + rp)).max() < 1e-5
