@@ -82,6 +82,7 @@ typedef struct trc_scene_info {
   int32_t bvh_nodes, bvh_leaves, bvh_max_depth, bvh_stack_need;
   double bvh_sah_cost;
   int64_t device_bytes;
+  int64_t texture_floats; /* size of the texture-gradient buffer of trc_render_adj */
 } trc_scene_info;
 
 /* ---- context: replaces EnvCreate / EnvRelease (pybind_utils.h:38-61) ---- */
@@ -162,10 +163,14 @@ trc_status trc_render_stats(trc_scene* scene, const trc_camera* cam, const trc_o
  * Samples per pixel 0 skips the term.  Their forward value is zero. */
 trc_status trc_render_boundary(trc_scene* scene, const trc_camera* cam, const trc_options* opt, int32_t kind,
                                const float* dLdI, float* dpos, trc_stream stream);
-/* Adjoint: accumulates dL/dV += J^T dLdI into dpos ([V][3] device) and the
- * material gradients into dmat ([M][8] device, may be NULL). */
+/* Adjoint: accumulates dL/dV += J^T dLdI into dpos ([V][3] device), the
+ * material gradients into dmat ([M][8] device, may be NULL) and the texel
+ * gradients of albedo / kd textures into dtex (may be NULL; all textures'
+ * [h][w][3] texels concatenated in texture order, texture_floats floats;
+ * BSDF::DiffTexture, BSDF.cpp:350-356).  Calls on one scene are ordered on
+ * one stream (smooth-shaded meshes use scene-owned scratch). */
 trc_status trc_render_adj(trc_scene* scene, const trc_camera* cam, const trc_options* opt, const float* dLdI,
-                          float* dpos, float* dmat, trc_stream stream);
+                          float* dpos, float* dmat, float* dtex, trc_stream stream);
 
 #ifdef __cplusplus
 }

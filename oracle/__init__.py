@@ -65,7 +65,8 @@ def lib():
         L.oracle_path_twin.argtypes = [vp, P(f32), ctypes.c_int, ctypes.c_int, u32, ctypes.c_int, ctypes.c_uint64,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, P(f64)]
         L.oracle_render_adj.argtypes = [vp, P(f32), ctypes.c_int, ctypes.c_int, u32, ctypes.c_int, ctypes.c_int,
-                                        P(f32), ctypes.c_uint64, ctypes.c_uint64, P(f64), P(f64), ctypes.c_int]
+                                        P(f32), ctypes.c_uint64, ctypes.c_uint64, P(f64), P(f64), ctypes.c_int,
+                                        P(f64)]
         _lib = L
     return _lib
 
@@ -159,6 +160,7 @@ class Scene:
             _p(f["tri_mat"], ctypes.c_int32), _p(f["tri_emitter"], ctypes.c_int32), _p(f["tri_flags"], ctypes.c_int32),
             _p(f["emit_rad"], ctypes.c_float), len(f["emit_rad"]), _p(mats, ctypes.c_float), len(mats),
             _p(uv, ctypes.c_float), len(uv) if uv is not None else 0, _p(uvi, ctypes.c_uint32))
+        self.tex_floats = int(sum(np.asarray(t).size for t in spec.textures))
         for t in spec.textures:
             t = np.ascontiguousarray(t, np.float32)
             lib().oracle_scene_add_texture(self.h, _p(t, ctypes.c_float), t.shape[1], t.shape[0])
@@ -207,7 +209,7 @@ class Scene:
                                  _p(film, ctypes.c_float), threads or default_threads())
         return film
 
-    PARAM_KINDS = {"vertex": 0, "roughness": 1, "albedo": 2, "spec": 3}
+    PARAM_KINDS = {"vertex": 0, "roughness": 1, "albedo": 2, "spec": 3, "texel": 4}
 
     def path_twin_fd(self, cam, w, h, seed, max_bounces, path, kind, index, comp=0, delta=1e-4):
         """Central difference (float64) of one path's material-form estimator
@@ -218,18 +220,21 @@ class Scene:
                                self.PARAM_KINDS[kind], int(index), int(comp), float(delta), _p(out, ctypes.c_double))
         return (out[:3] - out[3:]) / (2.0 * delta)
 
-    def render_adj(self, cam, w, h, seed, max_bounces, spp, dLdI, first=0, count=None, threads=None):
-        """Returns (dpos [V,3], dmat [M,8]) in float64."""
+    def render_adj(self, cam, w, h, seed, max_bounces, spp, dLdI, first=0, count=None, threads=None,
+                   with_tex=False):
+        """Returns (dpos [V,3], dmat [M,8]) in float64, plus the texture
+        gradients (textures' [h][w][3] texels concatenated) if with_tex."""
         dLdI = np.ascontiguousarray(dLdI, np.float32).reshape(3, w * h)
         if count is None:
             count = spp * w * h
         dpos = np.zeros((self.V, 3), np.float64)
         dmat = np.zeros((self.M, 8), np.float64)
+        dtex = np.zeros(self.tex_floats, np.float64) if with_tex else None
         c = self._cam(cam)
         lib().oracle_render_adj(self.h, _p(c, ctypes.c_float), w, h, seed, max_bounces, spp, _p(dLdI, ctypes.c_float),
                                 first, count, _p(dpos, ctypes.c_double), _p(dmat, ctypes.c_double),
-                                threads or default_threads())
-        return dpos, dmat
+                                threads or default_threads(), _p(dtex, ctypes.c_double))
+        return (dpos, dmat, dtex) if with_tex else (dpos, dmat)
 
 
 def _boundary_kinds():

@@ -545,6 +545,7 @@ struct Access<float> {
   float rough(int m) { return s.mats[m].roughness; }
   F3 albedo(int m) { return s.mats[m].albedo; }
   F3 spec(int m) { return s.mats[m].spec; }
+  F3 texel(int ti, size_t k) { const float* p = &s.tex[ti][3 * k]; return F3{p[0], p[1], p[2]}; }
 };
 
 template <>
@@ -553,7 +554,17 @@ struct Access<Var> {
   std::unordered_map<uint32_t, V3<Var>> vcache;
   std::unordered_map<int, Var> rcache;
   std::unordered_map<int, V3<Var>> acache, scache;
+  std::unordered_map<uint64_t, V3<Var>> tcache;  // (texture << 32 | texel) -> leaves
   explicit Access(const Scene& sc) : s(sc) {}
+  V3<Var> texel(int ti, size_t k) {
+    const uint64_t key = ((uint64_t)ti << 32) | (uint64_t)k;
+    auto it = tcache.find(key);
+    if (it != tcache.end()) return it->second;
+    const float* p = &s.tex[ti][3 * k];
+    V3<Var> r{leaf(p[0]), leaf(p[1]), leaf(p[2])};
+    tcache.emplace(key, r);
+    return r;
+  }
   V3<Var> pos(uint32_t v) {
     auto it = vcache.find(v);
     if (it != vcache.end()) return it->second;
@@ -589,7 +600,8 @@ struct Access<Var> {
 
 // Twin access: one parameter shifted by `delta` in the perturbed lane.
 // kind 0: vertex coordinate (index, comp); 1: roughness (index = material);
-// 2: albedo (index, comp); 3: specular (index, comp).
+// 2: albedo (index, comp); 3: specular (index, comp); 4: texel channel
+// (index = texture, comp = 3 * texel + channel).
 template <>
 struct Access<Pair> {
   const Scene& s;
@@ -620,6 +632,15 @@ struct Access<Pair> {
   }
   V3<Pair> albedo(int m) { return vec(s.mats[m].albedo, m, 2); }
   V3<Pair> spec(int m) { return vec(s.mats[m].spec, m, 3); }
+  V3<Pair> texel(int ti, size_t k) {
+    const float* p = &s.tex[ti][3 * k];
+    V3<Pair> r{Pair(p[0]), Pair(p[1]), Pair(p[2])};
+    if (kind == 4 && ti == index && (size_t)comp / 3 == k) {
+      Pair* c = comp % 3 == 0 ? &r.x : (comp % 3 == 1 ? &r.y : &r.z);
+      c->s += delta;
+    }
+    return r;
+  }
 };
 
 template <class S>
@@ -706,19 +727,21 @@ static S area_of(Access<S>& A, int tri) {
 template <class S> static V3<S> to_local(const Its<S>& it, V3<S> v) { return {dot(v, it.t), dot(v, it.b), dot(v, it.ns)}; }
 template <class S> static V3<S> to_world(const Its<S>& it, V3<S> l) { return (l.x * it.t + l.y * it.b) + l.z * it.ns; }
 
-// ImageTexture::Eval (BSDF.cpp:72-98)
-static F3 tex_lookup(const Scene& s, int ti, float u, float v) {
+// ImageTexture::Eval (BSDF.cpp:72-98): nearest texel index at
+// floor(frac(u) * (w-1)), floor(frac(v) * (h-1)).
+static size_t tex_index(const Scene& s, int ti, float u, float v) {
   int w = s.tex_w[ti], h = s.tex_h[ti];
   float fu = u - floorf(u), fv = v - floorf(v);
   int x = (int)(fu * (float)(w - 1)), y = (int)(fv * (float)(h - 1));
-  const float* p = &s.tex[ti][3 * ((size_t)y * w + x)];
-  return F3{p[0], p[1], p[2]};
+  return (size_t)y * w + x;
 }
 
+// The texel is a differentiable leaf (BSDF::DiffTexture, BSDF.cpp:350-356):
+// its gradient is the scatter of the lookup's adjoint.
 template <class S>
 static V3<S> albedo_at(Access<S>& A, const Its<S>& it) {
   const Mat& m = A.s.mats[it.mat];
-  if (m.tex >= 0) return cast3<S>(tex_lookup(A.s, m.tex, it.tu, it.tv));
+  if (m.tex >= 0) return A.texel(m.tex, tex_index(A.s, m.tex, it.tu, it.tv));
   return A.albedo(it.mat);
 }
 
@@ -2030,13 +2053,17 @@ void oracle_guide(void* sp, uint32_t seed, int g_spp, float g_eps, float* x0, fl
 // dmat [M][8] (roughness, albedo rgb, spec rgb).  Accumulated in double.
 void oracle_render_adj(void* sp, const float* cam, int w, int h, uint32_t seed, int max_bounces, int spp,
                        const float* dLdI, uint64_t first, uint64_t count, double* dpos, double* dmat,
-                       int threads) {
+                       int threads, double* dtex) {
   const Scene& s = *(Scene*)sp;
   Cam c = make_camera(cam, cam + 3, cam + 6, w, h, cam[9]);
   const uint64_t WH = (uint64_t)w * h;
   int nt = std::max(1, threads);
   std::vector<std::vector<double>> tp(nt, std::vector<double>(3 * (size_t)s.V, 0.0));
   std::vector<std::vector<double>> tm(nt, std::vector<double>(8 * s.mats.size(), 0.0));
+  // texture gradients: textures concatenated in index order
+  std::vector<size_t> toff(s.tex.size() + 1, 0);
+  for (size_t t = 0; t < s.tex.size(); ++t) toff[t + 1] = toff[t] + s.tex[t].size();
+  std::vector<std::vector<double>> tt(nt, std::vector<double>(dtex ? toff.back() : 0, 0.0));
   const float inv_spp = 1.0f / (float)spp;
   parallel_for(count, threads, [&](uint64_t a, uint64_t b, int tid) {
     Tape tape;
@@ -2077,6 +2104,11 @@ void oracle_render_adj(void* sp, const float* cam, int w, int h, uint32_t seed, 
         tm[tid][8 * kv.first + 5] += tape.adj[kv.second.y.i];
         tm[tid][8 * kv.first + 6] += tape.adj[kv.second.z.i];
       }
+      if (dtex)
+        for (auto& kv : A.tcache) {
+          double* d = &tt[tid][toff[kv.first >> 32] + 3 * (kv.first & 0xFFFFFFFFull)];
+          d[0] += tape.adj[kv.second.x.i]; d[1] += tape.adj[kv.second.y.i]; d[2] += tape.adj[kv.second.z.i];
+        }
     }
     g_tape = nullptr;
   });
@@ -2084,6 +2116,8 @@ void oracle_render_adj(void* sp, const float* cam, int w, int h, uint32_t seed, 
     for (size_t k = 0; k < tp[t].size(); ++k) dpos[k] += tp[t][k];
     if (dmat)
       for (size_t k = 0; k < tm[t].size(); ++k) dmat[k] += tm[t][k];
+    if (dtex)
+      for (size_t k = 0; k < tt[t].size(); ++k) dtex[k] += tt[t][k];
   }
 }
 

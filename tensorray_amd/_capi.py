@@ -41,7 +41,8 @@ class trc_scene_info(ctypes.Structure):
                 ("num_materials", ctypes.c_int32), ("num_emitter_triangles", ctypes.c_int32),
                 ("num_edges", ctypes.c_int32), ("bvh_nodes", ctypes.c_int32), ("bvh_leaves", ctypes.c_int32),
                 ("bvh_max_depth", ctypes.c_int32), ("bvh_stack_need", ctypes.c_int32),
-                ("bvh_sah_cost", ctypes.c_double), ("device_bytes", ctypes.c_int64)]
+                ("bvh_sah_cost", ctypes.c_double), ("device_bytes", ctypes.c_int64),
+                ("texture_floats", ctypes.c_int64)]
 
 
 # (name, restype, argtypes) of every exported entry point
@@ -76,7 +77,7 @@ SIGNATURES = [
     ("trc_render_boundary", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options),
                                              ctypes.c_int32, vp, vp, vp]),
     ("trc_render_adj", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options), vp, vp, vp,
-                                        vp]),
+                                        vp, vp]),
 ]
 
 _lib = None
@@ -274,9 +275,9 @@ class DeviceScene:
                                                      _stream_ptr(stream)), "trc_render_stats")
         return tuple(int(x) for x in out)
 
-    def render_adj(self, cam, opt, dldi_ptr, dpos_ptr, dmat_ptr=None, stream=None):
+    def render_adj(self, cam, opt, dldi_ptr, dpos_ptr, dmat_ptr=None, stream=None, dtex_ptr=None):
         self.ctx.check(self.ctx.lib.trc_render_adj(self.h, ctypes.byref(cam), ctypes.byref(opt), dldi_ptr, dpos_ptr,
-                                                   dmat_ptr, _stream_ptr(stream)), "trc_render_adj")
+                                                   dmat_ptr, dtex_ptr, _stream_ptr(stream)), "trc_render_adj")
 
     BOUNDARY_KINDS = {"primary": 0, "pixel": 1, "direct": 2, "indirect": 3}
 

@@ -119,6 +119,7 @@ void upload_scene(trc_scene* s, hipStream_t st) {
     t.texels = upload(s, h.textures[i], st);
     t.w = h.tex_w[i];
     t.h = h.tex_h[i];
+    t.grad_off = texs.empty() ? 0 : texs.back().grad_off + 3 * (int64_t)texs.back().w * texs.back().h;
     texs.push_back(t);
   }
   d.texs = upload(s, texs, st);
@@ -416,6 +417,9 @@ trc_status trc_scene_get_info(const trc_scene* s, trc_scene_info* out) {
   out->bvh_stack_need = h.bvh_stats.stack_need;
   out->bvh_sah_cost = h.bvh_stats.sah_cost;
   out->device_bytes = s->device_bytes;
+  int64_t tf = 0;
+  for (size_t i = 0; i < h.textures.size(); ++i) tf += (int64_t)h.textures[i].size();
+  out->texture_floats = tf;
   return TRC_OK;
 }
 
@@ -601,7 +605,7 @@ trc_status trc_render_boundary(trc_scene* s, const trc_camera* cam, const trc_op
 }
 
 trc_status trc_render_adj(trc_scene* s, const trc_camera* cam, const trc_options* opt, const float* dLdI,
-                          float* dpos, float* dmat, trc_stream stream) {
+                          float* dpos, float* dmat, float* dtex, trc_stream stream) {
   if (!s) return TRC_ERR_INVALID;
   return guarded(s->ctx, [&] {
     check_render(s, opt);
@@ -609,8 +613,8 @@ trc_status trc_render_adj(trc_scene* s, const trc_camera* cam, const trc_options
     if (!dLdI || !dpos) throw Fail(TRC_ERR_INVALID, "dLdI / dpos is NULL");
     long long np = (long long)opt->spp * C.width * C.height;
     LaunchCfg L = make_cfg(s->ctx, s, (hipStream_t)stream, np);
-    hip_check(launch_render_adj(s->dev, C, opt->seed, opt->max_bounces, opt->spp, L, dLdI, dpos, dmat, s->vn_adj,
-                                (hipStream_t)stream),
+    hip_check(launch_render_adj(s->dev, C, opt->seed, opt->max_bounces, opt->spp, L, dLdI, dpos, dmat, dtex,
+                                s->vn_adj, (hipStream_t)stream),
               "k_render_adj");
   });
 }

@@ -63,10 +63,13 @@ TR_HD Its post_intersect(const SceneDev& S, int32_t tri, float u, float v) {
 }
 
 // ImageTexture::Eval (BSDF.cpp:72-98): nearest texel at floor(frac(u) * (w-1)).
-TR_HD F3 tex_eval(const TexDesc& t, float u, float v) {
+TR_HD size_t tex_index(const TexDesc& t, float u, float v) {
   float fu = u - floorf(u), fv = v - floorf(v);
   int x = (int)(fu * (float)(t.w - 1)), y = (int)(fv * (float)(t.h - 1));
-  const float* p = t.texels + 3 * ((size_t)y * t.w + x);
+  return (size_t)y * t.w + x;
+}
+TR_HD F3 tex_eval(const TexDesc& t, float u, float v) {
+  const float* p = t.texels + 3 * tex_index(t, u, v);
   return F3{p[0], p[1], p[2]};
 }
 

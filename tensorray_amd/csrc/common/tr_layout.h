@@ -116,6 +116,7 @@ struct EdgeSet {
 struct TexDesc {
   const float* texels;
   int32_t w, h;
+  int64_t grad_off;  // float offset of this texture in the texture-gradient buffer
 };
 
 // Parameters of a pinhole camera (Camera::Init/Resize, Camera.cpp:89-137 and

@@ -272,15 +272,29 @@ struct FrameAdj {
   F3 t, b, ns;
 };
 
+// Texture gradient (BSDF::DiffTexture, BSDF.cpp:350-356): the nearest-texel
+// lookup scatters its adjoint into one texel.
+__device__ inline void tex_scatter(const SceneDev& S, const Material& m, const Its& s, F3 g, float* dtex) {
+  if (!dtex) return;
+  const TexDesc& t = S.texs[m.tex];
+  float* d = dtex + t.grad_off + 3 * tex_index(t, s.tu, s.tv);
+  if (g.x != 0.0f) atomicAdd(&d[0], g.x);
+  if (g.y != 0.0f) atomicAdd(&d[1], g.y);
+  if (g.z != 0.0f) atomicAdd(&d[2], g.z);
+}
+
 __device__ inline void bsdf_eval_vjp(const SceneDev& S, const Its& s, F3 wo_w, F3 wi_w, F3 adj_f,
-                                     F3& adj_wo_w, F3& adj_wi_w, FrameAdj& fa, float* mat_grad) {
+                                     F3& adj_wo_w, F3& adj_wi_w, FrameAdj& fa, float* mat_grad, float* dtex) {
   const Material& m = S.mats[s.mat];
   F3 wo = to_local(s, wo_w), wi = to_local(s, wi_w);
   F3 awo{0.0f, 0.0f, 0.0f}, awi{0.0f, 0.0f, 0.0f};
   if (m.type == kMatDiffuse) {
     float inner = diffuse_inner(wo, wi);
-    if (inner != 0.0f && m.tex < 0 && s.mat < kMaxMatGrad) {
-      F3 ga = adj_f * inner;
+    if (inner == 0.0f) return;
+    F3 ga = adj_f * inner;
+    if (m.tex >= 0) {
+      tex_scatter(S, m, s, ga, dtex);
+    } else if (s.mat < kMaxMatGrad) {
       atomicAdd(&mat_grad[s.mat * kMatGradStride + 1], ga.x);
       atomicAdd(&mat_grad[s.mat * kMatGradStride + 2], ga.y);
       atomicAdd(&mat_grad[s.mat * kMatGradStride + 3], ga.z);
@@ -305,6 +319,7 @@ __device__ inline void bsdf_eval_vjp(const SceneDev& S, const Its& s, F3 wo_w, F
     float ar;
     F3 akd, aF0;
     microfacet_inner_vjp(wo, wi, m.roughness, kd, F0, adj_f, &awo, &awi, &ar, &akd, &aF0);
+    if (m.tex >= 0) tex_scatter(S, m, s, akd, dtex);
     if (s.mat < kMaxMatGrad) {
       float* g = &mat_grad[s.mat * kMatGradStride];
       atomicAdd(&g[0], ar);
@@ -348,7 +363,7 @@ __device__ inline void tri_verts(const SceneDev& S, int32_t tri, F3& p0, F3& p1,
 
 // Backward sweep over a recorded path.  g = dL/dI of the pixel times 1/spp.
 __device__ void path_backward(const SceneDev& S, const CameraDev& C, const PathParams& P, const PathRec& R,
-                              F3 g, float* dpos, float* mat_grad, float* adj_vn) {
+                              F3 g, float* dpos, float* mat_grad, float* adj_vn, float* dtex) {
   const F3 zero{0.0f, 0.0f, 0.0f};
   const int K = R.K;
   if (K < 0) return;
@@ -409,7 +424,7 @@ __device__ void path_backward(const SceneDev& S, const CameraDev& C, const PathP
         float sv = dot(wn, f);
         float adj_absSh = sv * GL, adj_GL = sv * absSh, adj_JL = sv * absSh * GL;
         F3 adj_wl = zero;
-        bsdf_eval_vjp(S, its, wo, wl, adj_f, adj_wo, adj_wl, fa, mat_grad);
+        bsdf_eval_vjp(S, its, wo, wl, adj_f, adj_wo, adj_wl, fa, mat_grad, dtex);
         float adj_dsh = adj_absSh * sgnf(dsh);
         fa.ns += adj_dsh * wl;
         adj_wl += adj_dsh * its.ns;
@@ -451,7 +466,7 @@ __device__ void path_backward(const SceneDev& S, const CameraDev& C, const PathP
       float sv = dot(gT, f) * inv_pp;
       float adj_absc = sv * G, adj_G = sv * absc, adj_J = sv * absc * G;
       F3 adj_d = zero;
-      bsdf_eval_vjp(S, its, wo, d, adj_f, adj_wo, adj_d, fa, mat_grad);
+      bsdf_eval_vjp(S, its, wo, d, adj_f, adj_wo, adj_d, fa, mat_grad, dtex);
       float adj_dc = adj_absc * sgnf(dc);
       fa.ns += adj_dc * d;
       adj_d += adj_dc * its.ns;

@@ -97,7 +97,7 @@ __global__ void k_film_reduce(const float* paths, int spp, int wh, float* film) 
 
 __global__ void __launch_bounds__(kBlock) k_render_adj(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
                                                        uint64_t npaths, const float* dLdI, float* dpos,
-                                                       float* dmat, float* adj_vn) {
+                                                       float* dmat, float* adj_vn, float* dtex) {
   __shared__ uint32_t lds_stack[kStackLds * kBlock];
   __shared__ float mat_grad[kMaxMatGrad * kMatGradStride];
   for (int i = threadIdx.x; i < kMaxMatGrad * kMatGradStride; i += kBlock) mat_grad[i] = 0.0f;
@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(kBlock) k_render_adj(SceneDev S, CameraDev C, 
     if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) continue;
     PathRec R;
     path_forward<true>(S, C, P, p, tc, &R);
-    path_backward(S, C, P, R, g, dpos, mat_grad, adj_vn);
+    path_backward(S, C, P, R, g, dpos, mat_grad, adj_vn, dtex);
   }
   __syncthreads();
   if (dmat) {
@@ -155,11 +155,11 @@ hipError_t launch_film_reduce(const float* paths, int spp, int wh, float* film, 
 }
 
 hipError_t launch_render_adj(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
-                             const LaunchCfg& L, const float* dLdI, float* dpos, float* dmat, float* vn_adj,
-                             hipStream_t st) {
+                             const LaunchCfg& L, const float* dLdI, float* dpos, float* dmat, float* dtex,
+                             float* vn_adj, hipStream_t st) {
   PathParams P{seed, max_bounces, spp, 0u};
   uint64_t np = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
-  hipLaunchKernelGGL(k_render_adj, dim3(L.grid), dim3(kBlock), 0, st, S, C, P, L, np, dLdI, dpos, dmat, vn_adj);
+  hipLaunchKernelGGL(k_render_adj, dim3(L.grid), dim3(kBlock), 0, st, S, C, P, L, np, dLdI, dpos, dmat, vn_adj, dtex);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !vn_adj || S.n_verts == 0) return e;
   const int grid = (S.n_verts + kBlock - 1) / kBlock;

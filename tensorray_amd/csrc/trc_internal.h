@@ -88,7 +88,7 @@ hipError_t launch_boundary(int kind, const SceneDev& S, const CameraDev& C, uint
 // vn_adj: [V][3] zeroed scratch for smooth-shaded differentiable meshes or
 // null; it is folded into dpos and cleared again by a second kernel.
 hipError_t launch_render_adj(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
-                             const LaunchCfg& L, const float* dLdI, float* dpos, float* dmat, float* vn_adj,
-                             hipStream_t st);
+                             const LaunchCfg& L, const float* dLdI, float* dpos, float* dmat, float* dtex,
+                             float* vn_adj, hipStream_t st);
 
 }  // namespace trc

@@ -124,10 +124,16 @@ def test_adjoint_matches_oracle(gpu_ctx, oracle_mod, case):
     dl = np.random.default_rng(0).uniform(-1, 1, (3, W * H)).astype(np.float32)
     dpos = torch.zeros(info.num_vertices, 3, device="cuda")
     dmat = torch.zeros(info.num_materials, 8, device="cuda")
-    ds.render_adj(cam, opt, torch.from_numpy(dl).cuda().data_ptr(), dpos.data_ptr(), dmat.data_ptr())
+    dtex = torch.zeros(max(1, info.texture_floats), device="cuda")
+    ds.render_adj(cam, opt, torch.from_numpy(dl).cuda().data_ptr(), dpos.data_ptr(), dmat.data_ptr(),
+                  dtex_ptr=dtex.data_ptr() if info.texture_floats else None)
     torch.cuda.synchronize()
     S = oracle_mod.Scene(spec)
-    rp, rm = S.render_adj(cam_d, W, H, 99, B, spp, dl)
+    rp, rm, rt = S.render_adj(cam_d, W, H, 99, B, spp, dl, with_tex=True)
+    if info.texture_floats:
+        assert np.abs(rt).max() > 0
+        ok, e = _grad_close(dtex.cpu().numpy().astype(np.float64), rt)
+        assert ok, e
     mask = spec.diff_vertex_mask()
     ok, e = _grad_close(dpos.cpu().numpy().astype(np.float64)[mask], rp[mask])
     assert ok, e

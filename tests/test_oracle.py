@@ -429,3 +429,27 @@ def test_ggx_and_conductor_fresnel_closed_forms(oracle_mod):
     rp = ((eta * ci - ct) / (eta * ci + ct)) ** 2
     got = oracle_mod.fresnel_conductor(ci, eta, 0.0)
     assert np.abs(got - 0.5 * (rs + rp)).max() < 1e-5
+
+
+def test_texture_gradient_matches_finite_differences(oracle_mod):
+    """Texel gradients of a textured albedo (BSDF::DiffTexture): reverse-mode
+    AD of one path against the twin-lane central difference of that texel."""
+    spec = scenes.sphere_scene(subdiv=2, texture_res=16)
+    S = oracle_mod.Scene(spec)
+    cam = scenes.CAMERA_DEFAULT
+    W = H = 32
+    seed, B = 5, 2
+    base = S.render_paths(cam, W, H, seed, B, 0, W * H)
+    rng = np.random.default_rng(3)
+    checked = 0
+    for pth in rng.choice(np.where(base.sum(1) > 1e-3)[0], 24, replace=False):
+        g = np.zeros((3, W * H), np.float32)
+        g[:, pth] = rng.uniform(0.5, 1.5, 3)
+        _, _, dtex = S.render_adj(cam, W, H, seed, B, 1, g, first=int(pth), count=1, with_tex=True)
+        for k in np.argsort(-np.abs(dtex))[:2]:
+            if dtex[k] == 0.0:
+                continue
+            fd = S.path_twin_fd(cam, W, H, seed, B, pth, "texel", 0, int(k)) @ g[:, pth].astype(np.float64)
+            assert abs(fd - dtex[k]) <= 1e-3 * max(abs(fd), abs(dtex[k])) + 1e-7, (int(pth), int(k), fd, dtex[k])
+            checked += 1
+    assert checked >= 10
