@@ -8,8 +8,11 @@ Lambertian ground quad under a 1x1 area light (radiance 10) at y = 2.
 16 spp, max depth 4 (3 NEE + BSDF bounces after the primary hit), FP32.
 
 One step = forward render of all W*H*spp paths (per-path radiance + film
-reduction) + dL/dI = I - I_target on the device + the adjoint (path replay
-and backward sweep scattering into vertex and material gradients).
+reduction) + dL/dI = I - I_target on the device + the adjoint (backward
+sweep scattering into vertex and material gradients).  By default the
+forward writes a path checkpoint (hits and visibility of every ray query)
+and the adjoint re-evaluates the paths from it without tracing;
+--adjoint replay re-traces every path instead.
 I_target is the same scene with displacement amplitude 0.06, rendered once
 with seed 99.  Inputs are resident in HBM before the timed region.
 
@@ -38,7 +41,10 @@ def parse():
     p.add_argument("--spp", type=int, default=16)
     p.add_argument("--depth", type=int, default=4)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-spp", type=int, default=8, help="samples per pixel of the CPU baseline sample")
+    p.add_argument("--cpu-spp", type=int, default=16, help="samples per pixel of the CPU baseline sample")
+    p.add_argument("--cpu-runs", type=int, default=5, help="timed CPU baseline runs (median reported)")
+    p.add_argument("--adjoint", choices=["checkpoint", "replay"], default="checkpoint",
+                   help="adjoint from the forward's path checkpoint (no re-tracing) or by replaying the paths")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="after the timed region, copy a MIB-sized device buffer (reference kernel of known "
                         "traffic for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -46,15 +52,17 @@ def parse():
 
 
 def algorithmic_bytes(stats, spp_paths, B, n_light_tris):
-    """Algorithmic HBM bytes per path of the forward and the adjoint pass.
+    """Algorithmic HBM bytes per path of the forward and adjoint passes.
 
     Per ray query: 32 B ray + 16 B hit + 128 B per BVH node visited + 48 B
     per triangle tested (averages measured on the same rays and BVH by the
     statistics kernel).  Per bounce: 56 B shading data + NEE light-table
     search 4*ceil(log2 T_L) + 48 B light triangle; film read-modify-write 24 B.
-    The adjoint replays the forward (same queries) and adds 12 B of dL/dI and
-    per path vertex the vertex-gradient RMW (3 vertices * 12 B * 2) and the
-    material-gradient RMW (2 * 4 B).
+    The replay adjoint re-traces the forward (same queries) and adds 12 B of
+    dL/dI and per path vertex the vertex-gradient RMW (3 vertices * 12 B * 2)
+    and the material-gradient RMW (2 * 4 B).  With a checkpoint the forward
+    also writes 4 B + 12 B per vertex, and the adjoint reads them back in
+    place of every ray query.
     """
     paths, queries, nodes, tris = stats
     q_per_path = queries / paths
@@ -62,30 +70,38 @@ def algorithmic_bytes(stats, spp_paths, B, n_light_tris):
     bounces = max(0.0, (q_per_path - 1.0) / 2.0)  # 1 primary + (shadow + bsdf) per bounce
     b_shade = 56 * (1 + bounces)
     b_nee = bounces * (4 * math.ceil(math.log2(max(2, n_light_tris))) + 48)
+    b_grad = 12 + (1 + bounces) * (3 * 12 * 2 + 2 * 4)
+    b_ckpt = 4 + 12 * (1 + bounces)
     b_fwd = q_per_path * b_query + b_shade + b_nee + 24
-    b_adj = b_fwd + 12 + (1 + bounces) * (3 * 12 * 2 + 2 * 4)
-    return b_fwd, b_adj, dict(queries_per_path=q_per_path, nodes_per_query=nodes / queries,
-                              tris_per_query=tris / queries, bytes_per_query=b_query)
+    return dict(fwd=b_fwd, adj=b_fwd + b_grad, fwd_ckpt=b_fwd + b_ckpt, adj_ckpt=b_ckpt + b_shade + b_nee + b_grad), \
+        dict(queries_per_path=q_per_path, nodes_per_query=nodes / queries, tris_per_query=tris / queries,
+             bytes_per_query=b_query)
 
 
-def cpu_baseline(spec, cam, W, H, B, spp, cpu_spp):
-    """The oracle (scalar C++ restatement, all host threads) on a bounded
-    sample of the same workload: the first cpu_spp samples of every pixel,
-    forward + adjoint."""
+def cpu_baseline(spec, cam, W, H, B, spp, cpu_spp, runs):
+    """The oracle (scalar C++ restatement, one thread per host core up to
+    16) on the same workload: the first cpu_spp samples of every pixel,
+    forward + adjoint; median of `runs` timed runs after one warm-up."""
     import numpy as np
     import oracle
     threads = max(1, min(16, os.cpu_count() or 1))
     S = oracle.Scene(spec)
     n = cpu_spp * W * H
-    t0 = time.perf_counter()
-    S.render_paths(cam, W, H, 1234, B, 0, n, threads=threads)
-    t1 = time.perf_counter()
     dl = np.full((3, W * H), 0.01, np.float32)
-    S.render_adj(cam, W, H, 1234, B, spp, dl, first=0, count=n, threads=threads)
-    t2 = time.perf_counter()
-    return dict(value=n / (t2 - t0) / 1e6, unit="Mpaths/s", cores=threads, kind="port",
-                sample=f"{cpu_spp} spp x {W}x{H} = {n} paths of the same scene, depth {B + 1}, forward + adjoint; "
-                       f"fwd {t1 - t0:.2f} s, adj {t2 - t1:.2f} s")
+    times = []
+    for r in range(runs + 1):
+        t0 = time.perf_counter()
+        S.render_paths(cam, W, H, 1234, B, 0, n, threads=threads)
+        t1 = time.perf_counter()
+        S.render_adj(cam, W, H, 1234, B, spp, dl, first=0, count=n, threads=threads)
+        t2 = time.perf_counter()
+        if r > 0:
+            times.append((t2 - t0, t1 - t0, t2 - t1))
+    times.sort()
+    tot, tf, ta = times[len(times) // 2]
+    return dict(value=n / tot / 1e6, unit="Mpaths/s", cores=threads, kind="port",
+                sample=f"{cpu_spp} spp x {W}x{H} = {n} paths of the same scene, depth {B + 1}, forward + adjoint "
+                       f"(replay); median of {runs} runs after a warm-up: fwd {tf:.2f} s, adj {ta:.2f} s")
 
 
 def main():
@@ -117,7 +133,9 @@ def main():
     target = torch.zeros(3, W * H, device="cuda")
     tscene.render_fwd(cam, topt, target.data_ptr(), stream)
     film = torch.zeros(3, W * H, device="cuda")
-    paths = torch.empty(P, 3, device="cuda")
+    paths = torch.empty(P, 3, device="cuda") if a.adjoint == "replay" else None
+    nb = scene.checkpoint_bytes(cam, opt)
+    ckpt = torch.empty(nb, dtype=torch.uint8, device="cuda") if a.adjoint == "checkpoint" else None
     dpos = torch.zeros(info.num_vertices, 3, device="cuda")
     dmat = torch.zeros(info.num_materials, 8, device="cuda")
     lib = ctx.lib
@@ -130,17 +148,25 @@ def main():
     def step(timed):
         if timed:
             ev[0].record(stream)
-        scene.render_paths(cam, opt, paths.data_ptr(), stream)
+        if a.adjoint == "checkpoint":
+            scene.render_fwd_ckpt(cam, opt, film.data_ptr(), ckpt.data_ptr(), nb, stream)
+        else:
+            scene.render_paths(cam, opt, paths.data_ptr(), stream)
         if timed:
             ev[1].record(stream)
-        # film = mean over samples (deterministic reduction kernel)
-        scene.render_fwd_reduce(paths.data_ptr(), spp, W * H, film.data_ptr(), stream)
+        if a.adjoint == "replay":
+            # film = mean over samples (deterministic reduction kernel)
+            scene.render_fwd_reduce(paths.data_ptr(), spp, W * H, film.data_ptr(), stream)
         dldi = film - target
         dpos.zero_()
         dmat.zero_()
         if timed:
             ev[2].record(stream)
-        scene.render_adj(cam, opt, dldi.data_ptr(), dpos.data_ptr(), dmat.data_ptr(), stream)
+        if a.adjoint == "checkpoint":
+            scene.render_adj_ckpt(cam, opt, ckpt.data_ptr(), nb, dldi.data_ptr(), dpos.data_ptr(), dmat.data_ptr(),
+                                  stream=stream)
+        else:
+            scene.render_adj(cam, opt, dldi.data_ptr(), dpos.data_ptr(), dmat.data_ptr(), stream)
         if timed:
             ev[3].record(stream)
 
@@ -158,13 +184,16 @@ def main():
     ms_per_step = (t1 - t0) * 1e3 / a.steps
     mpaths = P / (ms_per_step * 1e-3) / 1e6
 
-    b_fwd, b_adj, st = algorithmic_bytes(stats, P, B, info.num_emitter_triangles)
+    bb, st = algorithmic_bytes(stats, P, B, info.num_emitter_triangles)
     fwd_avg = sum(fwd_ms) / len(fwd_ms)
     adj_avg = sum(adj_ms) / len(adj_ms)
+    ck = a.adjoint == "checkpoint"
     if adj_avg >= fwd_avg:
-        kname, kms, kbytes = "k_render_adj", adj_avg, b_adj * P
+        kname, kms = ("k_render_adj_ckpt", adj_avg) if ck else ("k_render_adj", adj_avg)
+        kbytes = (bb["adj_ckpt"] if ck else bb["adj"]) * P
     else:
-        kname, kms, kbytes = "k_render_paths", fwd_avg, b_fwd * P
+        kname, kms = "k_render_paths", fwd_avg
+        kbytes = (bb["fwd_ckpt"] if ck else bb["fwd"]) * P
     achieved = kbytes / (kms * 1e-3)
     traffic = None
     tpath = os.path.join(HERE, "profiles", "traffic.json")
@@ -186,7 +215,7 @@ def main():
         del src, dst
     cpu = None
     if not a.no_cpu_baseline:
-        cpu = cpu_baseline(spec, cam_d, W, H, B, spp, a.cpu_spp)
+        cpu = cpu_baseline(spec, cam_d, W, H, B, spp, a.cpu_spp, a.cpu_runs)
     out = {
         "metric": "Mpaths/s fwd+adjoint (512², depth 4) & p99 latency, served vs plain, 1 GPU",
         "value": mpaths,
@@ -202,7 +231,8 @@ def main():
         "data": "synthetic (seeded displaced icosphere scene)",
         "config": {"workload": f"plain: one request per launch sequence, {W}x{H}, {spp} spp, depth {B + 1}, "
                                f"{info.num_triangles} triangles (20,480-tri displaced icosphere + ground + light), "
-                               f"forward + adjoint", "mode": "plain", "paths_per_step": P},
+                               f"forward + adjoint ({a.adjoint})", "mode": "plain", "adjoint": a.adjoint,
+                   "paths_per_step": P},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

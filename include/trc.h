@@ -163,6 +163,49 @@ trc_status trc_render_stats(trc_scene* scene, const trc_camera* cam, const trc_o
  * Samples per pixel 0 skips the term.  Their forward value is zero. */
 trc_status trc_render_boundary(trc_scene* scene, const trc_camera* cam, const trc_options* opt, int32_t kind,
                                const float* dLdI, float* dpos, trc_stream stream);
+/* Path checkpoint (adjoint without re-tracing).  The checkpoint holds, per
+ * path, the outcome of every ray query -- hit triangle and barycentrics of
+ * each vertex and the NEE visibility bits -- in
+ * trc_checkpoint_bytes = spp*W*H * (4 + 12 * (max_bounces + 1)) bytes of
+ * caller-owned device memory (-1 for invalid arguments).  The adjoint then
+ * re-evaluates each path from its recorded hits instead of replaying the
+ * traversal (PathTracer::RenderD replays the whole forward pass,
+ * PathTracer.cpp:107-111); the result equals trc_render_adj's. */
+int64_t trc_checkpoint_bytes(const trc_camera* cam, const trc_options* opt);
+/* trc_render_fwd that also writes the checkpoint. */
+trc_status trc_render_fwd_ckpt(trc_scene* scene, const trc_camera* cam, const trc_options* opt, float* film,
+                               void* ckpt, int64_t ckpt_bytes, trc_stream stream);
+/* trc_render_adj from a checkpoint written by trc_render_fwd_ckpt with the
+ * same scene state, camera and options. */
+trc_status trc_render_adj_ckpt(trc_scene* scene, const trc_camera* cam, const trc_options* opt, const void* ckpt,
+                               int64_t ckpt_bytes, const float* dLdI, float* dpos, float* dmat, float* dtex,
+                               trc_stream stream);
+/* ---- batched rendering: many requests (scenes, cameras, sizes, depths) in
+ * one launch.  No counterpart in the reference, which renders one request
+ * per call (TensorRay.cpp:192-221); this is the device side of the serving
+ * loop.  A job covers paths [first_path, first_path + num_paths) of its
+ * request's spp*W*H paths (path index = sample * W*H + pixel), so a request
+ * can be split over several batches.  Each path's result is the same as in
+ * a trc_render_fwd_ckpt / trc_render_adj_ckpt call of the whole request. */
+typedef struct trc_job {
+  trc_scene* scene;
+  trc_camera cam;
+  trc_options opt;       /* seed, max_bounces, spp */
+  uint64_t first_path, num_paths;
+  float* paths;          /* forward: per-path radiance [spp*W*H][3] of the request */
+  void* ckpt;            /* the request's checkpoint (trc_checkpoint_bytes): forward writes, adjoint reads */
+  const float* dLdI;     /* adjoint: [3][H*W] */
+  float* dpos;           /* adjoint: [V][3] */
+  float* dmat;           /* adjoint: [M][8] or NULL */
+  float* dtex;           /* adjoint: texel gradients or NULL */
+  float* vn_scratch;     /* adjoint: zeroed [V][3] scratch when the scene has smooth differentiable meshes
+                            (left zeroed on return), else NULL */
+} trc_job;
+/* adjoint = 0: forward of every job (paths and checkpoint); 1: adjoint of
+ * every job from its checkpoint.  Stream-ordered; jobs of one request may
+ * appear in different batches but not twice in one. */
+trc_status trc_render_batch(trc_ctx* ctx, const trc_job* jobs, int32_t n, int32_t adjoint, trc_stream stream);
+
 /* Adjoint: accumulates dL/dV += J^T dLdI into dpos ([V][3] device), the
  * material gradients into dmat ([M][8] device, may be NULL) and the texel
  * gradients of albedo / kd textures into dtex (may be NULL; all textures'

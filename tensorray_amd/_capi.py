@@ -36,6 +36,13 @@ class trc_options(ctypes.Structure):
                 ("g_direct_spp", ctypes.c_int32), ("g_direct_thold", ctypes.c_float), ("g_eps", ctypes.c_float)]
 
 
+class trc_job(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_void_p), ("cam", trc_camera), ("opt", trc_options),
+                ("first_path", ctypes.c_uint64), ("num_paths", ctypes.c_uint64), ("paths", ctypes.c_void_p),
+                ("ckpt", ctypes.c_void_p), ("dLdI", ctypes.c_void_p), ("dpos", ctypes.c_void_p),
+                ("dmat", ctypes.c_void_p), ("dtex", ctypes.c_void_p), ("vn_scratch", ctypes.c_void_p)]
+
+
 class trc_scene_info(ctypes.Structure):
     _fields_ = [("num_vertices", ctypes.c_int32), ("num_triangles", ctypes.c_int32),
                 ("num_materials", ctypes.c_int32), ("num_emitter_triangles", ctypes.c_int32),
@@ -78,6 +85,12 @@ SIGNATURES = [
                                              ctypes.c_int32, vp, vp, vp]),
     ("trc_render_adj", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options), vp, vp, vp,
                                         vp, vp]),
+    ("trc_checkpoint_bytes", ctypes.c_int64, [ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options)]),
+    ("trc_render_batch", ctypes.c_int32, [vp, ctypes.POINTER(trc_job), ctypes.c_int32, ctypes.c_int32, vp]),
+    ("trc_render_fwd_ckpt", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options), vp, vp,
+                                             ctypes.c_int64, vp]),
+    ("trc_render_adj_ckpt", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options), vp,
+                                             ctypes.c_int64, vp, vp, vp, vp, vp]),
 ]
 
 _lib = None
@@ -135,6 +148,12 @@ class Context:
     @property
     def num_cus(self):
         return self.lib.trc_device_cus(self.h)
+
+    def render_batch(self, jobs, adjoint, stream=None):
+        """jobs: list of trc_job."""
+        arr = (trc_job * len(jobs))(*jobs)
+        self.check(self.lib.trc_render_batch(self.h, arr, len(jobs), 1 if adjoint else 0, _stream_ptr(stream)),
+                   "trc_render_batch")
 
     def close(self):
         if self.h is not None and self.h.value:
@@ -278,6 +297,33 @@ class DeviceScene:
     def render_adj(self, cam, opt, dldi_ptr, dpos_ptr, dmat_ptr=None, stream=None, dtex_ptr=None):
         self.ctx.check(self.ctx.lib.trc_render_adj(self.h, ctypes.byref(cam), ctypes.byref(opt), dldi_ptr, dpos_ptr,
                                                    dmat_ptr, dtex_ptr, _stream_ptr(stream)), "trc_render_adj")
+
+    def checkpoint_bytes(self, cam, opt):
+        return int(self.ctx.lib.trc_checkpoint_bytes(ctypes.byref(cam), ctypes.byref(opt)))
+
+    def render_fwd_ckpt(self, cam, opt, film_ptr, ckpt_ptr, ckpt_bytes, stream=None):
+        self.ctx.check(self.ctx.lib.trc_render_fwd_ckpt(self.h, ctypes.byref(cam), ctypes.byref(opt), film_ptr,
+                                                        ckpt_ptr, ckpt_bytes, _stream_ptr(stream)),
+                       "trc_render_fwd_ckpt")
+
+    def render_adj_ckpt(self, cam, opt, ckpt_ptr, ckpt_bytes, dldi_ptr, dpos_ptr, dmat_ptr=None, dtex_ptr=None,
+                        stream=None):
+        self.ctx.check(self.ctx.lib.trc_render_adj_ckpt(self.h, ctypes.byref(cam), ctypes.byref(opt), ckpt_ptr,
+                                                        ckpt_bytes, dldi_ptr, dpos_ptr, dmat_ptr, dtex_ptr,
+                                                        _stream_ptr(stream)), "trc_render_adj_ckpt")
+
+    def job(self, cam, opt, first, count, paths=None, ckpt=None, dldi=None, dpos=None, dmat=None, dtex=None,
+            vn_scratch=None):
+        """A trc_job of this scene (pointers are device addresses or None)."""
+        j = trc_job()
+        j.scene = self.h.value
+        j.cam = cam
+        j.opt = opt
+        j.first_path = first
+        j.num_paths = count
+        j.paths, j.ckpt, j.dLdI, j.dpos = paths, ckpt, dldi, dpos
+        j.dmat, j.dtex, j.vn_scratch = dmat, dtex, vn_scratch
+        return j
 
     BOUNDARY_KINDS = {"primary": 0, "pixel": 1, "direct": 2, "indirect": 3}
 

@@ -73,13 +73,16 @@ def build_pymodule():
     src = os.path.join(CSRC, "pybind", "tensorray_module.cpp")
     if not os.path.exists(src):
         return None
+    srcs = [src, os.path.join(CSRC, "pybind", "host_scene.cpp")]
     import pybind11
     suffix = sysconfig.get_config_var("EXT_SUFFIX")
     out = os.path.join(HERE, "TensorRay" + suffix)
     tmp = out + ".tmp"
     cmd = ["g++"] + HOST_FLAGS + [
         "-shared", "-I" + pybind11.get_include(), "-I" + sysconfig.get_paths()["include"],
-        src, "-o", tmp, "-L" + HERE, "-ltrc", "-Wl,-rpath,$ORIGIN",
+        "-fvisibility=hidden", "-Wno-deprecated-declarations",
+    ] + srcs + [
+        "-o", tmp, "-L" + HERE, "-ltrc", "-Wl,-rpath,$ORIGIN",
         "-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath," + os.path.join(ROCM, "lib"),
     ]
     _run(cmd)

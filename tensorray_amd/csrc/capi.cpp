@@ -46,6 +46,25 @@ std::map<std::pair<trc_ctx*, hipStream_t>, uint32_t*>& spill_map() {
   return m;
 }
 
+}  // namespace
+
+namespace trc {
+int resident_blocks_per_cu(const void* kernel, int block) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, int> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto key = std::make_pair(kernel, block);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess) per_cu = 0;
+  cache.emplace(key, per_cu);
+  return per_cu;
+}
+}  // namespace trc
+
+namespace {
+
 LaunchCfg make_cfg(trc_ctx* ctx, const trc_scene* s, hipStream_t st, long long work_items) {
   LaunchCfg L;
   int max_grid = ctx->num_cus * kBlocksPerCU;
@@ -56,6 +75,7 @@ LaunchCfg make_cfg(trc_ctx* ctx, const trc_scene* s, hipStream_t st, long long w
     throw Fail(TRC_ERR_INVALID, "BVH too deep for the traversal stack (" + std::to_string(stack_need) + ")");
   L.cap = kStackLds + kSpillDepth;
   L.err = ctx->err_flag;
+  L.cus = ctx->num_cus;
   std::lock_guard<std::mutex> lk(ctx->mu);
   auto key = std::make_pair(ctx, st);
   auto it = spill_map().find(key);
@@ -233,6 +253,14 @@ trc_status trc_context_destroy(trc_ctx* ctx) {
         ++it;
       }
     }
+  }
+  for (auto& b : ctx->batch) {
+    if (b.done) {
+      hipEventSynchronize(b.done);
+      hipEventDestroy(b.done);
+    }
+    if (b.host) hipHostFree(b.host);
+    if (b.dev) hipFree(b.dev);
   }
   if (ctx->err_flag) hipFree(ctx->err_flag);
   delete ctx;
@@ -473,27 +501,147 @@ trc_status trc_render_paths(trc_scene* s, const trc_camera* cam, const trc_optio
     if (!paths) throw Fail(TRC_ERR_INVALID, "paths buffer is NULL");
     long long np = (long long)opt->spp * C.width * C.height;
     LaunchCfg L = make_cfg(s->ctx, s, (hipStream_t)stream, np);
-    hip_check(launch_render_paths(s->dev, C, opt->seed, opt->max_bounces, opt->spp, L, paths, (hipStream_t)stream),
+    hip_check(launch_render_paths(s->dev, C, opt->seed, opt->max_bounces, opt->spp, L, paths, nullptr,
+                                  (hipStream_t)stream),
               "k_render_paths");
   });
+}
+
+static void render_fwd_impl(trc_scene* s, const trc_camera* cam, const trc_options* opt, float* film, void* ckpt,
+                     int64_t ckpt_bytes, trc_stream stream) {
+  check_render(s, opt);
+  CameraDev C = to_cam(cam);
+  if (!film) throw Fail(TRC_ERR_INVALID, "film is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  long long np = (long long)opt->spp * C.width * C.height;
+  if (ckpt && ckpt_bytes < (int64_t)checkpoint_bytes((uint64_t)np, opt->max_bounces))
+    throw Fail(TRC_ERR_INVALID, "checkpoint buffer too small (trc_checkpoint_bytes)");
+  float* paths = nullptr;
+  hip_check(hipMallocAsync((void**)&paths, (size_t)np * 12, st), "hipMallocAsync(paths)");
+  LaunchCfg L = make_cfg(s->ctx, s, st, np);
+  hipError_t e = launch_render_paths(s->dev, C, opt->seed, opt->max_bounces, opt->spp, L, paths, ckpt, st);
+  if (e == hipSuccess) e = launch_film_reduce(paths, opt->spp, C.width * C.height, film, st);
+  hipFreeAsync(paths, st);
+  hip_check(e, "k_render_paths/k_film_reduce");
 }
 
 trc_status trc_render_fwd(trc_scene* s, const trc_camera* cam, const trc_options* opt, float* film,
                           trc_stream stream) {
   if (!s) return TRC_ERR_INVALID;
+  return guarded(s->ctx, [&] { render_fwd_impl(s, cam, opt, film, nullptr, 0, stream); });
+}
+
+int64_t trc_checkpoint_bytes(const trc_camera* cam, const trc_options* opt) {
+  if (!cam || !opt || cam->width <= 0 || cam->height <= 0 || opt->spp <= 0 || opt->max_bounces < 0 ||
+      opt->max_bounces > kMaxBounces)
+    return -1;
+  const uint64_t np = (uint64_t)opt->spp * (uint64_t)cam->width * (uint64_t)cam->height;
+  return (int64_t)checkpoint_bytes(np, opt->max_bounces);
+}
+
+trc_status trc_render_fwd_ckpt(trc_scene* s, const trc_camera* cam, const trc_options* opt, float* film,
+                               void* ckpt, int64_t ckpt_bytes, trc_stream stream) {
+  if (!s) return TRC_ERR_INVALID;
+  return guarded(s->ctx, [&] {
+    if (!ckpt) throw Fail(TRC_ERR_INVALID, "checkpoint is NULL");
+    render_fwd_impl(s, cam, opt, film, ckpt, ckpt_bytes, stream);
+  });
+}
+
+trc_status trc_render_batch(trc_ctx* ctx, const trc_job* jobs, int32_t n, int32_t adjoint, trc_stream stream) {
+  if (!ctx) return TRC_ERR_INVALID;
+  return guarded(ctx, [&] {
+    if (ctx->device < 0) throw Fail(TRC_ERR_STATE, "host-only context cannot render");
+    if (n < 0 || (n > 0 && !jobs)) throw Fail(TRC_ERR_INVALID, "bad job list");
+    if (n == 0) return;
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<JobDev> jd((size_t)n);
+    const trc_scene* deepest = nullptr;
+    uint64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+      const trc_job& J = jobs[i];
+      if (!J.scene || J.scene->ctx != ctx) throw Fail(TRC_ERR_INVALID, "job " + std::to_string(i) + ": bad scene");
+      check_render(J.scene, &J.opt);
+      CameraDev C = to_cam(&J.cam);
+      const uint64_t np = (uint64_t)J.opt.spp * (uint64_t)C.width * (uint64_t)C.height;
+      if (J.num_paths == 0 || J.first_path + J.num_paths > np)
+        throw Fail(TRC_ERR_INVALID, "job " + std::to_string(i) + ": path range outside spp*W*H");
+      if (!J.ckpt) throw Fail(TRC_ERR_INVALID, "job " + std::to_string(i) + ": checkpoint is NULL");
+      if (!adjoint && !J.paths) throw Fail(TRC_ERR_INVALID, "job " + std::to_string(i) + ": paths is NULL");
+      if (adjoint && (!J.dLdI || !J.dpos)) throw Fail(TRC_ERR_INVALID, "job " + std::to_string(i) + ": dLdI / dpos is NULL");
+      if (adjoint && J.scene->dev.vf_off && !J.vn_scratch)
+        throw Fail(TRC_ERR_INVALID, "job " + std::to_string(i) + ": smooth differentiable mesh needs vn_scratch");
+      JobDev& d = jd[(size_t)i];
+      d.S = J.scene->dev;
+      d.C = C;
+      d.seed = J.opt.seed;
+      d.max_bounces = J.opt.max_bounces;
+      d.spp = J.opt.spp;
+      d.pad = 0;
+      d.first = J.first_path;
+      d.count = J.num_paths;
+      d.paths = J.paths;
+      d.ckpt = J.ckpt;
+      d.dLdI = J.dLdI;
+      d.dpos = J.dpos;
+      d.dmat = J.dmat;
+      d.dtex = J.dtex;
+      d.vn = J.scene->dev.vf_off ? J.vn_scratch : nullptr;
+      total += J.num_paths;
+      if (!deepest || J.scene->dev.stack_need > deepest->dev.stack_need) deepest = J.scene;
+    }
+    // tile size: a few paths per lane of the resident grid, at least 256
+    LaunchCfg L = make_cfg(ctx, deepest, st, (long long)total);
+    const uint64_t lanes = (uint64_t)ctx->num_cus * kBlocksPerCU * kBlock;
+    uint64_t tile = (total + lanes - 1) / lanes * 64;
+    tile = tile < 256 ? 256 : (tile > 65536 ? 65536 : tile);
+    tile = (tile + 63) / 64 * 64;
+    uint64_t ntiles = 0;
+    for (JobDev& d : jd) {
+      d.tile0 = ntiles;
+      ntiles += (d.count + tile - 1) / tile;
+    }
+    // job table: pinned staging -> device (stream-ordered), from a ring slot
+    // whose previous batch has completed
+    const size_t bytes = jd.size() * sizeof(JobDev);
+    std::lock_guard<std::mutex> lk(ctx->batch_mu);
+    trc_ctx::BatchSlot& b = ctx->batch[ctx->batch_next++ % trc_ctx::kBatchSlots];
+    if (b.done) hip_check(hipEventSynchronize(b.done), "hipEventSynchronize");
+    else hip_check(hipEventCreateWithFlags(&b.done, hipEventDisableTiming), "hipEventCreate");
+    if (b.cap < bytes) {
+      if (b.host) hipHostFree(b.host);
+      if (b.dev) hipFree(b.dev);
+      b.host = nullptr;
+      b.dev = nullptr;
+      b.cap = 0;
+      hip_check(hipHostMalloc(&b.host, bytes, hipHostMallocDefault), "hipHostMalloc(jobs)");
+      hip_check(hipMalloc(&b.dev, bytes), "hipMalloc(jobs)");
+      b.cap = bytes;
+    }
+    std::memcpy(b.host, jd.data(), bytes);
+    hip_check(hipMemcpyAsync(b.dev, b.host, bytes, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    hip_check(launch_render_batch((const JobDev*)b.dev, jd.data(), n, ntiles, (uint32_t)tile, adjoint, L, st),
+              adjoint ? "k_render_batch_adj" : "k_render_batch_fwd");
+    hip_check(hipEventRecord(b.done, st), "hipEventRecord");
+  });
+}
+
+trc_status trc_render_adj_ckpt(trc_scene* s, const trc_camera* cam, const trc_options* opt, const void* ckpt,
+                               int64_t ckpt_bytes, const float* dLdI, float* dpos, float* dmat, float* dtex,
+                               trc_stream stream) {
+  if (!s) return TRC_ERR_INVALID;
   return guarded(s->ctx, [&] {
     check_render(s, opt);
     CameraDev C = to_cam(cam);
-    if (!film) throw Fail(TRC_ERR_INVALID, "film is NULL");
-    hipStream_t st = (hipStream_t)stream;
+    if (!dLdI || !dpos) throw Fail(TRC_ERR_INVALID, "dLdI / dpos is NULL");
+    if (!ckpt) throw Fail(TRC_ERR_INVALID, "checkpoint is NULL");
     long long np = (long long)opt->spp * C.width * C.height;
-    float* paths = nullptr;
-    hip_check(hipMallocAsync((void**)&paths, (size_t)np * 12, st), "hipMallocAsync(paths)");
-    LaunchCfg L = make_cfg(s->ctx, s, st, np);
-    hipError_t e = launch_render_paths(s->dev, C, opt->seed, opt->max_bounces, opt->spp, L, paths, st);
-    if (e == hipSuccess) e = launch_film_reduce(paths, opt->spp, C.width * C.height, film, st);
-    hipFreeAsync(paths, st);
-    hip_check(e, "k_render_paths/k_film_reduce");
+    if (ckpt_bytes < (int64_t)checkpoint_bytes((uint64_t)np, opt->max_bounces))
+      throw Fail(TRC_ERR_INVALID, "checkpoint buffer too small (trc_checkpoint_bytes)");
+    LaunchCfg L = make_cfg(s->ctx, s, (hipStream_t)stream, np);
+    hip_check(launch_render_adj_ckpt(s->dev, C, opt->seed, opt->max_bounces, opt->spp, L, ckpt, dLdI, dpos, dmat,
+                                     dtex, s->vn_adj, (hipStream_t)stream),
+              "k_render_adj_ckpt");
   });
 }
 

@@ -26,6 +26,7 @@ __device__ inline TraceCtx make_trace_ctx_b(uint32_t* lds, const LaunchCfg& L) {
   t.lds = lds + threadIdx.x;
   t.lds_stride = kBlock;
   t.lds_depth = kStackLds;
+  t.leafq = lds + kStackLds * kBlock + threadIdx.x;
   const int gtid = blockIdx.x * kBlock + threadIdx.x;
   t.spill = L.spill + gtid;
   t.spill_stride = L.grid * kBlock;
@@ -271,7 +272,7 @@ struct BoundaryParams {
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_boundary_primary(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
                                                              EdgeSet E, uint64_t N, const float* dLdI, float* dpos) {
-  __shared__ uint32_t lds_stack[kStackLds * kBlock];
+  __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
   const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
   const F3 eye = cam_eye(C);
@@ -312,7 +313,7 @@ __global__ void __launch_bounds__(kBlock) k_boundary_primary(SceneDev S, CameraD
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_boundary_pixel(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
                                                            uint64_t N, const float* dLdI, float* dpos) {
-  __shared__ uint32_t lds_stack[kStackLds * kBlock];
+  __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
   const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
   const F3 eye = cam_eye(C);
@@ -370,7 +371,7 @@ __global__ void __launch_bounds__(kBlock) k_boundary_pixel(SceneDev S, CameraDev
 // and records |baseVal| / spp of the guiding-mode evaluation, where only
 // the two hit tests and the same-point test of Edge.cpp:201-262 apply.
 __global__ void __launch_bounds__(kBlock) k_guide_eval(SceneDev S, LaunchCfg L, uint32_t seed, GuideDev G) {
-  __shared__ uint32_t lds_stack[kStackLds * kBlock];
+  __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
   const EdgeSet& E = S.sec;
   const uint64_t N = (uint64_t)G.n * (uint64_t)G.spp;
@@ -471,7 +472,7 @@ __global__ void __launch_bounds__(kGuideBlock) k_guide_finish(const float* sec_c
 
 __global__ void __launch_bounds__(kBlock) k_boundary_direct(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
                                                             GuideDev G, uint64_t N, const float* dLdI, float* dpos) {
-  __shared__ uint32_t lds_stack[kStackLds * kBlock];
+  __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
   const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
   const EdgeSet& E = S.sec;
@@ -543,7 +544,7 @@ __global__ void __launch_bounds__(kBlock) k_boundary_direct(SceneDev S, CameraDe
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_boundary_indirect(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
                                                               uint64_t N, const float* dLdI, float* dpos) {
-  __shared__ uint32_t lds_stack[kStackLds * kBlock];
+  __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
   const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
   const EdgeSet& E = S.sec;
@@ -631,7 +632,8 @@ __global__ void __launch_bounds__(kBlock) k_boundary_indirect(SceneDev S, Camera
 
 hipError_t launch_guide_build(const SceneDev& S, const LaunchCfg& L, uint32_t seed, const GuideDev& g,
                               hipStream_t st) {
-  hipLaunchKernelGGL(k_guide_eval, dim3(L.grid), dim3(kBlock), 0, st, S, L, seed, g);
+  const LaunchCfg Lg = fit_grid(k_guide_eval, L);
+  hipLaunchKernelGGL(k_guide_eval, dim3(Lg.grid), dim3(kBlock), 0, st, S, Lg, seed, g);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_guide_finish, dim3(1), dim3(kGuideBlock), 0, st, S.sec.cdf, g);
@@ -643,14 +645,19 @@ hipError_t launch_boundary(int kind, const SceneDev& S, const CameraDev& C, uint
                            float* dpos, hipStream_t st) {
   BoundaryParams P{seed, max_bounces, spp, 0u};
   uint64_t N = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
-  if (kind == 0)
-    hipLaunchKernelGGL(k_boundary_primary, dim3(L.grid), dim3(kBlock), 0, st, S, C, P, L, prim, N, dLdI, dpos);
-  else if (kind == 1)
-    hipLaunchKernelGGL(k_boundary_pixel, dim3(L.grid), dim3(kBlock), 0, st, S, C, P, L, N, dLdI, dpos);
-  else if (kind == 2)
-    hipLaunchKernelGGL(k_boundary_direct, dim3(L.grid), dim3(kBlock), 0, st, S, C, P, L, guide, N, dLdI, dpos);
-  else
-    hipLaunchKernelGGL(k_boundary_indirect, dim3(L.grid), dim3(kBlock), 0, st, S, C, P, L, N, dLdI, dpos);
+  if (kind == 0) {
+    const LaunchCfg Lk = fit_grid(k_boundary_primary, L);
+    hipLaunchKernelGGL(k_boundary_primary, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, prim, N, dLdI, dpos);
+  } else if (kind == 1) {
+    const LaunchCfg Lk = fit_grid(k_boundary_pixel, L);
+    hipLaunchKernelGGL(k_boundary_pixel, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, N, dLdI, dpos);
+  } else if (kind == 2) {
+    const LaunchCfg Lk = fit_grid(k_boundary_direct, L);
+    hipLaunchKernelGGL(k_boundary_direct, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, guide, N, dLdI, dpos);
+  } else {
+    const LaunchCfg Lk = fit_grid(k_boundary_indirect, L);
+    hipLaunchKernelGGL(k_boundary_indirect, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, N, dLdI, dpos);
+  }
   return hipGetLastError();
 }
 

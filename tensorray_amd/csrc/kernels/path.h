@@ -39,13 +39,85 @@ struct PathRec {
 
 TR_HD F3 nan_to_zero(F3 v) { return finite3(v) ? v : F3{0.0f, 0.0f, 0.0f}; }
 
+// Path checkpoint: the outcome of every ray query of a path, written by the
+// forward pass so that the adjoint re-evaluates the path without tracing.
+// Per path: meta = (K + 1) | (NEE visibility of bounce b) << (8 + b), where
+// K is the last hit vertex (-1: the camera ray missed), and for each vertex
+// i <= K the hit triangle and barycentrics.  Fields are [vertex][path]
+// planes, so lanes with consecutive paths touch consecutive words.
+struct CkptDev {
+  uint32_t* meta;   // [npaths]
+  int32_t* tri;     // [nv][npaths]
+  float* u;         // [nv][npaths]
+  float* v;         // [nv][npaths]
+  uint64_t npaths;
+  int32_t nv;       // max_bounces + 1
+};
+// 4 bytes of meta + 12 bytes per vertex.
+TR_HD uint64_t ckpt_bytes(uint64_t npaths, int max_bounces) {
+  return npaths * (4ull + 12ull * (uint64_t)(max_bounces + 1));
+}
+TR_HD CkptDev ckpt_view(void* base, uint64_t npaths, int max_bounces) {
+  CkptDev c;
+  const uint64_t nv = (uint64_t)(max_bounces + 1);
+  c.meta = (uint32_t*)base;
+  c.tri = (int32_t*)(c.meta + npaths);
+  c.u = (float*)(c.tri + nv * npaths);
+  c.v = c.u + nv * npaths;
+  c.npaths = npaths;
+  c.nv = (int32_t)nv;
+  return c;
+}
+__device__ inline void ckpt_put_vertex(const CkptDev& c, uint64_t path, int i, int32_t tri, float u, float v) {
+  const uint64_t k = (uint64_t)i * c.npaths + path;
+  c.tri[k] = tri;
+  c.u[k] = u;
+  c.v[k] = v;
+}
+
+// Where a path's ray queries come from: the BVH (TraceCtx) or a checkpoint.
+__device__ inline Hit src_camera(const SceneDev& S, TraceCtx& tc, F3 o, F3 d) {
+  return trace_ray<false>(S, tc, o, d, kShadowEpsilon, 1e32f);
+}
+__device__ inline bool src_visible(const SceneDev& S, TraceCtx& tc, int b, F3 o, F3 wi, float dist) {
+  return trace_ray<true>(S, tc, o, wi, kShadowEpsilon, dist - kShadowEpsilon).tri < 0;
+}
+__device__ inline Hit src_bsdf(const SceneDev& S, TraceCtx& tc, int b, F3 o, F3 d) {
+  return trace_ray<false>(S, tc, o, d, kShadowEpsilon, 1e32f);
+}
+
+struct CkptSrc {
+  const CkptDev* c;
+  uint64_t path;
+  uint32_t meta;
+  __device__ Hit vertex(int i) const {
+    Hit h;
+    const int K = (int)(meta & 0xFFu) - 1;
+    if (i > K) {
+      h.tri = -1; h.u = 0.0f; h.v = 0.0f; h.t = -1.0f;
+      return h;
+    }
+    const uint64_t k = (uint64_t)i * c->npaths + path;
+    h.tri = c->tri[k];
+    h.u = c->u[k];
+    h.v = c->v[k];
+    h.t = 0.0f;
+    return h;
+  }
+};
+__device__ inline Hit src_camera(const SceneDev&, CkptSrc& s, F3, F3) { return s.vertex(0); }
+__device__ inline bool src_visible(const SceneDev&, CkptSrc& s, int b, F3, F3, float) {
+  return ((s.meta >> (8 + b)) & 1u) != 0u;
+}
+__device__ inline Hit src_bsdf(const SceneDev&, CkptSrc& s, int b, F3, F3) { return s.vertex(b + 1); }
+
 // Bounce loop (PathTracer.cpp:55-73, EvalRadianceDirect Integrator.cpp:178-303)
 // from surface point `its` reached along `raydir` with throughput `beta`.
 // Bounce b draws blocks block0 + 2b (light) and block0 + 2b + 1 (BSDF) of
 // (seed, domain, index).  kRecord fills the replay record (interior
 // adjoint); kPerBounce stores each executed bounce's contribution.
-template <bool kRecord, bool kPerBounce>
-__device__ void path_bounces(const SceneDev& S, TraceCtx& tc, Its its, F3 beta, F3 raydir, uint32_t seed,
+template <bool kRecord, bool kPerBounce, class Src>
+__device__ void path_bounces(const SceneDev& S, Src& tc, Its its, F3 beta, F3 raydir, uint32_t seed,
                              uint32_t domain, uint64_t index, uint32_t block0, int B, F3& contrib, PathRec* R,
                              F3* per_bounce) {
   const F3 zero{0.0f, 0.0f, 0.0f};
@@ -66,8 +138,7 @@ __device__ void path_bounces(const SceneDev& S, TraceCtx& tc, Its its, F3 beta, 
       bool stable = dist > kShadowEpsilon && fabsf(dsh) > kEdgeEpsilon;
       bool noleak = (dsh * dgeo) > 0.0f;
       bool visible = false;
-      if (stable && noleak)
-        visible = trace_ray<true>(S, tc, its.p, wi, kShadowEpsilon, dist - kShadowEpsilon).tri < 0;
+      if (stable && noleak) visible = src_visible(S, tc, b, its.p, wi, dist);
       float valid = (stable && noleak && visible) ? 1.0f : 0.0f;
       float Gl = fabsf(dot(ls.n, -wi)) / d2;
       F3 f = bsdf_eval(S, its, wo, wi) * fabsf(dsh);
@@ -88,7 +159,7 @@ __device__ void path_bounces(const SceneDev& S, TraceCtx& tc, Its its, F3 beta, 
     // ---- BSDF sampling ----
     float pdfb;
     F3 wib = bsdf_sample(S, its, wo, rb.a, rb.b, &pdfb);
-    Hit h2 = trace_ray<false>(S, tc, its.p, wib, kShadowEpsilon, 1e32f);
+    Hit h2 = src_bsdf(S, tc, b, its.p, wib);
     bool ok = h2.tri >= 0 && pdfb > 0.0f && bsdf_dir_valid(S, its, wib);
     if (!ok) {
       if (finite3(ret)) {
@@ -140,9 +211,9 @@ __device__ void path_bounces(const SceneDev& S, TraceCtx& tc, Its its, F3 beta, 
   }
 }
 
-template <bool kRecord>
+template <bool kRecord, class Src>
 __device__ F3 path_forward(const SceneDev& S, const CameraDev& C, const PathParams& P, uint64_t path,
-                           TraceCtx& tc, PathRec* R) {
+                           Src& tc, PathRec* R) {
   const F3 zero{0.0f, 0.0f, 0.0f};
   const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
   const uint32_t pix = (uint32_t)(path % WH);
@@ -153,7 +224,7 @@ __device__ F3 path_forward(const SceneDev& S, const CameraDev& C, const PathPara
   const F3 eye{C.eye[0], C.eye[1], C.eye[2]};
   const F3 cdir{C.dir[0], C.dir[1], C.dir[2]};
   F3 d0 = camera_dir(C, (float)px + rc.a, (float)py + rc.b);
-  Hit h = trace_ray<false>(S, tc, eye, d0, kShadowEpsilon, 1e32f);
+  Hit h = src_camera(S, tc, eye, d0);
   if (h.tri < 0) return zero;
   Its its = post_intersect(S, h.tri, h.u, h.v);
   // EvalFilter: G / detach(G) with G = |cos_x| / (cos_y^3 d^2); times J
@@ -186,6 +257,227 @@ __device__ F3 path_forward(const SceneDev& S, const CameraDev& C, const PathPara
   path_bounces<kRecord, false>(S, tc, its, beta, raydir, P.seed, kRngInterior, path, 1u, P.max_bounces, contrib,
                                R, nullptr);
   return contrib;
+}
+
+// ---------------------------------------------------------------------------
+// Forward paths as a per-lane state machine with path regeneration.
+//
+// path_forward runs one path to completion per lane, so a wave waits for its
+// longest path and every traversal is entered by whichever lanes still live.
+// Here each loop iteration traces exactly one ray per lane -- the camera ray,
+// an NEE shadow ray or a BSDF ray, depending on the lane's stage -- then
+// shades it; a lane whose path ends takes the next path of its wave's chunk
+// (ballot + popcount hand-out, no global atomics).  Each path evaluates the
+// same expressions in the same order as path_forward<false>, so the
+// per-path radiance is identical.
+// ---------------------------------------------------------------------------
+enum PathStage : uint32_t { kStageIdle = 0, kStageCamera = 1, kStageShadow = 2, kStageBsdf = 3 };
+
+struct PathLane {
+  uint64_t path;
+  uint32_t stage, b;
+  int32_t tri;           // current vertex x_b
+  float u, v;
+  F3 beta, raydir, contrib, ret;
+  float la, lb;          // light-sample uniforms of bounce b
+  float pdfb;            // BSDF-sample pdf of bounce b
+  F3 wib;                // BSDF-sampled direction of bounce b
+  // ray of this iteration
+  F3 o, d;
+  float tmin, tmax;
+  bool any;
+};
+
+// NEE geometry of bounce b at `its` (Integrator.cpp:196-213).
+struct NeeGeo {
+  LightSample ls;
+  F3 wi;
+  float d2, dist, dsh;
+  bool ray;  // stable && noleak: a shadow ray is traced
+};
+__device__ inline NeeGeo nee_geo(const SceneDev& S, const Its& its, float la, float lb) {
+  NeeGeo g;
+  g.ls = light_sample(S, la, lb);
+  F3 wi = g.ls.p - its.p;
+  g.d2 = len2(wi);
+  g.dist = sqrtf(g.d2);
+  g.wi = wi / g.dist;
+  g.dsh = dot(its.ns, g.wi);
+  float dgeo = dot(its.ng, g.wi);
+  bool stable = g.dist > kShadowEpsilon && fabsf(g.dsh) > kEdgeEpsilon;
+  bool noleak = (g.dsh * dgeo) > 0.0f;
+  g.ray = stable && noleak;
+  return g;
+}
+
+// Sets up the shadow ray of bounce L.b at vertex `its`.
+__device__ inline void lane_setup_nee(const SceneDev& S, const PathParams& P, const Its& its, PathLane& L) {
+  Rnd4 rl = rng4(P.seed, kRngInterior, L.path, 1u + 2u * L.b);
+  L.la = rl.a;
+  L.lb = rl.b;
+  NeeGeo g = nee_geo(S, its, L.la, L.lb);
+  L.stage = kStageShadow;
+  L.o = its.p;
+  L.d = g.wi;
+  L.tmin = kShadowEpsilon;
+  L.tmax = g.ray ? g.dist - kShadowEpsilon : -1.0f;
+  L.any = true;
+}
+
+// The wave renders paths [begin, end) (wave-uniform bounds).
+template <bool kCkpt, class Finish>
+__device__ void render_paths_regen(const SceneDev& S, const CameraDev& C, const PathParams& P, TraceCtx& tc,
+                                   uint64_t begin, uint64_t end, const CkptDev& ck, Finish finish) {
+  const F3 zero{0.0f, 0.0f, 0.0f};
+  const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
+  const F3 eye{C.eye[0], C.eye[1], C.eye[2]};
+  const F3 cdir{C.dir[0], C.dir[1], C.dir[2]};
+  const int lane = (int)(threadIdx.x & 63u);
+  uint64_t next = begin;  // wave-uniform
+  PathLane L;
+  L.stage = kStageIdle;
+  uint32_t meta = 0u;  // checkpoint meta word of the current path
+  for (;;) {
+    // ---- regenerate: idle lanes take the next paths of the wave's chunk ----
+    const bool idle = L.stage == kStageIdle;
+    const uint64_t idle_mask = __ballot(idle);
+    if (idle) {
+      const uint64_t rank = (uint64_t)__popcll(idle_mask & ((1ull << lane) - 1ull));
+      if (next + rank < end) {
+        L.path = next + rank;
+        const uint32_t pix = (uint32_t)(L.path % WH);
+        const uint32_t px = pix % (uint32_t)C.width, py = pix / (uint32_t)C.width;
+        Rnd4 rc = rng4(P.seed, kRngInterior, L.path, 0u);
+        L.stage = kStageCamera;
+        L.o = eye;
+        L.d = camera_dir(C, (float)px + rc.a, (float)py + rc.b);
+        L.tmin = kShadowEpsilon;
+        L.tmax = 1e32f;
+        L.any = false;
+      }
+    }
+    {
+      const uint64_t n_idle = (uint64_t)__popcll(idle_mask);
+      next = next + n_idle < end ? next + n_idle : end;
+    }
+    if (__ballot(L.stage != kStageIdle) == 0ull) break;
+    // ---- one ray per lane ----
+    Hit h;
+    if (L.stage != kStageIdle) h = trace_ray_rt(S, tc, L.o, L.d, L.tmin, L.tmax, L.any);
+    // ---- shade ----
+    if (L.stage == kStageCamera) {
+      if (h.tri < 0) {
+        if (kCkpt) ck.meta[L.path] = 0u;
+        finish(L.path, zero);
+        L.stage = kStageIdle;
+        continue;
+      }
+      if (kCkpt) {
+        ckpt_put_vertex(ck, L.path, 0, h.tri, h.u, h.v);
+        meta = 1u;
+      }
+      Its its = post_intersect(S, h.tri, h.u, h.v);
+      float beta0;
+      {
+        F3 cd = its.p - eye;
+        float dist = length(cd);
+        cd = cd / dist;
+        float cosy = dot(cd, cdir);
+        float cosx = dot(-cd, its.ng);
+        float G = fabsf(cosx) / (cosy * cosy * cosy * (dist * dist));
+        float area = S.shade[its.tri].area;
+        beta0 = (G / G) * (area / area);
+      }
+      L.beta = F3{beta0, beta0, beta0};
+      L.raydir = normalize(its.p - eye);
+      L.contrib = zero;
+      if (its.emitter >= 0) {
+        F3 Le = emitted(S, its.emitter, its.ng, -L.raydir);
+        F3 e0 = L.beta * Le;
+        if (finite3(e0)) L.contrib = L.contrib + e0;
+      }
+      L.tri = h.tri; L.u = h.u; L.v = h.v;
+      L.b = 0u;
+      if (P.max_bounces <= 0) {
+        if (kCkpt) ck.meta[L.path] = meta;
+        finish(L.path, L.contrib);
+        L.stage = kStageIdle;
+        continue;
+      }
+      lane_setup_nee(S, P, its, L);
+    } else if (L.stage == kStageShadow) {
+      Its its = post_intersect(S, L.tri, L.u, L.v);
+      NeeGeo g = nee_geo(S, its, L.la, L.lb);
+      const F3 wo = -L.raydir;
+      bool visible = g.ray && h.tri < 0;
+      if (kCkpt && visible) meta |= 1u << (8u + L.b);
+      float valid = visible ? 1.0f : 0.0f;
+      float Gl = fabsf(dot(g.ls.n, -g.wi)) / g.d2;
+      F3 f = bsdf_eval(S, its, wo, g.wi) * fabsf(g.dsh);
+      F3 Le = emitted(S, g.ls.emitter, g.ls.n, -g.wi);
+      float JL = S.shade[g.ls.tri].area / S.shade[g.ls.tri].area;
+      F3 val = valid * L.beta * f * JL * Gl * Le / g.ls.pdf;
+      float pb = bsdf_pdf(S, its, wo, g.wi) * Gl;
+      float mis = (g.ls.pdf * g.ls.pdf) / (g.ls.pdf * g.ls.pdf + pb * pb);
+      val = val * mis;
+      L.ret = zero;
+      L.ret = L.ret + val;
+      Rnd4 rb = rng4(P.seed, kRngInterior, L.path, 1u + 2u * L.b + 1u);
+      L.wib = bsdf_sample(S, its, wo, rb.a, rb.b, &L.pdfb);
+      L.stage = kStageBsdf;
+      L.o = its.p;
+      L.d = L.wib;
+      L.tmin = kShadowEpsilon;
+      L.tmax = 1e32f;
+      L.any = false;
+    } else if (L.stage == kStageBsdf) {
+      Its its = post_intersect(S, L.tri, L.u, L.v);
+      const F3 wo = -L.raydir;
+      bool ok = h.tri >= 0 && L.pdfb > 0.0f && bsdf_dir_valid(S, its, L.wib);
+      if (!ok) {
+        if (finite3(L.ret)) L.contrib = L.contrib + L.ret;
+        if (kCkpt) ck.meta[L.path] = meta;
+        finish(L.path, L.contrib);
+        L.stage = kStageIdle;
+        continue;
+      }
+      Its its2 = post_intersect(S, h.tri, h.u, h.v);
+      F3 dd = its2.p - its.p;
+      float dd2 = len2(dd);
+      F3 nd = dd / sqrtf(dd2);
+      float G2 = fabsf(dot(its2.ng, -nd)) / dd2;
+      F3 f2 = bsdf_eval(S, its, wo, nd) * fabsf(dot(its.ns, nd));
+      float ppdf = L.pdfb * G2;
+      float J2 = S.shade[its2.tri].area / S.shade[its2.tri].area;
+      F3 beta2 = L.beta * f2 * G2 * J2 / ppdf;
+      if (!(G2 > 0.0f)) beta2 = zero;
+      F3 ret = L.ret;
+      if (its2.emitter >= 0) {
+        F3 Le2 = emitted(S, its2.emitter, its2.ng, -nd);
+        F3 val2 = beta2 * Le2;
+        float pn = light_pdf_tri(S, its2.tri);
+        float mis2 = (ppdf * ppdf) / (ppdf * ppdf + pn * pn);
+        val2 = val2 * mis2;
+        ret = ret + val2;
+      }
+      if (finite3(ret)) L.contrib = L.contrib + ret;
+      L.tri = h.tri; L.u = h.u; L.v = h.v;
+      L.beta = beta2;
+      L.raydir = nd;
+      L.b += 1u;
+      if (kCkpt) {
+        ckpt_put_vertex(ck, L.path, (int)L.b, h.tri, h.u, h.v);
+        meta += 1u;
+      }
+      if ((int)L.b >= P.max_bounces) {
+        if (kCkpt) ck.meta[L.path] = meta;
+        finish(L.path, L.contrib);
+        L.stage = kStageIdle;
+        continue;
+      }
+      lane_setup_nee(S, P, its2, L);
+    }
+  }
 }
 
 }  // namespace trc

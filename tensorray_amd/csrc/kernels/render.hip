@@ -21,6 +21,7 @@ __device__ inline TraceCtx make_trace_ctx(uint32_t* lds, const LaunchCfg& L) {
   t.lds = lds + threadIdx.x;
   t.lds_stride = kBlock;
   t.lds_depth = kStackLds;
+  t.leafq = lds + kStackLds * kBlock + threadIdx.x;
   const int gtid = blockIdx.x * kBlock + threadIdx.x;
   t.spill = L.spill + gtid;
   t.spill_stride = L.grid * kBlock;
@@ -32,7 +33,7 @@ __device__ inline TraceCtx make_trace_ctx(uint32_t* lds, const LaunchCfg& L) {
 
 __global__ void __launch_bounds__(kBlock) k_intersect(SceneDev S, LaunchCfg L, const float* rays, int n,
                                                       int32_t* tri, float* uvt, int any_hit) {
-  __shared__ uint32_t lds_stack[kStackLds * kBlock];
+  __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx(lds_stack, L);
   const int stride = L.grid * kBlock;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
@@ -46,24 +47,29 @@ __global__ void __launch_bounds__(kBlock) k_intersect(SceneDev S, LaunchCfg L, c
   }
 }
 
+template <bool kCkpt>
 __global__ void __launch_bounds__(kBlock) k_render_paths(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
-                                                         uint64_t npaths, float* paths) {
-  __shared__ uint32_t lds_stack[kStackLds * kBlock];
+                                                         uint64_t npaths, float* paths, CkptDev ck) {
+  __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx(lds_stack, L);
-  const uint64_t stride = (uint64_t)L.grid * kBlock;
-  for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < npaths; p += stride) {
-    F3 c = path_forward<false>(S, C, P, p, tc, nullptr);
+  // one contiguous chunk of paths per wave
+  const uint64_t nwaves = (uint64_t)L.grid * (kBlock / 64);
+  const uint64_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const uint64_t chunk = (npaths + nwaves - 1) / nwaves;
+  const uint64_t begin = wave * chunk < npaths ? wave * chunk : npaths;
+  const uint64_t end = begin + chunk < npaths ? begin + chunk : npaths;
+  render_paths_regen<kCkpt>(S, C, P, tc, begin, end, ck, [&](uint64_t p, F3 c) {
     paths[3 * p] = c.x;
     paths[3 * p + 1] = c.y;
     paths[3 * p + 2] = c.z;
-  }
+  });
 }
 
 // Work statistics of the forward paths: counters[0..3] += (paths, queries,
 // node visits, triangle tests).  Used to compute algorithmic bytes.
 __global__ void __launch_bounds__(kBlock) k_render_stats(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
                                                          uint64_t npaths, unsigned long long* counters) {
-  __shared__ uint32_t lds_stack[kStackLds * kBlock];
+  __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx(lds_stack, L);
   const uint64_t stride = (uint64_t)L.grid * kBlock;
   uint64_t n = 0;
@@ -98,7 +104,7 @@ __global__ void k_film_reduce(const float* paths, int spp, int wh, float* film) 
 __global__ void __launch_bounds__(kBlock) k_render_adj(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
                                                        uint64_t npaths, const float* dLdI, float* dpos,
                                                        float* dmat, float* adj_vn, float* dtex) {
-  __shared__ uint32_t lds_stack[kStackLds * kBlock];
+  __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   __shared__ float mat_grad[kMaxMatGrad * kMatGradStride];
   for (int i = threadIdx.x; i < kMaxMatGrad * kMatGradStride; i += kBlock) mat_grad[i] = 0.0f;
   __syncthreads();
@@ -122,22 +128,201 @@ __global__ void __launch_bounds__(kBlock) k_render_adj(SceneDev S, CameraDev C, 
   }
 }
 
+// Adjoint from a path checkpoint: the path is re-evaluated from its recorded
+// hits (no traversal), then swept backwards as in k_render_adj.
+__global__ void __launch_bounds__(kBlock) k_render_adj_ckpt(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
+                                                            uint64_t npaths, CkptDev ck, const float* dLdI,
+                                                            float* dpos, float* dmat, float* adj_vn, float* dtex) {
+  __shared__ float mat_grad[kMaxMatGrad * kMatGradStride];
+  for (int i = threadIdx.x; i < kMaxMatGrad * kMatGradStride; i += kBlock) mat_grad[i] = 0.0f;
+  __syncthreads();
+  const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
+  const float inv_spp = 1.0f / (float)P.spp;
+  const uint64_t stride = (uint64_t)L.grid * kBlock;
+  for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < npaths; p += stride) {
+    const uint32_t pix = (uint32_t)(p % WH);
+    F3 g{dLdI[pix] * inv_spp, dLdI[WH + pix] * inv_spp, dLdI[2 * WH + pix] * inv_spp};
+    if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) continue;
+    CkptSrc src{&ck, p, ck.meta[p]};
+    if ((src.meta & 0xFFu) == 0u) continue;  // camera ray missed: no gradient
+    PathRec R;
+    path_forward<true>(S, C, P, p, src, &R);
+    path_backward(S, C, P, R, g, dpos, mat_grad, adj_vn, dtex);
+  }
+  __syncthreads();
+  if (dmat) {
+    const int n = (S.n_mats < kMaxMatGrad ? S.n_mats : kMaxMatGrad) * kMatGradStride;
+    for (int i = threadIdx.x; i < n; i += kBlock)
+      if (mat_grad[i] != 0.0f) atomicAdd(&dmat[i], mat_grad[i]);
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_vertex_normal_adj(SceneDev S, float* adj_vn, float* dpos) {
   const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
   if (v < (uint32_t)S.n_verts) vertex_normal_backward(S, v, adj_vn, dpos);
 }
 
+// ---------------------------------------------------------------------------
+// Batched launches: the grid walks the batch's tiles (contiguous path ranges
+// of one job); a wave takes tiles wave, wave + nwaves, ...  The job of a tile
+// is wave-uniform, so its scene / camera descriptors are read once per tile
+// into scalar registers.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ inline T wave_uniform_load(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "word-sized struct");
+  T out;
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(p);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  return out;
+}
+
+__device__ inline int job_of_tile(const JobDev* jobs, int njobs, uint64_t t) {
+  int lo = 0, hi = njobs - 1;  // last job with tile0 <= t
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].tile0 <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
+__global__ void __launch_bounds__(kBlock) k_render_batch_fwd(const JobDev* jobs, int njobs, uint64_t ntiles,
+                                                             uint32_t tile, LaunchCfg L) {
+  __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
+  TraceCtx tc = make_trace_ctx(lds_stack, L);
+  const uint64_t nwaves = (uint64_t)L.grid * (kBlock / 64);
+  const uint64_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+  for (uint64_t t = wave; t < ntiles; t += nwaves) {
+    const int j = job_of_tile(jobs, njobs, t);
+    const SceneDev S = wave_uniform_load(&jobs[j].S);
+    const CameraDev C = wave_uniform_load(&jobs[j].C);
+    const PathParams P{jobs[j].seed, jobs[j].max_bounces, jobs[j].spp, 0u};
+    const uint64_t first = jobs[j].first, count = jobs[j].count, t0 = jobs[j].tile0;
+    const uint64_t npaths = (uint64_t)P.spp * (uint64_t)C.width * (uint64_t)C.height;
+    const CkptDev ck = ckpt_view(jobs[j].ckpt, npaths, P.max_bounces);
+    float* paths = jobs[j].paths;
+    const uint64_t begin = first + (t - t0) * tile;
+    const uint64_t end = begin + tile < first + count ? begin + tile : first + count;
+    render_paths_regen<true>(S, C, P, tc, begin, end, ck, [&](uint64_t p, F3 c) {
+      paths[3 * p] = c.x;
+      paths[3 * p + 1] = c.y;
+      paths[3 * p + 2] = c.z;
+    });
+  }
+}
+
+// Material gradients are summed per wave in LDS and flushed at the end of
+// each tile (one job per tile).
+__global__ void __launch_bounds__(kBlock) k_render_batch_adj(const JobDev* jobs, int njobs, uint64_t ntiles,
+                                                             uint32_t tile, LaunchCfg L) {
+  __shared__ float mat_lds[(kBlock / 64) * kMaxMatGrad * kMatGradStride];
+  const int lane = (int)(threadIdx.x & 63u);
+  float* mat_grad = mat_lds + (threadIdx.x >> 6) * (kMaxMatGrad * kMatGradStride);
+  const uint64_t nwaves = (uint64_t)L.grid * (kBlock / 64);
+  const uint64_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+  for (uint64_t t = wave; t < ntiles; t += nwaves) {
+    const int j = job_of_tile(jobs, njobs, t);
+    const SceneDev S = wave_uniform_load(&jobs[j].S);
+    const CameraDev C = wave_uniform_load(&jobs[j].C);
+    const PathParams P{jobs[j].seed, jobs[j].max_bounces, jobs[j].spp, 0u};
+    const uint64_t first = jobs[j].first, count = jobs[j].count, t0 = jobs[j].tile0;
+    const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
+    const uint64_t npaths = (uint64_t)P.spp * (uint64_t)WH;
+    const CkptDev ck = ckpt_view(jobs[j].ckpt, npaths, P.max_bounces);
+    const float* dLdI = jobs[j].dLdI;
+    float* dpos = jobs[j].dpos;
+    float* dmat = jobs[j].dmat;
+    float* dtex = jobs[j].dtex;
+    float* vn = jobs[j].vn;
+    const float inv_spp = 1.0f / (float)P.spp;
+    for (int i = lane; i < kMaxMatGrad * kMatGradStride; i += 64) mat_grad[i] = 0.0f;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t begin = first + (t - t0) * tile;
+    const uint64_t end = begin + tile < first + count ? begin + tile : first + count;
+    for (uint64_t p = begin + (uint64_t)lane; p < end; p += 64) {
+      const uint32_t pix = (uint32_t)(p % WH);
+      F3 g{dLdI[pix] * inv_spp, dLdI[WH + pix] * inv_spp, dLdI[2 * WH + pix] * inv_spp};
+      if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) continue;
+      CkptSrc src{&ck, p, ck.meta[p]};
+      if ((src.meta & 0xFFu) == 0u) continue;
+      PathRec R;
+      path_forward<true>(S, C, P, p, src, &R);
+      path_backward(S, C, P, R, g, dpos, mat_grad, vn, dtex);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (dmat) {
+      const int n = (S.n_mats < kMaxMatGrad ? S.n_mats : kMaxMatGrad) * kMatGradStride;
+      for (int i = lane; i < n; i += 64)
+        if (mat_grad[i] != 0.0f) atomicAdd(&dmat[i], mat_grad[i]);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+hipError_t launch_render_batch(const JobDev* jobs_dev, const JobDev* jobs_host, int njobs, uint64_t ntiles,
+                               uint32_t tile, int adjoint, const LaunchCfg& L, hipStream_t st) {
+  if (!adjoint) {
+    const LaunchCfg Lk = fit_grid(k_render_batch_fwd, L);
+    hipLaunchKernelGGL(k_render_batch_fwd, dim3(Lk.grid), dim3(kBlock), 0, st, jobs_dev, njobs, ntiles, tile, Lk);
+    return hipGetLastError();
+  }
+  const LaunchCfg Lk = fit_grid(k_render_batch_adj, L);
+  hipLaunchKernelGGL(k_render_batch_adj, dim3(Lk.grid), dim3(kBlock), 0, st, jobs_dev, njobs, ntiles, tile, Lk);
+  hipError_t e = hipGetLastError();
+  for (int j = 0; j < njobs && e == hipSuccess; ++j) {
+    const JobDev& J = jobs_host[j];
+    if (!J.vn || !J.S.vf_off || J.S.n_verts == 0) continue;
+    const int grid = (J.S.n_verts + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_vertex_normal_adj, dim3(grid), dim3(kBlock), 0, st, J.S, J.vn, J.dpos);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
 hipError_t launch_intersect(const SceneDev& S, const LaunchCfg& L, const float* rays, int n, int32_t* tri,
                             float* uvt, int any_hit, hipStream_t st) {
-  hipLaunchKernelGGL(k_intersect, dim3(L.grid), dim3(kBlock), 0, st, S, L, rays, n, tri, uvt, any_hit);
+  const LaunchCfg Lk = fit_grid(k_intersect, L);
+  hipLaunchKernelGGL(k_intersect, dim3(Lk.grid), dim3(kBlock), 0, st, S, Lk, rays, n, tri, uvt, any_hit);
   return hipGetLastError();
 }
 
 hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
-                               const LaunchCfg& L, float* paths, hipStream_t st) {
+                               const LaunchCfg& L, float* paths, void* ckpt, hipStream_t st) {
   PathParams P{seed, max_bounces, spp, 0u};
   uint64_t np = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
-  hipLaunchKernelGGL(k_render_paths, dim3(L.grid), dim3(kBlock), 0, st, S, C, P, L, np, paths);
+  if (ckpt) {
+    const CkptDev ck = ckpt_view(ckpt, np, max_bounces);
+    const LaunchCfg Lk = fit_grid(k_render_paths<true>, L);
+    hipLaunchKernelGGL(k_render_paths<true>, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, np, paths, ck);
+  } else {
+    const CkptDev ck{};
+    const LaunchCfg Lk = fit_grid(k_render_paths<false>, L);
+    hipLaunchKernelGGL(k_render_paths<false>, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, np, paths, ck);
+  }
+  return hipGetLastError();
+}
+
+uint64_t checkpoint_bytes(uint64_t npaths, int max_bounces) { return ckpt_bytes(npaths, max_bounces); }
+
+hipError_t launch_render_adj_ckpt(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
+                                  const LaunchCfg& L, const void* ckpt, const float* dLdI, float* dpos, float* dmat,
+                                  float* dtex, float* vn_adj, hipStream_t st) {
+  PathParams P{seed, max_bounces, spp, 0u};
+  uint64_t np = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
+  const CkptDev ck = ckpt_view(const_cast<void*>(ckpt), np, max_bounces);
+  const LaunchCfg Lk = fit_grid(k_render_adj_ckpt, L);
+  hipLaunchKernelGGL(k_render_adj_ckpt, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, np, ck, dLdI, dpos, dmat,
+                     vn_adj, dtex);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !vn_adj || S.n_verts == 0) return e;
+  const int grid = (S.n_verts + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_vertex_normal_adj, dim3(grid), dim3(kBlock), 0, st, S, vn_adj, dpos);
   return hipGetLastError();
 }
 
@@ -145,7 +330,8 @@ hipError_t launch_render_stats(const SceneDev& S, const CameraDev& C, uint32_t s
                                const LaunchCfg& L, unsigned long long* counters, hipStream_t st) {
   PathParams P{seed, max_bounces, spp, 0u};
   uint64_t np = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
-  hipLaunchKernelGGL(k_render_stats, dim3(L.grid), dim3(kBlock), 0, st, S, C, P, L, np, counters);
+  const LaunchCfg Lk = fit_grid(k_render_stats, L);
+  hipLaunchKernelGGL(k_render_stats, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, np, counters);
   return hipGetLastError();
 }
 
@@ -159,7 +345,8 @@ hipError_t launch_render_adj(const SceneDev& S, const CameraDev& C, uint32_t see
                              float* vn_adj, hipStream_t st) {
   PathParams P{seed, max_bounces, spp, 0u};
   uint64_t np = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
-  hipLaunchKernelGGL(k_render_adj, dim3(L.grid), dim3(kBlock), 0, st, S, C, P, L, np, dLdI, dpos, dmat, vn_adj, dtex);
+  const LaunchCfg Lk = fit_grid(k_render_adj, L);
+  hipLaunchKernelGGL(k_render_adj, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, np, dLdI, dpos, dmat, vn_adj, dtex);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !vn_adj || S.n_verts == 0) return e;
   const int grid = (S.n_verts + kBlock - 1) / kBlock;

@@ -92,6 +92,7 @@ TR_HD float safe_inv(float d) {
 // spill slice for deeper entries.
 struct TraceCtx {
   uint32_t* lds;        // &lds_stack[threadIdx.x]
+  uint32_t* leafq;      // postponed-leaf queue of this lane (kLeafQueue entries, stride lds_stride)
   int lds_stride;       // blockDim.x
   int lds_depth;        // entries held in LDS
   uint32_t* spill;      // &spill[gtid]
@@ -115,11 +116,22 @@ __device__ inline uint32_t stack_pop(TraceCtx& c, int& sp) {
 
 __device__ inline F3 tri_vertex(const float* v) { return F3{v[0], v[1], v[2]}; }
 
-// Closest hit (any_hit = false) or first found hit (any_hit = true) in
-// (tmin, tmax).  Ties in t resolve to the smallest triangle id, so the result
-// does not depend on traversal order.
-template <bool kAnyHit>
-__device__ Hit trace_ray(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, float tmin, float tmax) {
+// Closest hit (any = false) or first found hit (any = true) in (tmin, tmax).
+// Ties in t resolve to the smallest triangle id, so the result does not
+// depend on traversal order.  A ray with tmax < tmin visits the root only
+// and misses (used for lanes without a query).
+//
+// Per node visit: slab tests of the four children; hit inner children are
+// sorted near to far and pushed, hit leaves are appended to a per-lane queue
+// in LDS.  Leaves are tested later, in one flat loop over (leaf, triangle)
+// pairs, when the wave decides to: at least three quarters of its live lanes
+// have queued leaves, or some lane has no inner node left or a nearly full
+// queue (the threshold was picked by timing 1/4, 1/2 and 3/4).
+// Testing a leaf late only loosens culling (best_t shrinks later); it never
+// changes the result, since the closest hit does not depend on test order.
+// Lanes run this loop in step with ballots, so the triangle test executes
+// with most lanes active instead of once per child slot that any lane hit.
+__device__ inline Hit trace_ray_rt(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, float tmin, float tmax, bool any) {
   ctx.n_queries++;
   RayPre rp = ray_pre(o, d);
   const float ix = safe_inv(d.x), iy = safe_inv(d.y), iz = safe_inv(d.z);
@@ -129,74 +141,110 @@ __device__ Hit trace_ray(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, float tmi
   uint32_t best_id = 0xFFFFFFFFu;
   int sp = 0;
   uint32_t node = 0;
+  bool inner_done = false;  // no inner node left to visit
+  int nq = 0;               // queued leaves
+  bool found = false;       // any-hit: a hit was found
+  const int qs = ctx.lds_stride;
   for (;;) {
-    ctx.n_nodes++;
-    const float4* np = reinterpret_cast<const float4*>(S.nodes + node);
-    float4 lox = np[0], loy = np[1], loz = np[2], hix = np[3], hiy = np[4], hiz = np[5];
-    uint4 ch = reinterpret_cast<const uint4*>(np)[6];
-    float lo_x[4] = {lox.x, lox.y, lox.z, lox.w}, lo_y[4] = {loy.x, loy.y, loy.z, loy.w};
-    float lo_z[4] = {loz.x, loz.y, loz.z, loz.w}, hi_x[4] = {hix.x, hix.y, hix.z, hix.w};
-    float hi_y[4] = {hiy.x, hiy.y, hiy.z, hiy.w}, hi_z[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
-    uint32_t cid[4] = {ch.x, ch.y, ch.z, ch.w};
-    // inner children hit: (distance, node); others keep (inf, kEmpty)
-    float kt[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
-    uint32_t kc[4] = {kEmpty, kEmpty, kEmpty, kEmpty};
+    // ---- visit one inner node (lanes with room in their leaf queue) ----
+    if (!inner_done && nq <= kLeafQueue - 4) {
+      ctx.n_nodes++;
+      const float4* np = reinterpret_cast<const float4*>(S.nodes + node);
+      float4 lox = np[0], loy = np[1], loz = np[2], hix = np[3], hiy = np[4], hiz = np[5];
+      uint4 ch = reinterpret_cast<const uint4*>(np)[6];
+      float lo_x[4] = {lox.x, lox.y, lox.z, lox.w}, lo_y[4] = {loy.x, loy.y, loy.z, loy.w};
+      float lo_z[4] = {loz.x, loz.y, loz.z, loz.w}, hi_x[4] = {hix.x, hix.y, hix.z, hix.w};
+      float hi_y[4] = {hiy.x, hiy.y, hiy.z, hiy.w}, hi_z[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+      uint32_t cid[4] = {ch.x, ch.y, ch.z, ch.w};
+      float kt[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+      uint32_t kc[4] = {kEmpty, kEmpty, kEmpty, kEmpty};
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      float t0x = (lo_x[s] - o.x) * ix, t1x = (hi_x[s] - o.x) * ix;
-      float t0y = (lo_y[s] - o.y) * iy, t1y = (hi_y[s] - o.y) * iy;
-      float t0z = (lo_z[s] - o.z) * iz, t1z = (hi_z[s] - o.z) * iz;
-      float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-      float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best_t));
-      uint32_t c = cid[s];
-      bool hit = c != kEmpty && tn <= tf * kSlabScale;
-      if (!hit) continue;
-      if (c & kLeafBit) {
-        uint32_t first = leaf_first(c), cnt = leaf_count(c);
-        for (uint32_t i = 0; i < cnt; ++i) {
-          ctx.n_tris++;
-          const TriRecord& tr = S.tris[first + i];
-          float4 a = *reinterpret_cast<const float4*>(tr.v0);
-          float4 b = *reinterpret_cast<const float4*>(tr.v1);
-          float4 e = *reinterpret_cast<const float4*>(tr.v2);
-          float tt, uu, vv;
-          if (!tri_intersect(rp, F3{a.x, a.y, a.z}, F3{b.x, b.y, b.z}, F3{e.x, e.y, e.z}, &tt, &uu, &vv))
-            continue;
-          uint32_t id = __float_as_uint(a.w);
-          if (!(tt > tmin && tt < tmax)) continue;
-          if (kAnyHit) {
-            h.tri = (int32_t)id; h.u = uu; h.v = vv; h.t = tt;
-            return h;
-          }
-          if (tt < best_t || (tt == best_t && id < best_id)) {
-            best_t = tt; best_id = id; h.u = uu; h.v = vv;
-          }
+      for (int s = 0; s < 4; ++s) {
+        float t0x = (lo_x[s] - o.x) * ix, t1x = (hi_x[s] - o.x) * ix;
+        float t0y = (lo_y[s] - o.y) * iy, t1y = (hi_y[s] - o.y) * iy;
+        float t0z = (lo_z[s] - o.z) * iz, t1z = (hi_z[s] - o.z) * iz;
+        float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+        float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best_t));
+        uint32_t c = cid[s];
+        bool hit = c != kEmpty && tn <= tf * kSlabScale;
+        if (hit && (c & kLeafBit)) {
+          ctx.leafq[nq * qs] = c;
+          ++nq;
         }
+        if (hit && !(c & kLeafBit)) {
+          kt[s] = tn;
+          kc[s] = c;
+        }
+      }
+      // 4-element sorting network, near to far
+#define TR_CSWAP(i, j)                                                   \
+      if (kt[j] < kt[i]) {                                               \
+        float _t = kt[i]; kt[i] = kt[j]; kt[j] = _t;                     \
+        uint32_t _c = kc[i]; kc[i] = kc[j]; kc[j] = _c;                  \
+      }
+      TR_CSWAP(0, 1) TR_CSWAP(2, 3) TR_CSWAP(0, 2) TR_CSWAP(1, 3) TR_CSWAP(1, 2)
+#undef TR_CSWAP
+      if (kc[0] != kEmpty) {
+        int npush = (kc[1] != kEmpty) + (kc[2] != kEmpty) + (kc[3] != kEmpty);
+        if (sp + npush > ctx.cap) {
+          *ctx.err = 1u;
+          inner_done = true;
+        } else {
+          if (kc[3] != kEmpty) stack_push(ctx, sp, kc[3]);
+          if (kc[2] != kEmpty) stack_push(ctx, sp, kc[2]);
+          if (kc[1] != kEmpty) stack_push(ctx, sp, kc[1]);
+          node = kc[0];
+        }
+      } else if (sp > 0) {
+        node = stack_pop(ctx, sp);
       } else {
-        kt[s] = tn;
-        kc[s] = c;
+        inner_done = true;
       }
     }
-    // 4-element sorting network, near to far
-#define TR_CSWAP(i, j)                                                   \
-    if (kt[j] < kt[i]) {                                                 \
-      float _t = kt[i]; kt[i] = kt[j]; kt[j] = _t;                       \
-      uint32_t _c = kc[i]; kc[i] = kc[j]; kc[j] = _c;                    \
+    // ---- decide, wave-wide, whether queued leaves are tested now ----
+    const bool pend = nq > 0;
+    const bool live = !inner_done || pend;
+    const uint64_t m_live = __ballot(live);
+    if (m_live == 0ull) break;
+    const uint64_t m_pend = __ballot(pend);
+    const uint64_t m_urgent = __ballot(pend && (inner_done || nq > kLeafQueue - 4));
+    const bool flush = m_urgent != 0ull || 4 * __popcll(m_pend) >= 3 * __popcll(m_live);
+    if (flush && pend) {
+      int qi = 0;
+      uint32_t cur = 0u, left = 0u;
+      for (;;) {
+        if (left == 0u) {
+          if (qi == nq) break;
+          const uint32_t c = ctx.leafq[qi * qs];
+          ++qi;
+          cur = leaf_first(c);
+          left = leaf_count(c);
+        }
+        ctx.n_tris++;
+        const TriRecord& tr = S.tris[cur];
+        ++cur;
+        --left;
+        float4 a = *reinterpret_cast<const float4*>(tr.v0);
+        float4 b = *reinterpret_cast<const float4*>(tr.v1);
+        float4 e = *reinterpret_cast<const float4*>(tr.v2);
+        float tt, uu, vv;
+        if (!tri_intersect(rp, F3{a.x, a.y, a.z}, F3{b.x, b.y, b.z}, F3{e.x, e.y, e.z}, &tt, &uu, &vv)) continue;
+        uint32_t id = __float_as_uint(a.w);
+        if (!(tt > tmin && tt < tmax)) continue;
+        if (any) {
+          h.tri = (int32_t)id; h.u = uu; h.v = vv; h.t = tt;
+          found = true;
+          break;
+        }
+        if (tt < best_t || (tt == best_t && id < best_id)) {
+          best_t = tt; best_id = id; h.u = uu; h.v = vv;
+        }
+      }
+      nq = 0;
+      if (found) inner_done = true;
     }
-    TR_CSWAP(0, 1) TR_CSWAP(2, 3) TR_CSWAP(0, 2) TR_CSWAP(1, 3) TR_CSWAP(1, 2)
-#undef TR_CSWAP
-    if (kc[0] != kEmpty) {
-      int npush = (kc[1] != kEmpty) + (kc[2] != kEmpty) + (kc[3] != kEmpty);
-      if (sp + npush > ctx.cap) { *ctx.err = 1u; break; }
-      if (kc[3] != kEmpty) stack_push(ctx, sp, kc[3]);
-      if (kc[2] != kEmpty) stack_push(ctx, sp, kc[2]);
-      if (kc[1] != kEmpty) stack_push(ctx, sp, kc[1]);
-      node = kc[0];
-      continue;
-    }
-    if (sp == 0) break;
-    node = stack_pop(ctx, sp);
   }
+  if (found) return h;
   if (best_id != 0xFFFFFFFFu) {
     h.tri = (int32_t)best_id;
     h.t = best_t;
@@ -204,6 +252,11 @@ __device__ Hit trace_ray(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, float tmi
     h.u = 0.0f; h.v = 0.0f;
   }
   return h;
+}
+
+template <bool kAnyHit>
+__device__ inline Hit trace_ray(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, float tmin, float tmax) {
+  return trace_ray_rt(S, ctx, o, d, tmin, tmax, kAnyHit);
 }
 
 }  // namespace trc

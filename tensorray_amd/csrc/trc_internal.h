@@ -19,6 +19,18 @@ struct trc_ctx {
   uint32_t* spill = nullptr;
   size_t spill_words = 0;
   std::mutex mu;
+  // job tables of batched launches: a ring of pinned staging + device
+  // buffers, each reused once the batch that last used it has finished
+  struct BatchSlot {
+    void* host = nullptr;
+    void* dev = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+  };
+  static constexpr int kBatchSlots = 4;
+  BatchSlot batch[kBatchSlots];
+  unsigned batch_next = 0;
+  std::mutex batch_mu;
 };
 
 struct DevBuf {
@@ -52,18 +64,62 @@ namespace trc {
 // launch configuration shared by the megakernels
 constexpr int kBlock = 128;
 constexpr int kStackLds = 16;
+constexpr int kLeafQueue = 8;                          // postponed leaves per lane (LDS)
+constexpr int kTraceLdsWords = kStackLds + kLeafQueue;  // LDS words per lane for traversal
 
 struct LaunchCfg {
   int grid;             // persistent grid size (blocks)
   uint32_t* spill;      // spill buffer [depth][grid*kBlock]
   int cap;              // stack capacity
   uint32_t* err;
+  int cus;              // compute units of the device
 };
+
+// A persistent grid-stride kernel should not launch more blocks than can be
+// resident at once: blocks beyond that only start when resident ones have
+// finished all their iterations, and then run at low occupancy.  Clamp the
+// grid to (resident blocks per CU for this kernel) x CUs.
+int resident_blocks_per_cu(const void* kernel, int block);  // cached occupancy query, 0 if unknown
+
+template <class Kernel>
+inline LaunchCfg fit_grid(Kernel kernel, LaunchCfg L, int block = kBlock) {
+  const int per_cu = resident_blocks_per_cu((const void*)kernel, block);
+  if (per_cu > 0 && L.cus > 0) {
+    const int cap = per_cu * L.cus;
+    if (L.grid > cap) L.grid = cap;
+  }
+  return L;
+}
 
 hipError_t launch_intersect(const SceneDev& S, const LaunchCfg& L, const float* rays, int n, int32_t* tri,
                             float* uvt, int any_hit, hipStream_t st);
+// One job of a batched launch (device copy of trc_job).
+struct JobDev {
+  SceneDev S;
+  CameraDev C;
+  uint32_t seed;
+  int32_t max_bounces, spp, pad;
+  uint64_t first, count;   // path range of the job
+  uint64_t tile0;          // first tile of the job in the batch
+  float* paths;
+  void* ckpt;
+  const float* dLdI;
+  float* dpos;
+  float* dmat;
+  float* dtex;
+  float* vn;
+};
+// tiles of `tile` paths; jobs sorted by tile0
+hipError_t launch_render_batch(const JobDev* jobs_dev, const JobDev* jobs_host, int njobs, uint64_t ntiles,
+                               uint32_t tile, int adjoint, const LaunchCfg& L, hipStream_t st);
+
+// ckpt: null or a path checkpoint of checkpoint_bytes(spp*W*H, max_bounces) bytes
 hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
-                               const LaunchCfg& L, float* paths, hipStream_t st);
+                               const LaunchCfg& L, float* paths, void* ckpt, hipStream_t st);
+uint64_t checkpoint_bytes(uint64_t npaths, int max_bounces);
+hipError_t launch_render_adj_ckpt(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
+                                  const LaunchCfg& L, const void* ckpt, const float* dLdI, float* dpos, float* dmat,
+                                  float* dtex, float* vn_adj, hipStream_t st);
 hipError_t launch_render_stats(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                                const LaunchCfg& L, unsigned long long* counters, hipStream_t st);
 hipError_t launch_film_reduce(const float* paths, int spp, int wh, float* film, hipStream_t st);

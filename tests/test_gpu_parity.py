@@ -276,3 +276,47 @@ def test_guided_direct_matches_oracle(gpu_ctx, oracle_mod):
     with pytest.raises(_capi.TRCError, match="g_eps"):
         ds.render_boundary(cam, bad, "direct", torch.from_numpy(dl).cuda().data_ptr(), dpos.data_ptr())
     ds.close()
+
+
+@pytest.mark.parametrize("case", ["config1", "metric_small", "textured", "smooth", "conductor"])
+def test_checkpointed_adjoint(gpu_ctx, oracle_mod, case):
+    """trc_render_fwd_ckpt writes the same film as trc_render_fwd; the
+    adjoint from the checkpoint (no traversal) matches the oracle's AD like
+    the replay adjoint does."""
+    torch = _torch()
+    make, W, H, spp, B = CASES[case]
+    spec = make()
+    ds = _capi.DeviceScene(gpu_ctx, spec)
+    info = ds.info()
+    cam_d = scenes.CAMERA_DEFAULT
+    cam = _capi.make_camera(W, H, **cam_d)
+    opt = _capi.make_options(seed=99, max_bounces=B, spp=spp)
+    nb = ds.checkpoint_bytes(cam, opt)
+    assert nb == spp * W * H * (4 + 12 * (B + 1))
+    ck = torch.empty(nb, dtype=torch.uint8, device="cuda")
+    f1 = torch.empty(3, W * H, device="cuda")
+    f2 = torch.empty(3, W * H, device="cuda")
+    ds.render_fwd(cam, opt, f1.data_ptr())
+    ds.render_fwd_ckpt(cam, opt, f2.data_ptr(), ck.data_ptr(), nb)
+    torch.cuda.synchronize()
+    assert torch.equal(f1, f2)
+    dl = np.random.default_rng(0).uniform(-1, 1, (3, W * H)).astype(np.float32)
+    dpos = torch.zeros(info.num_vertices, 3, device="cuda")
+    dmat = torch.zeros(info.num_materials, 8, device="cuda")
+    dtex = torch.zeros(max(1, info.texture_floats), device="cuda")
+    ds.render_adj_ckpt(cam, opt, ck.data_ptr(), nb, torch.from_numpy(dl).cuda().data_ptr(), dpos.data_ptr(),
+                       dmat.data_ptr(), dtex.data_ptr() if info.texture_floats else None)
+    torch.cuda.synchronize()
+    S = oracle_mod.Scene(spec)
+    rp, rm, rt = S.render_adj(cam_d, W, H, 99, B, spp, dl, with_tex=True)
+    mask = spec.diff_vertex_mask()
+    ok, e = _grad_close(dpos.cpu().numpy().astype(np.float64)[mask], rp[mask])
+    assert ok, e
+    ok, e = _grad_close(dmat.cpu().numpy().astype(np.float64)[:, :7], rm[:, :7])
+    assert ok, e
+    if info.texture_floats:
+        ok, e = _grad_close(dtex.cpu().numpy().astype(np.float64), rt)
+        assert ok, e
+    with pytest.raises(_capi.TRCError, match="too small"):
+        ds.render_adj_ckpt(cam, opt, ck.data_ptr(), nb - 1, torch.from_numpy(dl).cuda().data_ptr(), dpos.data_ptr())
+    ds.close()
