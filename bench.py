@@ -192,7 +192,8 @@ def main():
         kname, kms = ("k_render_adj_ckpt", adj_avg) if ck else ("k_render_adj", adj_avg)
         kbytes = (bb["adj_ckpt"] if ck else bb["adj"]) * P
     else:
-        kname, kms = "k_render_paths", fwd_avg
+        # the forward kernel with (true) or without (false) checkpoint writes
+        kname, kms = ("k_render_paths<true>" if ck else "k_render_paths<false>"), fwd_avg
         kbytes = (bb["fwd_ckpt"] if ck else bb["fwd"]) * P
     achieved = kbytes / (kms * 1e-3)
     traffic = None
