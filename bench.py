@@ -45,6 +45,10 @@ def parse():
     p.add_argument("--cpu-runs", type=int, default=5, help="timed CPU baseline runs (median reported)")
     p.add_argument("--adjoint", choices=["checkpoint", "replay"], default="checkpoint",
                    help="adjoint from the forward's path checkpoint (no re-tracing) or by replaying the paths")
+    p.add_argument("--serve-requests", type=int, default=48,
+                   help="requests of the served-vs-plain comparison (0 skips it)")
+    p.add_argument("--serve-large-scenes", action="store_true",
+                   help="serve over 1k / 20k / 200k / 1M-triangle scenes (slower set-up) instead of 1k / 20k")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="after the timed region, copy a MIB-sized device buffer (reference kernel of known "
                         "traffic for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -102,6 +106,61 @@ def cpu_baseline(spec, cam, W, H, B, spp, cpu_spp, runs):
     return dict(value=n / tot / 1e6, unit="Mpaths/s", cores=threads, kind="port",
                 sample=f"{cpu_spp} spp x {W}x{H} = {n} paths of the same scene, depth {B + 1}, forward + adjoint "
                        f"(replay); median of {runs} runs after a warm-up: fwd {tf:.2f} s, adj {ta:.2f} s")
+
+
+def served_comparison(ctx, a):
+    """Served (continuous batching) vs plain (one request per launch
+    sequence) on the same request list: 512x512, depth 4, spp uniform in
+    1..16, over resident scenes.  Saturation throughput (all requests queued
+    at t = 0) for both, then p50 / p99 latency under Poisson arrivals at 0.75x
+    the plain path's measured request rate."""
+    import torch
+    from tensorray_amd import scenes, serving
+    specs = {"s1k": scenes.sphere_scene(subdiv=3, amplitude=0.05, noise_seed=2),
+             "s20k": scenes.metric_scene(amplitude=0.05, noise_seed=1)}
+    if a.serve_large_scenes:
+        specs["s200k"] = scenes.sphere_scene(subdiv=7, amplitude=0.05, noise_seed=3)
+        specs["s1m"] = scenes.sphere_scene(subdiv=8, amplitude=0.05, noise_seed=4)
+    srv = serving.Server(ctx, budget_bytes=64 << 30, batch_paths=8 << 20, chunk_paths=4 << 20)
+    for k, v in specs.items():
+        srv.add_scene(k, v)
+    names = list(specs)
+
+    def reqs(rate, seed):
+        return serving.make_requests(names, a.serve_requests, rate, seed=seed, res=(512, 512), spp=(1, 16),
+                                     depth=(4, 4))
+
+    # warm-up of both paths
+    serving.run_plain(ctx, srv.scenes, reqs(0, 7)[:4], realtime=False)
+    srv.run(reqs(0, 7)[:4], realtime=False)
+    out = {}
+    for mode in ("plain", "served"):
+        rs = reqs(0, 2024)
+        t0 = time.perf_counter()
+        if mode == "plain":
+            serving.run_plain(ctx, srv.scenes, rs, realtime=False)
+        else:
+            srv.run(rs, realtime=False)
+        wall = time.perf_counter() - t0
+        paths = sum(r.paths for r in rs)
+        out[mode] = dict(mpaths_per_s=paths / wall / 1e6, wall_s=wall, requests=len(rs), paths=paths)
+    rate = 0.75 * a.serve_requests / out["plain"]["wall_s"]
+    for mode in ("plain", "served"):
+        rs = reqs(rate, 2025)
+        if mode == "plain":
+            serving.run_plain(ctx, srv.scenes, rs, realtime=True)
+        else:
+            srv.run(rs, realtime=True)
+        lat = serving.percentiles([r.latency * 1e3 for r in rs])
+        out[mode].update(latency_ms_at_075_load=lat)
+    out["poisson_rate_per_s"] = rate
+    out["served_over_plain"] = out["served"]["mpaths_per_s"] / out["plain"]["mpaths_per_s"]
+    out["workload"] = (f"{a.serve_requests} requests, 512x512, depth 4, spp U(1..16) (numpy PCG64 seed 2024), "
+                       f"scenes {', '.join(f'{k}={v.num_triangles()} tris' for k, v in specs.items())}; "
+                       f"HBM budget 64 GiB, batches of <= 8M paths")
+    for ds in srv.scenes.values():
+        ds.close()
+    return out
 
 
 def main():
@@ -214,6 +273,7 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    served = served_comparison(ctx, a) if a.serve_requests > 0 else None
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(spec, cam_d, W, H, B, spp, a.cpu_spp, a.cpu_runs)
@@ -236,6 +296,7 @@ def main():
                    "paths_per_step": P},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "served": served,
     }
     print(json.dumps(out))
     scene.close()

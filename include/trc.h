@@ -206,6 +206,27 @@ typedef struct trc_job {
  * appear in different batches but not twice in one. */
 trc_status trc_render_batch(trc_ctx* ctx, const trc_job* jobs, int32_t n, int32_t adjoint, trc_stream stream);
 
+/* Prepared batch (multi-view rendering): trc_plan_create validates the jobs,
+ * uploads the job table once and allocates what the launch needs
+ * (privatized gradient copies), so that trc_plan_launch only launches
+ * kernels -- no allocation, no host synchronization -- and may be captured
+ * into a hipGraph and replayed.  The jobs' buffers must stay valid for the
+ * plan's lifetime.  No counterpart in the reference, which renders views one
+ * at a time (renderer_multi_view.py:55-78). */
+typedef struct trc_plan trc_plan;
+trc_status trc_plan_create(trc_ctx* ctx, const trc_job* jobs, int32_t n, int32_t adjoint, trc_plan** out);
+/* Paths per tile of the persistent launch (multiple of 64; 0 = built-in heuristic). */
+trc_status trc_plan_set_tile(trc_plan* plan, uint32_t tile_paths);
+uint32_t trc_plan_tile(const trc_plan* plan);
+trc_status trc_plan_launch(trc_plan* plan, trc_stream stream);
+/* Autotuner: launches the plan `reps` times per candidate tile size, keeps
+ * the fastest (median of HIP-event times) and returns it in *best.  The
+ * launches write the outputs; an adjoint plan accumulates into its gradient
+ * buffers, which the caller clears afterwards. */
+trc_status trc_plan_tune(trc_plan* plan, const uint32_t* candidates, int32_t n, int32_t reps, uint32_t* best,
+                         trc_stream stream);
+trc_status trc_plan_destroy(trc_plan* plan);
+
 /* Adjoint: accumulates dL/dV += J^T dLdI into dpos ([V][3] device), the
  * material gradients into dmat ([M][8] device, may be NULL) and the texel
  * gradients of albedo / kd textures into dtex (may be NULL; all textures'

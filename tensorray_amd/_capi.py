@@ -91,6 +91,13 @@ SIGNATURES = [
                                              ctypes.c_int64, vp]),
     ("trc_render_adj_ckpt", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options), vp,
                                              ctypes.c_int64, vp, vp, vp, vp, vp]),
+    ("trc_plan_create", ctypes.c_int32, [vp, ctypes.POINTER(trc_job), ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.POINTER(vp)]),
+    ("trc_plan_set_tile", ctypes.c_int32, [vp, ctypes.c_uint32]),
+    ("trc_plan_tile", ctypes.c_uint32, [vp]),
+    ("trc_plan_launch", ctypes.c_int32, [vp, vp]),
+    ("trc_plan_tune", ctypes.c_int32, [vp, c_u32_p, ctypes.c_int32, ctypes.c_int32, c_u32_p, vp]),
+    ("trc_plan_destroy", ctypes.c_int32, [vp]),
 ]
 
 _lib = None
@@ -155,10 +162,62 @@ class Context:
         self.check(self.lib.trc_render_batch(self.h, arr, len(jobs), 1 if adjoint else 0, _stream_ptr(stream)),
                    "trc_render_batch")
 
+    def plan(self, jobs, adjoint):
+        """A prepared batch (trc_plan_create) over `jobs`."""
+        return Plan(self, jobs, adjoint)
+
     def close(self):
         if self.h is not None and self.h.value:
             self.lib.trc_context_destroy(self.h)
             self.h = None
+
+
+class Plan:
+    """A prepared batch: the job table is uploaded once; launch() only
+    launches kernels (capturable into a graph).  The jobs' buffers must
+    outlive the plan."""
+
+    def __init__(self, ctx, jobs, adjoint):
+        self.ctx = ctx
+        self.adjoint = bool(adjoint)
+        self.jobs = list(jobs)
+        arr = (trc_job * len(self.jobs))(*self.jobs)
+        h = vp()
+        ctx.check(ctx.lib.trc_plan_create(ctx.h, arr, len(self.jobs), 1 if adjoint else 0, ctypes.byref(h)),
+                  "trc_plan_create")
+        self.h = h
+
+    @property
+    def tile(self):
+        return int(self.ctx.lib.trc_plan_tile(self.h))
+
+    @tile.setter
+    def tile(self, paths):
+        self.ctx.check(self.ctx.lib.trc_plan_set_tile(self.h, int(paths)), "trc_plan_set_tile")
+
+    def launch(self, stream=None):
+        self.ctx.check(self.ctx.lib.trc_plan_launch(self.h, _stream_ptr(stream)), "trc_plan_launch")
+
+    def tune(self, candidates, reps=3, stream=None):
+        """Times each candidate tile (median of `reps` launches after one
+        warm-up), keeps the fastest and returns it."""
+        import numpy as np
+        c = np.ascontiguousarray(candidates, dtype=np.uint32)
+        best = ctypes.c_uint32(0)
+        self.ctx.check(self.ctx.lib.trc_plan_tune(self.h, _up(c), len(c), int(reps), ctypes.byref(best),
+                                                  _stream_ptr(stream)), "trc_plan_tune")
+        return int(best.value)
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            self.ctx.lib.trc_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def make_camera(width, height, pos=(0.0, 1.0, 3.0), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=45.0):

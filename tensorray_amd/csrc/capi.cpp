@@ -1,8 +1,10 @@
 // C ABI implementation (include/trc.h): context, resident scenes, launches.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 
@@ -65,7 +67,23 @@ int resident_blocks_per_cu(const void* kernel, int block) {
 
 namespace {
 
-LaunchCfg make_cfg(trc_ctx* ctx, const trc_scene* s, hipStream_t st, long long work_items) {
+// Stream-ordered zeroed gradient copies for a scene of n_verts vertices
+// (none when the mesh is large enough that atomics rarely collide).
+GradCopies make_grad_copies(int n_verts, hipStream_t st) {
+  GradCopies g;
+  const uint64_t n3 = 3ull * (uint64_t)n_verts;
+  g.copies = grad_copies_for(n3);
+  g.n3 = (uint32_t)n3;
+  if (g.copies <= 1) return g;
+  const uint64_t n = n3 * g.copies;
+  hip_check(hipMallocAsync((void**)&g.base, n * sizeof(float), st), "hipMallocAsync(gradient copies)");
+  hip_check(launch_fill_zero(g.base, n, st), "k_fill_zero");
+  return g;
+}
+
+// Launch shape for `work_items` items over scene `s`, with the spill buffer of
+// stream `st` (none when with_spill is false; the caller supplies one).
+LaunchCfg make_cfg(trc_ctx* ctx, const trc_scene* s, hipStream_t st, long long work_items, bool with_spill = true) {
   LaunchCfg L;
   int max_grid = ctx->num_cus * kBlocksPerCU;
   long long need = (work_items + kBlock - 1) / kBlock;
@@ -76,6 +94,8 @@ LaunchCfg make_cfg(trc_ctx* ctx, const trc_scene* s, hipStream_t st, long long w
   L.cap = kStackLds + kSpillDepth;
   L.err = ctx->err_flag;
   L.cus = ctx->num_cus;
+  L.spill = nullptr;
+  if (!with_spill) return L;
   std::lock_guard<std::mutex> lk(ctx->mu);
   auto key = std::make_pair(ctx, st);
   auto it = spill_map().find(key);
@@ -548,59 +568,106 @@ trc_status trc_render_fwd_ckpt(trc_scene* s, const trc_camera* cam, const trc_op
   });
 }
 
+// ---- batched launches ----
+// Validates the jobs and fills their device records; counts the floats of
+// the privatized gradient copies the adjoint jobs need (bases unset).
+static std::vector<JobDev> prepare_batch(trc_ctx* ctx, const trc_job* jobs, int32_t n, int32_t adjoint,
+                                         const trc_scene** deepest, uint64_t* total, uint64_t* grad_floats) {
+  if (ctx->device < 0) throw Fail(TRC_ERR_STATE, "host-only context cannot render");
+  if (n < 0 || (n > 0 && !jobs)) throw Fail(TRC_ERR_INVALID, "bad job list");
+  std::vector<JobDev> jd((size_t)n);
+  *deepest = nullptr;
+  *total = 0;
+  *grad_floats = 0;
+  for (int i = 0; i < n; ++i) {
+    const trc_job& J = jobs[i];
+    const std::string tag = "job " + std::to_string(i) + ": ";
+    if (!J.scene || J.scene->ctx != ctx) throw Fail(TRC_ERR_INVALID, tag + "bad scene");
+    check_render(J.scene, &J.opt);
+    CameraDev C = to_cam(&J.cam);
+    const uint64_t np = (uint64_t)J.opt.spp * (uint64_t)C.width * (uint64_t)C.height;
+    if (J.num_paths == 0 || J.first_path + J.num_paths > np) throw Fail(TRC_ERR_INVALID, tag + "path range outside spp*W*H");
+    if (!J.ckpt) throw Fail(TRC_ERR_INVALID, tag + "checkpoint is NULL");
+    if (!adjoint && !J.paths) throw Fail(TRC_ERR_INVALID, tag + "paths is NULL");
+    if (adjoint && (!J.dLdI || !J.dpos)) throw Fail(TRC_ERR_INVALID, tag + "dLdI / dpos is NULL");
+    if (adjoint && J.scene->dev.vf_off && !J.vn_scratch)
+      throw Fail(TRC_ERR_INVALID, tag + "smooth differentiable mesh needs vn_scratch");
+    JobDev& d = jd[(size_t)i];
+    std::memset(&d, 0, sizeof(d));
+    d.S = J.scene->dev;
+    d.C = C;
+    d.seed = J.opt.seed;
+    d.max_bounces = J.opt.max_bounces;
+    d.spp = J.opt.spp;
+    d.first = J.first_path;
+    d.count = J.num_paths;
+    d.paths = J.paths;
+    d.ckpt = J.ckpt;
+    d.dLdI = J.dLdI;
+    d.dpos = J.dpos;
+    d.dmat = J.dmat;
+    d.dtex = J.dtex;
+    d.vn = J.scene->dev.vf_off ? J.vn_scratch : nullptr;
+    d.gc = GradCopies{};
+    if (adjoint) {
+      const uint64_t n3 = 3ull * (uint64_t)d.S.n_verts;
+      const uint32_t k = grad_copies_for(n3);
+      if (k > 1) {
+        d.gc.copies = k;
+        d.gc.n3 = (uint32_t)n3;
+        *grad_floats += n3 * k;
+      }
+    }
+    *total += J.num_paths;
+    if (!*deepest || J.scene->dev.stack_need > (*deepest)->dev.stack_need) *deepest = J.scene;
+  }
+  return jd;
+}
+
+static void assign_grad_copies(std::vector<JobDev>& jd, float* base) {
+  uint64_t off = 0;
+  for (JobDev& d : jd)
+    if (d.gc.copies > 1) {
+      d.gc.base = base + off;
+      off += (uint64_t)d.gc.n3 * d.gc.copies;
+    }
+}
+
+// Default tile: about four tiles per resident wave (waves take tiles
+// round-robin, so finer tiles even out paths of unequal length), at least
+// one path per lane, at most 4096 paths.
+static uint64_t default_tile(const trc_ctx* ctx, uint64_t total) {
+  const uint64_t waves = (uint64_t)ctx->num_cus * kBlocksPerCU * (kBlock / 64);
+  uint64_t tile = total / (waves * 4) / 64 * 64;
+  return tile < 64 ? 64 : (tile > 4096 ? 4096 : tile);
+}
+
+static uint64_t assign_tiles(std::vector<JobDev>& jd, uint64_t tile) {
+  uint64_t ntiles = 0;
+  for (JobDev& d : jd) {
+    d.tile0 = ntiles;
+    ntiles += (d.count + tile - 1) / tile;
+  }
+  return ntiles;
+}
+
 trc_status trc_render_batch(trc_ctx* ctx, const trc_job* jobs, int32_t n, int32_t adjoint, trc_stream stream) {
   if (!ctx) return TRC_ERR_INVALID;
   return guarded(ctx, [&] {
-    if (ctx->device < 0) throw Fail(TRC_ERR_STATE, "host-only context cannot render");
-    if (n < 0 || (n > 0 && !jobs)) throw Fail(TRC_ERR_INVALID, "bad job list");
-    if (n == 0) return;
     hipStream_t st = (hipStream_t)stream;
-    std::vector<JobDev> jd((size_t)n);
-    const trc_scene* deepest = nullptr;
-    uint64_t total = 0;
-    for (int i = 0; i < n; ++i) {
-      const trc_job& J = jobs[i];
-      if (!J.scene || J.scene->ctx != ctx) throw Fail(TRC_ERR_INVALID, "job " + std::to_string(i) + ": bad scene");
-      check_render(J.scene, &J.opt);
-      CameraDev C = to_cam(&J.cam);
-      const uint64_t np = (uint64_t)J.opt.spp * (uint64_t)C.width * (uint64_t)C.height;
-      if (J.num_paths == 0 || J.first_path + J.num_paths > np)
-        throw Fail(TRC_ERR_INVALID, "job " + std::to_string(i) + ": path range outside spp*W*H");
-      if (!J.ckpt) throw Fail(TRC_ERR_INVALID, "job " + std::to_string(i) + ": checkpoint is NULL");
-      if (!adjoint && !J.paths) throw Fail(TRC_ERR_INVALID, "job " + std::to_string(i) + ": paths is NULL");
-      if (adjoint && (!J.dLdI || !J.dpos)) throw Fail(TRC_ERR_INVALID, "job " + std::to_string(i) + ": dLdI / dpos is NULL");
-      if (adjoint && J.scene->dev.vf_off && !J.vn_scratch)
-        throw Fail(TRC_ERR_INVALID, "job " + std::to_string(i) + ": smooth differentiable mesh needs vn_scratch");
-      JobDev& d = jd[(size_t)i];
-      d.S = J.scene->dev;
-      d.C = C;
-      d.seed = J.opt.seed;
-      d.max_bounces = J.opt.max_bounces;
-      d.spp = J.opt.spp;
-      d.pad = 0;
-      d.first = J.first_path;
-      d.count = J.num_paths;
-      d.paths = J.paths;
-      d.ckpt = J.ckpt;
-      d.dLdI = J.dLdI;
-      d.dpos = J.dpos;
-      d.dmat = J.dmat;
-      d.dtex = J.dtex;
-      d.vn = J.scene->dev.vf_off ? J.vn_scratch : nullptr;
-      total += J.num_paths;
-      if (!deepest || J.scene->dev.stack_need > deepest->dev.stack_need) deepest = J.scene;
+    const trc_scene* deepest;
+    uint64_t total, gfloats;
+    std::vector<JobDev> jd = prepare_batch(ctx, jobs, n, adjoint, &deepest, &total, &gfloats);
+    if (n == 0) return;
+    float* gbase = nullptr;
+    if (gfloats) {
+      hip_check(hipMallocAsync((void**)&gbase, gfloats * sizeof(float), st), "hipMallocAsync(gradient copies)");
+      hip_check(launch_fill_zero(gbase, gfloats, st), "k_fill_zero");
+      assign_grad_copies(jd, gbase);
     }
-    // tile size: a few paths per lane of the resident grid, at least 256
     LaunchCfg L = make_cfg(ctx, deepest, st, (long long)total);
-    const uint64_t lanes = (uint64_t)ctx->num_cus * kBlocksPerCU * kBlock;
-    uint64_t tile = (total + lanes - 1) / lanes * 64;
-    tile = tile < 256 ? 256 : (tile > 65536 ? 65536 : tile);
-    tile = (tile + 63) / 64 * 64;
-    uint64_t ntiles = 0;
-    for (JobDev& d : jd) {
-      d.tile0 = ntiles;
-      ntiles += (d.count + tile - 1) / tile;
-    }
+    const uint64_t tile = default_tile(ctx, total);
+    const uint64_t ntiles = assign_tiles(jd, tile);
     // job table: pinned staging -> device (stream-ordered), from a ring slot
     // whose previous batch has completed
     const size_t bytes = jd.size() * sizeof(JobDev);
@@ -620,10 +687,153 @@ trc_status trc_render_batch(trc_ctx* ctx, const trc_job* jobs, int32_t n, int32_
     }
     std::memcpy(b.host, jd.data(), bytes);
     hip_check(hipMemcpyAsync(b.dev, b.host, bytes, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-    hip_check(launch_render_batch((const JobDev*)b.dev, jd.data(), n, ntiles, (uint32_t)tile, adjoint, L, st),
-              adjoint ? "k_render_batch_adj" : "k_render_batch_fwd");
+    hipError_t e = launch_render_batch((const JobDev*)b.dev, jd.data(), n, ntiles, (uint32_t)tile, adjoint, L, st);
+    if (gbase) hipFreeAsync(gbase, st);
+    hip_check(e, adjoint ? "k_render_batch_adj" : "k_render_batch_fwd");
     hip_check(hipEventRecord(b.done, st), "hipEventRecord");
   });
+}
+
+// ---- prepared batches ----
+struct trc_plan {
+  trc_ctx* ctx = nullptr;
+  int adjoint = 0;
+  std::vector<JobDev> jd;
+  JobDev* dev_jobs = nullptr;
+  float* gbase = nullptr;      // persistent gradient copies (re-zeroed by each launch's reduction)
+  uint32_t* spill = nullptr;   // the plan's own traversal spill buffer
+  LaunchCfg L{};
+  uint64_t total = 0, tile = 0, ntiles = 0;
+};
+
+namespace {
+void plan_free(trc_plan* p) {
+  if (p->dev_jobs) hipFree(p->dev_jobs);
+  if (p->gbase) hipFree(p->gbase);
+  if (p->spill) hipFree(p->spill);
+  p->dev_jobs = nullptr;
+  p->gbase = nullptr;
+  p->spill = nullptr;
+}
+
+void plan_upload_tiles(trc_plan* p, uint64_t tile) {
+  p->tile = tile;
+  p->ntiles = assign_tiles(p->jd, tile);
+  hip_check(hipMemcpy(p->dev_jobs, p->jd.data(), p->jd.size() * sizeof(JobDev), hipMemcpyHostToDevice),
+            "hipMemcpy(jobs)");
+}
+}  // namespace
+
+trc_status trc_plan_create(trc_ctx* ctx, const trc_job* jobs, int32_t n, int32_t adjoint, trc_plan** out) {
+  if (!ctx || !out) return TRC_ERR_INVALID;
+  *out = nullptr;
+  return guarded(ctx, [&] {
+    if (n <= 0) throw Fail(TRC_ERR_INVALID, "a plan needs at least one job");
+    std::unique_ptr<trc_plan> p(new trc_plan);
+    p->ctx = ctx;
+    p->adjoint = adjoint ? 1 : 0;
+    const trc_scene* deepest;
+    uint64_t gfloats;
+    p->jd = prepare_batch(ctx, jobs, n, p->adjoint, &deepest, &p->total, &gfloats);
+    try {
+      hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+      hip_check(hipMalloc((void**)&p->dev_jobs, p->jd.size() * sizeof(JobDev)), "hipMalloc(plan jobs)");
+      if (gfloats) {
+        hip_check(hipMalloc((void**)&p->gbase, gfloats * sizeof(float)), "hipMalloc(plan gradient copies)");
+        hip_check(launch_fill_zero(p->gbase, gfloats, nullptr), "k_fill_zero");
+        assign_grad_copies(p->jd, p->gbase);
+        for (JobDev& d : p->jd) d.gc.keep = 1;
+      }
+      // launch shape of the deepest scene, with a spill buffer owned by the plan
+      // so that launches never allocate
+      p->L = make_cfg(ctx, deepest, nullptr, (long long)p->total, false);
+      const size_t words = (size_t)kSpillDepth * (size_t)ctx->num_cus * kBlocksPerCU * kBlock;
+      hip_check(hipMalloc((void**)&p->spill, words * 4), "hipMalloc(plan spill)");
+      p->L.spill = p->spill;
+      plan_upload_tiles(p.get(), default_tile(ctx, p->total));
+      hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    } catch (...) {
+      plan_free(p.get());
+      throw;
+    }
+    *out = p.release();
+  });
+}
+
+trc_status trc_plan_set_tile(trc_plan* plan, uint32_t tile_paths) {
+  if (!plan) return TRC_ERR_INVALID;
+  return guarded(plan->ctx, [&] {
+    if (tile_paths % 64 != 0) throw Fail(TRC_ERR_INVALID, "tile_paths must be a multiple of 64");
+    plan_upload_tiles(plan, tile_paths ? tile_paths : default_tile(plan->ctx, plan->total));
+  });
+}
+
+uint32_t trc_plan_tile(const trc_plan* plan) { return plan ? (uint32_t)plan->tile : 0u; }
+
+trc_status trc_plan_launch(trc_plan* plan, trc_stream stream) {
+  if (!plan) return TRC_ERR_INVALID;
+  return guarded(plan->ctx, [&] {
+    hip_check(launch_render_batch(plan->dev_jobs, plan->jd.data(), (int)plan->jd.size(), plan->ntiles,
+                                  (uint32_t)plan->tile, plan->adjoint, plan->L, (hipStream_t)stream),
+              plan->adjoint ? "k_render_batch_adj" : "k_render_batch_fwd");
+  });
+}
+
+trc_status trc_plan_tune(trc_plan* plan, const uint32_t* candidates, int32_t n, int32_t reps, uint32_t* best,
+                         trc_stream stream) {
+  if (!plan) return TRC_ERR_INVALID;
+  return guarded(plan->ctx, [&] {
+    if (n <= 0 || !candidates || reps <= 0) throw Fail(TRC_ERR_INVALID, "no candidates to tune over");
+    for (int i = 0; i < n; ++i)
+      if (candidates[i] == 0 || candidates[i] % 64 != 0)
+        throw Fail(TRC_ERR_INVALID, "candidate tiles must be positive multiples of 64");
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    float best_ms = 0.0f;
+    uint32_t best_tile = 0;
+    std::string err;
+    for (int i = 0; i < n && err.empty(); ++i) {
+      plan_upload_tiles(plan, candidates[i]);
+      std::vector<float> ms;
+      for (int r = 0; r <= reps; ++r) {  // the first launch warms up
+        hipError_t e = hipEventRecord(e0, st);
+        if (e == hipSuccess)
+          e = launch_render_batch(plan->dev_jobs, plan->jd.data(), (int)plan->jd.size(), plan->ntiles,
+                                  (uint32_t)plan->tile, plan->adjoint, plan->L, st);
+        if (e == hipSuccess) e = hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float t = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        if (e != hipSuccess) {
+          err = std::string("plan tuning: ") + hipGetErrorString(e);
+          break;
+        }
+        if (r > 0) ms.push_back(t);
+      }
+      if (!err.empty()) break;
+      std::sort(ms.begin(), ms.end());
+      const float med = ms[ms.size() / 2];
+      if (best_tile == 0 || med < best_ms) {
+        best_ms = med;
+        best_tile = candidates[i];
+      }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (!err.empty()) throw Fail(TRC_ERR_HIP, err);
+    plan_upload_tiles(plan, best_tile);
+    if (best) *best = best_tile;
+  });
+}
+
+trc_status trc_plan_destroy(trc_plan* plan) {
+  if (!plan) return TRC_OK;
+  hipDeviceSynchronize();
+  plan_free(plan);
+  delete plan;
+  return TRC_OK;
 }
 
 trc_status trc_render_adj_ckpt(trc_scene* s, const trc_camera* cam, const trc_options* opt, const void* ckpt,
@@ -639,9 +849,11 @@ trc_status trc_render_adj_ckpt(trc_scene* s, const trc_camera* cam, const trc_op
     if (ckpt_bytes < (int64_t)checkpoint_bytes((uint64_t)np, opt->max_bounces))
       throw Fail(TRC_ERR_INVALID, "checkpoint buffer too small (trc_checkpoint_bytes)");
     LaunchCfg L = make_cfg(s->ctx, s, (hipStream_t)stream, np);
-    hip_check(launch_render_adj_ckpt(s->dev, C, opt->seed, opt->max_bounces, opt->spp, L, ckpt, dLdI, dpos, dmat,
-                                     dtex, s->vn_adj, (hipStream_t)stream),
-              "k_render_adj_ckpt");
+    GradCopies gc = make_grad_copies(s->dev.n_verts, (hipStream_t)stream);
+    hipError_t e = launch_render_adj_ckpt(s->dev, C, opt->seed, opt->max_bounces, opt->spp, L, ckpt, dLdI, dpos, dmat,
+                                          dtex, s->vn_adj, gc, (hipStream_t)stream);
+    if (gc.base) hipFreeAsync(gc.base, (hipStream_t)stream);
+    hip_check(e, "k_render_adj_ckpt");
   });
 }
 
@@ -761,9 +973,11 @@ trc_status trc_render_adj(trc_scene* s, const trc_camera* cam, const trc_options
     if (!dLdI || !dpos) throw Fail(TRC_ERR_INVALID, "dLdI / dpos is NULL");
     long long np = (long long)opt->spp * C.width * C.height;
     LaunchCfg L = make_cfg(s->ctx, s, (hipStream_t)stream, np);
-    hip_check(launch_render_adj(s->dev, C, opt->seed, opt->max_bounces, opt->spp, L, dLdI, dpos, dmat, dtex,
-                                s->vn_adj, (hipStream_t)stream),
-              "k_render_adj");
+    GradCopies gc = make_grad_copies(s->dev.n_verts, (hipStream_t)stream);
+    hipError_t e = launch_render_adj(s->dev, C, opt->seed, opt->max_bounces, opt->spp, L, dLdI, dpos, dmat, dtex,
+                                     s->vn_adj, gc, (hipStream_t)stream);
+    if (gc.base) hipFreeAsync(gc.base, (hipStream_t)stream);
+    hip_check(e, "k_render_adj");
   });
 }
 

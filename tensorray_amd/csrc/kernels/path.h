@@ -324,21 +324,67 @@ __device__ inline void lane_setup_nee(const SceneDev& S, const PathParams& P, co
   L.any = true;
 }
 
-// The wave renders paths [begin, end) (wave-uniform bounds).
+// The path ranges a wave renders: tiles t, t + step, ... (below tile `tend`)
+// of `tile` paths each, tile k covering [base + k*tile, ...) clipped to
+// `limit`.  The wave moves to its next tile as soon as the current one is
+// handed out, without waiting for its lanes to drain.  All fields are
+// wave-uniform.
+struct TileSupply {
+  uint64_t next, end;            // paths not yet handed out of the current tile
+  uint64_t t, tend, step;        // tile cursor
+  uint64_t base, limit, tile, t0;
+  __device__ void load() {
+    next = base + (t - t0) * tile;
+    end = next + tile < limit ? next + tile : limit;
+  }
+  // Starts at tile t; tiles t0.. map to paths base..limit.
+  __device__ static TileSupply make(uint64_t t, uint64_t tend, uint64_t step, uint64_t t0, uint64_t base,
+                                    uint64_t limit, uint64_t tile) {
+    TileSupply s;
+    s.t = t;
+    s.tend = tend;
+    s.step = step;
+    s.t0 = t0;
+    s.base = base;
+    s.limit = limit;
+    s.tile = tile;
+    if (t < tend) s.load();
+    else s.next = s.end = 0;
+    return s;
+  }
+  // Moves to the wave's next tile; false (and an empty range) when none is
+  // left, in which case t stays at the first tile past tend.
+  __device__ bool more() {
+    if (t >= tend) {
+      next = end;
+      return false;
+    }
+    t += step;
+    if (t >= tend) {
+      next = end;
+      return false;
+    }
+    load();
+    return true;
+  }
+};
+
+// The wave renders the paths of its tile supply.
 template <bool kCkpt, class Finish>
 __device__ void render_paths_regen(const SceneDev& S, const CameraDev& C, const PathParams& P, TraceCtx& tc,
-                                   uint64_t begin, uint64_t end, const CkptDev& ck, Finish finish) {
+                                   TileSupply& sup, const CkptDev& ck, Finish finish) {
   const F3 zero{0.0f, 0.0f, 0.0f};
   const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
   const F3 eye{C.eye[0], C.eye[1], C.eye[2]};
   const F3 cdir{C.dir[0], C.dir[1], C.dir[2]};
   const int lane = (int)(threadIdx.x & 63u);
-  uint64_t next = begin;  // wave-uniform
   PathLane L;
   L.stage = kStageIdle;
   uint32_t meta = 0u;  // checkpoint meta word of the current path
   for (;;) {
-    // ---- regenerate: idle lanes take the next paths of the wave's chunk ----
+    // ---- regenerate: idle lanes take the next paths of the wave's tiles ----
+    if (sup.next >= sup.end) sup.more();
+    const uint64_t next = sup.next, end = sup.end;
     const bool idle = L.stage == kStageIdle;
     const uint64_t idle_mask = __ballot(idle);
     if (idle) {
@@ -358,7 +404,7 @@ __device__ void render_paths_regen(const SceneDev& S, const CameraDev& C, const 
     }
     {
       const uint64_t n_idle = (uint64_t)__popcll(idle_mask);
-      next = next + n_idle < end ? next + n_idle : end;
+      sup.next = next + n_idle < end ? next + n_idle : end;
     }
     if (__ballot(L.stage != kStageIdle) == 0ull) break;
     // ---- one ray per lane ----

@@ -49,16 +49,15 @@ __global__ void __launch_bounds__(kBlock) k_intersect(SceneDev S, LaunchCfg L, c
 
 template <bool kCkpt>
 __global__ void __launch_bounds__(kBlock) k_render_paths(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
-                                                         uint64_t npaths, float* paths, CkptDev ck) {
+                                                         uint64_t npaths, uint32_t tile, float* paths, CkptDev ck) {
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx(lds_stack, L);
-  // one contiguous chunk of paths per wave
+  // tiles of `tile` paths, dealt to the waves round-robin
   const uint64_t nwaves = (uint64_t)L.grid * (kBlock / 64);
   const uint64_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
-  const uint64_t chunk = (npaths + nwaves - 1) / nwaves;
-  const uint64_t begin = wave * chunk < npaths ? wave * chunk : npaths;
-  const uint64_t end = begin + chunk < npaths ? begin + chunk : npaths;
-  render_paths_regen<kCkpt>(S, C, P, tc, begin, end, ck, [&](uint64_t p, F3 c) {
+  const uint64_t ntiles = (npaths + tile - 1) / tile;
+  TileSupply sup = TileSupply::make(wave, ntiles, nwaves, 0, 0, npaths, tile);
+  render_paths_regen<kCkpt>(S, C, P, tc, sup, ck, [&](uint64_t p, F3 c) {
     paths[3 * p] = c.x;
     paths[3 * p + 1] = c.y;
     paths[3 * p + 2] = c.z;
@@ -103,7 +102,8 @@ __global__ void k_film_reduce(const float* paths, int spp, int wh, float* film) 
 
 __global__ void __launch_bounds__(kBlock) k_render_adj(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
                                                        uint64_t npaths, const float* dLdI, float* dpos,
-                                                       float* dmat, float* adj_vn, float* dtex) {
+                                                       float* dmat, float* adj_vn, float* dtex, GradCopies gc) {
+  if (gc.base) dpos = gc.base + (size_t)(blockIdx.x % gc.copies) * gc.n3;
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   __shared__ float mat_grad[kMaxMatGrad * kMatGradStride];
   for (int i = threadIdx.x; i < kMaxMatGrad * kMatGradStride; i += kBlock) mat_grad[i] = 0.0f;
@@ -132,7 +132,9 @@ __global__ void __launch_bounds__(kBlock) k_render_adj(SceneDev S, CameraDev C, 
 // hits (no traversal), then swept backwards as in k_render_adj.
 __global__ void __launch_bounds__(kBlock) k_render_adj_ckpt(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
                                                             uint64_t npaths, CkptDev ck, const float* dLdI,
-                                                            float* dpos, float* dmat, float* adj_vn, float* dtex) {
+                                                            float* dpos, float* dmat, float* adj_vn, float* dtex,
+                                                            GradCopies gc) {
+  if (gc.base) dpos = gc.base + (size_t)(blockIdx.x % gc.copies) * gc.n3;
   __shared__ float mat_grad[kMaxMatGrad * kMatGradStride];
   for (int i = threadIdx.x; i < kMaxMatGrad * kMatGradStride; i += kBlock) mat_grad[i] = 0.0f;
   __syncthreads();
@@ -163,6 +165,47 @@ __global__ void __launch_bounds__(kBlock) k_vertex_normal_adj(SceneDev S, float*
 }
 
 // ---------------------------------------------------------------------------
+// Privatized gradient copies
+// ---------------------------------------------------------------------------
+__global__ void k_fill_zero(float* p, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0f;
+}
+
+__global__ void k_grad_reduce(float* base, uint32_t copies, uint32_t n3, float* dpos, uint32_t rezero) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += stride) {
+    float s = 0.0f;
+    for (uint32_t k = 0; k < copies; ++k) {
+      float* q = base + (size_t)k * n3 + i;
+      s += *q;
+      if (rezero) *q = 0.0f;
+    }
+    if (s != 0.0f) dpos[i] += s;
+  }
+}
+
+uint32_t grad_copies_for(uint64_t n3) {
+  if (n3 == 0 || n3 > (4u << 20)) return 1;
+  uint64_t k = (16u << 20) / n3;  // at most 64 MB of copies
+  return (uint32_t)(k > 64 ? 64 : (k < 1 ? 1 : k));
+}
+
+hipError_t launch_fill_zero(float* p, uint64_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  uint64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_grad_reduce(const GradCopies& g, float* dpos, hipStream_t st) {
+  const uint32_t blocks = (g.n3 + 255) / 256;
+  hipLaunchKernelGGL(k_grad_reduce, dim3(blocks < 4096 ? blocks : 4096), dim3(256), 0, st, g.base, g.copies, g.n3,
+                     dpos, g.keep);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Batched launches: the grid walks the batch's tiles (contiguous path ranges
 // of one job); a wave takes tiles wave, wave + nwaves, ...  The job of a tile
 // is wave-uniform, so its scene / camera descriptors are read once per tile
@@ -189,13 +232,13 @@ __device__ inline int job_of_tile(const JobDev* jobs, int njobs, uint64_t t) {
   return __builtin_amdgcn_readfirstlane(lo);
 }
 
-__global__ void __launch_bounds__(kBlock) k_render_batch_fwd(const JobDev* jobs, int njobs, uint64_t ntiles,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) k_render_batch_fwd(const JobDev* jobs, int njobs, uint64_t ntiles,
                                                              uint32_t tile, LaunchCfg L) {
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx(lds_stack, L);
   const uint64_t nwaves = (uint64_t)L.grid * (kBlock / 64);
   const uint64_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
-  for (uint64_t t = wave; t < ntiles; t += nwaves) {
+  for (uint64_t t = wave; t < ntiles;) {
     const int j = job_of_tile(jobs, njobs, t);
     const SceneDev S = wave_uniform_load(&jobs[j].S);
     const CameraDev C = wave_uniform_load(&jobs[j].C);
@@ -204,13 +247,15 @@ __global__ void __launch_bounds__(kBlock) k_render_batch_fwd(const JobDev* jobs,
     const uint64_t npaths = (uint64_t)P.spp * (uint64_t)C.width * (uint64_t)C.height;
     const CkptDev ck = ckpt_view(jobs[j].ckpt, npaths, P.max_bounces);
     float* paths = jobs[j].paths;
-    const uint64_t begin = first + (t - t0) * tile;
-    const uint64_t end = begin + tile < first + count ? begin + tile : first + count;
-    render_paths_regen<true>(S, C, P, tc, begin, end, ck, [&](uint64_t p, F3 c) {
+    // the wave's tiles of this job, rendered without draining between them
+    const uint64_t tend = t0 + (count + tile - 1) / tile;
+    TileSupply sup = TileSupply::make(t, tend, nwaves, t0, first, first + count, tile);
+    render_paths_regen<true>(S, C, P, tc, sup, ck, [&](uint64_t p, F3 c) {
       paths[3 * p] = c.x;
       paths[3 * p + 1] = c.y;
       paths[3 * p + 2] = c.z;
     });
+    t = sup.t;
   }
 }
 
@@ -233,7 +278,8 @@ __global__ void __launch_bounds__(kBlock) k_render_batch_adj(const JobDev* jobs,
     const uint64_t npaths = (uint64_t)P.spp * (uint64_t)WH;
     const CkptDev ck = ckpt_view(jobs[j].ckpt, npaths, P.max_bounces);
     const float* dLdI = jobs[j].dLdI;
-    float* dpos = jobs[j].dpos;
+    const GradCopies gc = wave_uniform_load(&jobs[j].gc);
+    float* dpos = gc.base ? gc.base + (size_t)(blockIdx.x % gc.copies) * gc.n3 : jobs[j].dpos;
     float* dmat = jobs[j].dmat;
     float* dtex = jobs[j].dtex;
     float* vn = jobs[j].vn;
@@ -277,6 +323,8 @@ hipError_t launch_render_batch(const JobDev* jobs_dev, const JobDev* jobs_host, 
   hipError_t e = hipGetLastError();
   for (int j = 0; j < njobs && e == hipSuccess; ++j) {
     const JobDev& J = jobs_host[j];
+    if (J.gc.base) e = launch_grad_reduce(J.gc, J.dpos, st);
+    if (e != hipSuccess) break;
     if (!J.vn || !J.S.vf_off || J.S.n_verts == 0) continue;
     const int grid = (J.S.n_verts + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_vertex_normal_adj, dim3(grid), dim3(kBlock), 0, st, J.S, J.vn, J.dpos);
@@ -292,6 +340,13 @@ hipError_t launch_intersect(const SceneDev& S, const LaunchCfg& L, const float* 
   return hipGetLastError();
 }
 
+// About four tiles per wave of the grid, 64..4096 paths each.
+static uint32_t forward_tile(uint64_t npaths, int grid) {
+  const uint64_t waves = (uint64_t)grid * (kBlock / 64);
+  const uint64_t t = npaths / (waves * 4) / 64 * 64;
+  return (uint32_t)(t < 64 ? 64 : (t > 4096 ? 4096 : t));
+}
+
 hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                                const LaunchCfg& L, float* paths, void* ckpt, hipStream_t st) {
   PathParams P{seed, max_bounces, spp, 0u};
@@ -299,11 +354,13 @@ hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t s
   if (ckpt) {
     const CkptDev ck = ckpt_view(ckpt, np, max_bounces);
     const LaunchCfg Lk = fit_grid(k_render_paths<true>, L);
-    hipLaunchKernelGGL(k_render_paths<true>, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, np, paths, ck);
+    hipLaunchKernelGGL(k_render_paths<true>, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, np,
+                       forward_tile(np, Lk.grid), paths, ck);
   } else {
     const CkptDev ck{};
     const LaunchCfg Lk = fit_grid(k_render_paths<false>, L);
-    hipLaunchKernelGGL(k_render_paths<false>, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, np, paths, ck);
+    hipLaunchKernelGGL(k_render_paths<false>, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, np,
+                       forward_tile(np, Lk.grid), paths, ck);
   }
   return hipGetLastError();
 }
@@ -312,14 +369,15 @@ uint64_t checkpoint_bytes(uint64_t npaths, int max_bounces) { return ckpt_bytes(
 
 hipError_t launch_render_adj_ckpt(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                                   const LaunchCfg& L, const void* ckpt, const float* dLdI, float* dpos, float* dmat,
-                                  float* dtex, float* vn_adj, hipStream_t st) {
+                                  float* dtex, float* vn_adj, const GradCopies& gc, hipStream_t st) {
   PathParams P{seed, max_bounces, spp, 0u};
   uint64_t np = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
   const CkptDev ck = ckpt_view(const_cast<void*>(ckpt), np, max_bounces);
   const LaunchCfg Lk = fit_grid(k_render_adj_ckpt, L);
   hipLaunchKernelGGL(k_render_adj_ckpt, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, np, ck, dLdI, dpos, dmat,
-                     vn_adj, dtex);
+                     vn_adj, dtex, gc);
   hipError_t e = hipGetLastError();
+  if (e == hipSuccess && gc.base) e = launch_grad_reduce(gc, dpos, st);
   if (e != hipSuccess || !vn_adj || S.n_verts == 0) return e;
   const int grid = (S.n_verts + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_vertex_normal_adj, dim3(grid), dim3(kBlock), 0, st, S, vn_adj, dpos);
@@ -342,12 +400,14 @@ hipError_t launch_film_reduce(const float* paths, int spp, int wh, float* film, 
 
 hipError_t launch_render_adj(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                              const LaunchCfg& L, const float* dLdI, float* dpos, float* dmat, float* dtex,
-                             float* vn_adj, hipStream_t st) {
+                             float* vn_adj, const GradCopies& gc, hipStream_t st) {
   PathParams P{seed, max_bounces, spp, 0u};
   uint64_t np = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
   const LaunchCfg Lk = fit_grid(k_render_adj, L);
-  hipLaunchKernelGGL(k_render_adj, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, np, dLdI, dpos, dmat, vn_adj, dtex);
+  hipLaunchKernelGGL(k_render_adj, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, np, dLdI, dpos, dmat, vn_adj, dtex,
+                     gc);
   hipError_t e = hipGetLastError();
+  if (e == hipSuccess && gc.base) e = launch_grad_reduce(gc, dpos, st);
   if (e != hipSuccess || !vn_adj || S.n_verts == 0) return e;
   const int grid = (S.n_verts + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_vertex_normal_adj, dim3(grid), dim3(kBlock), 0, st, S, vn_adj, dpos);

@@ -93,6 +93,21 @@ inline LaunchCfg fit_grid(Kernel kernel, LaunchCfg L, int block = kBlock) {
 
 hipError_t launch_intersect(const SceneDev& S, const LaunchCfg& L, const float* rays, int n, int32_t* tri,
                             float* uvt, int any_hit, hipStream_t st);
+// Privatized vertex-gradient accumulation: `copies` zeroed copies of the
+// [V][3] gradient; block b adds into copy b % copies, and a reduction adds
+// them into dpos.  Cuts float-atomic contention on small meshes, where every
+// path lands on the same few hundred vertices.
+struct GradCopies {
+  float* base = nullptr;  // [copies][n3] or null (atomics go to dpos directly)
+  uint32_t copies = 1;
+  uint32_t n3 = 0;        // 3 * V
+  uint32_t keep = 0;      // nonzero: the copies outlive the launch, and the reduction re-zeroes them
+};
+// Number of copies for a mesh of `n3` gradient floats (1 = none).
+uint32_t grad_copies_for(uint64_t n3);
+hipError_t launch_fill_zero(float* p, uint64_t n, hipStream_t st);
+hipError_t launch_grad_reduce(const GradCopies& g, float* dpos, hipStream_t st);
+
 // One job of a batched launch (device copy of trc_job).
 struct JobDev {
   SceneDev S;
@@ -108,6 +123,7 @@ struct JobDev {
   float* dmat;
   float* dtex;
   float* vn;
+  GradCopies gc;
 };
 // tiles of `tile` paths; jobs sorted by tile0
 hipError_t launch_render_batch(const JobDev* jobs_dev, const JobDev* jobs_host, int njobs, uint64_t ntiles,
@@ -119,7 +135,7 @@ hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t s
 uint64_t checkpoint_bytes(uint64_t npaths, int max_bounces);
 hipError_t launch_render_adj_ckpt(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                                   const LaunchCfg& L, const void* ckpt, const float* dLdI, float* dpos, float* dmat,
-                                  float* dtex, float* vn_adj, hipStream_t st);
+                                  float* dtex, float* vn_adj, const GradCopies& gc, hipStream_t st);
 hipError_t launch_render_stats(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                                const LaunchCfg& L, unsigned long long* counters, hipStream_t st);
 hipError_t launch_film_reduce(const float* paths, int spp, int wh, float* film, hipStream_t st);
@@ -145,6 +161,6 @@ hipError_t launch_boundary(int kind, const SceneDev& S, const CameraDev& C, uint
 // null; it is folded into dpos and cleared again by a second kernel.
 hipError_t launch_render_adj(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                              const LaunchCfg& L, const float* dLdI, float* dpos, float* dmat, float* dtex,
-                             float* vn_adj, hipStream_t st);
+                             float* vn_adj, const GradCopies& gc, hipStream_t st);
 
 }  // namespace trc

@@ -1,0 +1,299 @@
+"""Serving loop: continuous batching of render-and-gradient requests on one
+GPU.
+
+The reference answers one synchronous request per call (TensorRay.cpp:
+192-221); here a server keeps scenes resident in HBM and runs many requests
+of different scenes, resolutions, sample counts and depths through the same
+kernel launches (trc_render_batch):
+
+* admission: a request is admitted when its transient buffers (per-path
+  radiance, path checkpoint, images, gradient buffers) fit the HBM budget
+  next to the requests already in flight; otherwise it waits in FIFO order;
+* iteration-level scheduling: every step forms one forward batch and one
+  adjoint batch from chunks (path ranges) of the admitted requests, oldest
+  first, up to `batch_paths` paths each; a request whose forward finishes in
+  a step gets its film reduction and dL/dI = I - target on the stream and
+  joins the adjoint batch of the same step; requests arriving meanwhile are
+  admitted at the next step, so their work overlaps with requests already on
+  the device;
+* completion: an event per request; latency = completion - arrival.
+
+Each path's result does not depend on the batch it ran in (counter-based
+RNG, per-request buffers), so a request's film is bit-identical alone and in
+a mix, and its gradients agree up to the order of float atomics.
+"""
+import collections
+import dataclasses
+import time
+from typing import Dict, List, Optional
+
+from tensorray_amd import _capi
+
+
+@dataclasses.dataclass
+class Request:
+    scene: str
+    width: int
+    height: int
+    spp: int
+    max_bounces: int
+    seed: int
+    cam: dict                       # pos, target, up, fov
+    target: object = None           # torch [3][H*W] on the device; dL/dI = I - target (zeros if None)
+    arrival: float = 0.0            # seconds after the start of the run
+    rid: int = -1
+    # results
+    film: object = None             # torch [3][H*W]
+    dpos: object = None             # torch [V][3]
+    dmat: object = None             # torch [M][8]
+    t_admit: Optional[float] = None
+    t_done: Optional[float] = None
+
+    @property
+    def paths(self):
+        return self.spp * self.width * self.height
+
+    @property
+    def latency(self):
+        return None if self.t_done is None else self.t_done - self.arrival
+
+
+class _Active:
+    """Device state of an admitted request."""
+
+    def __init__(self, req, ds, torch, stream):
+        self.req = req
+        self.ds = ds
+        info = ds.info()
+        self.cam = _capi.make_camera(req.width, req.height, **req.cam)
+        self.opt = _capi.make_options(seed=req.seed, max_bounces=req.max_bounces, spp=req.spp)
+        P = req.paths
+        self.nb = ds.checkpoint_bytes(self.cam, self.opt)
+        with torch.cuda.stream(stream):
+            self.paths = torch.empty(P, 3, device="cuda")
+            self.ckpt = torch.empty(self.nb, dtype=torch.uint8, device="cuda")
+            self.film = torch.empty(3, req.width * req.height, device="cuda")
+            self.dldi = None
+            req.dpos = torch.zeros(info.num_vertices, 3, device="cuda")
+            req.dmat = torch.zeros(info.num_materials, 8, device="cuda")
+            self.vn = torch.zeros(info.num_vertices, 3, device="cuda")
+        self.fwd_next = 0
+        self.adj_next = 0
+        self.done_event = None
+
+    @staticmethod
+    def bytes_needed(req, ds):
+        info = ds.info()
+        P = req.paths
+        ck = P * (4 + 12 * (req.max_bounces + 1))
+        img = 4 * 3 * req.width * req.height * 3          # film, dL/dI, target
+        grads = 4 * (3 * info.num_vertices * 2 + 8 * info.num_materials)
+        return 12 * P + ck + img + grads
+
+
+class Server:
+    """Continuous-batching server over resident scenes (one GPU, one stream)."""
+
+    def __init__(self, ctx, budget_bytes=64 << 30, batch_paths=8 << 20, chunk_paths=4 << 20, max_inflight=2):
+        import torch
+        self.torch = torch
+        self.ctx = ctx
+        self.budget = int(budget_bytes)
+        self.batch_paths = int(batch_paths)
+        self.chunk_paths = int(chunk_paths)
+        self.scenes: Dict[str, _capi.DeviceScene] = {}
+        self.waiting = collections.deque()
+        self.active: List[_Active] = []
+        self.finishing: List[_Active] = []
+        self.in_use = 0
+        self.stream = torch.cuda.current_stream()
+        self.t0 = None
+        self.steps = 0
+        self.peak_in_use = 0
+        # at most max_inflight steps queued on the GPU, so that arrivals join
+        # the next batch instead of queueing behind many launched ones
+        self.max_inflight = max_inflight
+        self.inflight = collections.deque()
+
+    def add_scene(self, name, spec):
+        """Uploads a scene (BVH, triangles, textures) once; it stays resident."""
+        self.scenes[name] = _capi.DeviceScene(self.ctx, spec)
+        return self.scenes[name]
+
+    def now(self):
+        return time.perf_counter() - self.t0
+
+    def submit(self, req):
+        if req.scene not in self.scenes:
+            raise KeyError(f"unknown scene {req.scene!r}")
+        need = _Active.bytes_needed(req, self.scenes[req.scene])
+        if need > self.budget:
+            raise ValueError(f"request {req.rid} needs {need} bytes, more than the whole budget {self.budget}")
+        self.waiting.append(req)
+
+    def _admit(self):
+        while self.waiting:
+            req = self.waiting[0]
+            ds = self.scenes[req.scene]
+            need = _Active.bytes_needed(req, ds)
+            if self.in_use + need > self.budget:
+                break
+            self.waiting.popleft()
+            a = _Active(req, ds, self.torch, self.stream)
+            a.bytes = need
+            self.in_use += need
+            self.peak_in_use = max(self.peak_in_use, self.in_use)
+            req.t_admit = self.now()
+            self.active.append(a)
+
+    def _collect(self):
+        """Retires requests whose adjoint has completed."""
+        still = []
+        for a in self.finishing:
+            if a.done_event.query():
+                a.req.t_done = self.now()
+                self.in_use -= a.bytes
+                a.paths = a.ckpt = a.dldi = a.vn = None
+            else:
+                still.append(a)
+        self.finishing = still
+
+    def step(self):
+        """One scheduling iteration; returns False when there is nothing to do."""
+        torch = self.torch
+        self._collect()
+        self._admit()
+        if not self.active:
+            return bool(self.finishing or self.waiting)
+        while len(self.inflight) >= self.max_inflight:
+            self.inflight.popleft().synchronize()
+        self.steps += 1
+        # forward batch: chunks of requests still in their forward pass, oldest first
+        fjobs, fdone, budget = [], [], self.batch_paths
+        for a in self.active:
+            if budget <= 0:
+                break
+            P = a.req.paths
+            if a.fwd_next >= P:
+                continue
+            n = min(P - a.fwd_next, self.chunk_paths, budget)
+            fjobs.append(a.ds.job(a.cam, a.opt, a.fwd_next, n, paths=a.paths.data_ptr(), ckpt=a.ckpt.data_ptr()))
+            a.fwd_next += n
+            budget -= n
+            if a.fwd_next >= P:
+                fdone.append(a)
+        if fjobs:
+            self.ctx.render_batch(fjobs, adjoint=False, stream=self.stream)
+        with torch.cuda.stream(self.stream):
+            for a in fdone:
+                a.ds.render_fwd_reduce(a.paths.data_ptr(), a.req.spp, a.req.width * a.req.height,
+                                       a.film.data_ptr(), self.stream)
+                a.dldi = a.film - a.req.target if a.req.target is not None else a.film.clone()
+                a.req.film = a.film
+        # adjoint batch: chunks of requests whose forward is complete
+        ajobs, adone, budget = [], [], self.batch_paths
+        for a in self.active:
+            if budget <= 0:
+                break
+            P = a.req.paths
+            if a.fwd_next < P or a.adj_next >= P:
+                continue
+            n = min(P - a.adj_next, self.chunk_paths, budget)
+            ajobs.append(a.ds.job(a.cam, a.opt, a.adj_next, n, ckpt=a.ckpt.data_ptr(), dldi=a.dldi.data_ptr(),
+                                  dpos=a.req.dpos.data_ptr(), dmat=a.req.dmat.data_ptr(),
+                                  vn_scratch=a.vn.data_ptr()))
+            a.adj_next += n
+            budget -= n
+            if a.adj_next >= P:
+                adone.append(a)
+        if ajobs:
+            self.ctx.render_batch(ajobs, adjoint=True, stream=self.stream)
+        for a in adone:
+            a.done_event = torch.cuda.Event()
+            a.done_event.record(self.stream)
+            self.active.remove(a)
+            self.finishing.append(a)
+        ev = torch.cuda.Event()
+        ev.record(self.stream)
+        self.inflight.append(ev)
+        return True
+
+    def run(self, requests, realtime=True):
+        """Serves `requests` (sorted by arrival); with realtime, a request is
+        submitted once its arrival time has passed.  Returns the requests."""
+        torch = self.torch
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        torch.cuda.synchronize()
+        self.t0 = time.perf_counter()
+        i = 0
+        while i < len(reqs) or self.waiting or self.active or self.finishing:
+            t = self.now()
+            while i < len(reqs) and (not realtime or reqs[i].arrival <= t):
+                self.submit(reqs[i])
+                i += 1
+            busy = self.step()
+            if not busy and i < len(reqs):
+                # idle until the next arrival
+                time.sleep(max(0.0, min(reqs[i].arrival - self.now(), 0.001)))
+            elif self.finishing and not self.active and not self.waiting:
+                time.sleep(0.0001)
+        torch.cuda.synchronize()
+        self._collect()
+        return reqs
+
+
+def run_plain(ctx, scenes, requests, realtime=True):
+    """The plain path: one request at a time, each a forward (with
+    checkpoint) + dL/dI + adjoint launch sequence, completed before the next
+    starts."""
+    import torch
+    reqs = sorted(requests, key=lambda r: r.arrival)
+    stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in reqs:
+        if realtime:
+            while time.perf_counter() - t0 < r.arrival:
+                time.sleep(0.0002)
+        ds = scenes[r.scene]
+        info = ds.info()
+        cam = _capi.make_camera(r.width, r.height, **r.cam)
+        opt = _capi.make_options(seed=r.seed, max_bounces=r.max_bounces, spp=r.spp)
+        nb = ds.checkpoint_bytes(cam, opt)
+        ck = torch.empty(nb, dtype=torch.uint8, device="cuda")
+        r.film = torch.empty(3, r.width * r.height, device="cuda")
+        ds.render_fwd_ckpt(cam, opt, r.film.data_ptr(), ck.data_ptr(), nb, stream)
+        dldi = r.film - r.target if r.target is not None else r.film.clone()
+        r.dpos = torch.zeros(info.num_vertices, 3, device="cuda")
+        r.dmat = torch.zeros(info.num_materials, 8, device="cuda")
+        ds.render_adj_ckpt(cam, opt, ck.data_ptr(), nb, dldi.data_ptr(), r.dpos.data_ptr(), r.dmat.data_ptr(),
+                           stream=stream)
+        torch.cuda.synchronize()
+        r.t_done = time.perf_counter() - t0
+    return reqs
+
+
+def percentiles(values, qs=(50, 99)):
+    import numpy as np
+    v = np.asarray(sorted(values), float)
+    return {f"p{q}": float(np.percentile(v, q)) for q in qs}
+
+
+def make_requests(scene_names, n, rate, seed=2024, res=(64, 1024), spp=(4, 64), depth=(2, 6), cams=None):
+    """Poisson arrivals (mean `rate` requests / s) of requests with
+    resolution, spp and depth drawn uniformly (numpy PCG64(seed)); depth d
+    means max_bounces d - 1."""
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        side = int(rng.integers(res[0], res[1] + 1))
+        out.append(Request(scene=scene_names[int(rng.integers(len(scene_names)))], width=side, height=side,
+                           spp=int(rng.integers(spp[0], spp[1] + 1)),
+                           max_bounces=int(rng.integers(depth[0], depth[1] + 1)) - 1, seed=int(rng.integers(1 << 30)),
+                           cam=dict(cams[0]) if cams else dict(pos=(0.0, 1.0, 3.0), target=(0.0, 0.0, 0.0),
+                                                               up=(0.0, 1.0, 0.0), fov=45.0),
+                           arrival=t, rid=i))
+    return out
