@@ -47,8 +47,13 @@ def parse():
                    help="adjoint from the forward's path checkpoint (no re-tracing) or by replaying the paths")
     p.add_argument("--serve-requests", type=int, default=48,
                    help="requests of the served-vs-plain comparison (0 skips it)")
+    p.add_argument("--serve-sweep", action="store_true",
+                   help="also run the serving sweep: 4 resident scenes (1k / 20k / 200k / 1M triangles), "
+                        "res U(64..1024), spp U(4..64), depth U(2..6), Poisson rates 0.25..1.25 x plain capacity")
     p.add_argument("--serve-large-scenes", action="store_true",
                    help="serve over 1k / 20k / 200k / 1M-triangle scenes (slower set-up) instead of 1k / 20k")
+    p.add_argument("--multiview-views", type=int, default=32,
+                   help="views of the multi-view workload (0 skips it)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="after the timed region, copy a MIB-sized device buffer (reference kernel of known "
                         "traffic for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -163,6 +168,125 @@ def served_comparison(ctx, a):
     return out
 
 
+def served_sweep(ctx, a):
+    """Serving over four resident scenes of 1,280 / 20,480 / 200,000 /
+    1,003,520 triangles (displaced spheres on a ground plane): requests with
+    resolution uniform over 64..1024, spp over 4..64, depth over 2..6
+    (numpy PCG64 seed 2024).  Saturation throughput of plain and served, then
+    p50 / p99 latency under Poisson arrivals at 0.25 .. 1.25 x the plain
+    path's measured request rate."""
+    from tensorray_amd import scenes, serving
+    t0 = time.perf_counter()
+    specs = {"s1k": scenes.sphere_scene(subdiv=3, amplitude=0.05, noise_seed=2),
+             "s20k": scenes.sphere_scene(subdiv=5, amplitude=0.05, noise_seed=1),
+             "s200k": scenes.sphere_scene(freq=100, amplitude=0.05, noise_seed=3),
+             "s1m": scenes.sphere_scene(freq=224, amplitude=0.05, noise_seed=4)}
+    srv = serving.Server(ctx, budget_bytes=64 << 30, batch_paths=8 << 20, chunk_paths=4 << 20)
+    for k, v in specs.items():
+        srv.add_scene(k, v)
+    build_s = time.perf_counter() - t0
+    names = list(specs)
+    n = a.serve_requests
+
+    def reqs(rate, seed):
+        return serving.make_requests(names, n, rate, seed=seed, res=(64, 1024), spp=(4, 64), depth=(2, 6))
+
+    serving.run_plain(ctx, srv.scenes, reqs(0, 7)[:4], realtime=False)
+    srv.run(reqs(0, 7)[:4], realtime=False)
+    out = dict(scenes={k: v.num_triangles() for k, v in specs.items()}, scene_build_s=build_s)
+    for mode in ("plain", "served"):
+        rs = reqs(0, 2024)
+        t1 = time.perf_counter()
+        if mode == "plain":
+            serving.run_plain(ctx, srv.scenes, rs, realtime=False)
+        else:
+            srv.run(rs, realtime=False)
+        wall = time.perf_counter() - t1
+        paths = sum(r.paths for r in rs)
+        out[mode] = dict(mpaths_per_s=paths / wall / 1e6, wall_s=wall, requests=n, paths=paths)
+    cap = n / out["plain"]["wall_s"]
+    out["plain_capacity_req_per_s"] = cap
+    out["served_over_plain"] = out["served"]["mpaths_per_s"] / out["plain"]["mpaths_per_s"]
+    sweep = []
+    for load in (0.25, 0.5, 0.75, 1.0, 1.25):
+        row = dict(load=load, rate_per_s=load * cap)
+        for mode in ("plain", "served"):
+            rs = reqs(load * cap, 2025)
+            t1 = time.perf_counter()
+            if mode == "plain":
+                serving.run_plain(ctx, srv.scenes, rs, realtime=True)
+            else:
+                srv.run(rs, realtime=True)
+            wall = time.perf_counter() - t1
+            lat = serving.percentiles([r.latency * 1e3 for r in rs])
+            row[mode] = dict(p50_ms=lat["p50"], p99_ms=lat["p99"],
+                             mpaths_per_s=sum(r.paths for r in rs) / wall / 1e6)
+        sweep.append(row)
+    out["latency_sweep"] = sweep
+    out["workload"] = (f"{n} requests per run over scenes {', '.join(f'{k}={v}' for k, v in out['scenes'].items())} "
+                       f"tris; res U(64..1024), spp U(4..64), depth U(2..6), numpy PCG64 seed 2024; HBM budget 64 GiB")
+    for ds in srv.scenes.values():
+        ds.close()
+    return out
+
+
+def multiview_comparison(ctx, a):
+    """Multi-view workload: views on a seeded Fibonacci sphere of radius 3
+    around a 200,000-triangle displaced geodesic sphere with a 1024^2 Perlin
+    albedo texture; resolutions uniform over 128..512, spp from {8, 16, 32},
+    depth 4.  Tuned prepared batches captured in a graph vs the plain path
+    (one forward + adjoint launch sequence per view, stream-ordered)."""
+    import torch
+    from tensorray_amd import _capi, multiview, scenes
+    t0 = time.perf_counter()
+    spec = scenes.multiview_scene(freq=100, amplitude=0.05, noise_seed=1, texture_res=1024, texture_seed=7)
+    ds = _capi.DeviceScene(ctx, spec)
+    build_s = time.perf_counter() - t0
+    cams = scenes.fibonacci_cameras(a.multiview_views, radius=3.0, seed=0)
+    views = multiview.random_views(cams, seed=0, res=(128, 512), spp_choices=(8, 16, 32), depth=4)
+    mv = multiview.MultiView(ctx, ds, views)
+    P = mv.total_paths
+    tiles = mv.tune((64, 128, 256, 512, 1024), reps=2)
+    mv.capture()
+    stream = mv.stream
+    for _ in range(2):
+        mv.replay()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(a.steps):
+        mv.replay()
+    torch.cuda.synchronize()
+    graph_ms = (time.perf_counter() - t1) * 1e3 / a.steps
+
+    def plain_step():
+        mv.clear_grads()
+        for k, v in enumerate(views):
+            cam, opt = mv.cams[k], mv.opts[k]
+            ds.render_fwd_ckpt(cam, opt, mv.film[k].data_ptr(), mv.ckpt[k].data_ptr(), mv.ckpt[k].numel(), stream)
+            mv.dldi[k].copy_(mv.film[k])
+            ds.render_adj_ckpt(cam, opt, mv.ckpt[k].data_ptr(), mv.ckpt[k].numel(), mv.dldi[k].data_ptr(),
+                               mv.dpos.data_ptr(), mv.dmat.data_ptr(), mv.dtex.data_ptr(), stream=stream)
+
+    plain_step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(a.steps):
+        plain_step()
+    torch.cuda.synchronize()
+    plain_ms = (time.perf_counter() - t1) * 1e3 / a.steps
+    out = dict(views=len(views), paths=P, triangles=ds.info().num_triangles, scene_build_s=build_s,
+               tile_fwd=tiles[0], tile_adj=tiles[1],
+               graph=dict(ms_per_step=graph_ms, mpaths_per_s=P / graph_ms / 1e3),
+               plain=dict(ms_per_step=plain_ms, mpaths_per_s=P / plain_ms / 1e3),
+               graph_over_plain=plain_ms / graph_ms,
+               workload=f"{len(views)} views (Fibonacci sphere r=3, seed 0), res U(128..512), spp {{8,16,32}}, "
+                        f"depth 4, 200,000-tri displaced geodesic sphere, 1024^2 Perlin albedo; forward + adjoint "
+                        f"(checkpoint) of all views per step")
+    mv.close()
+    ds.close()
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -274,6 +398,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     served = served_comparison(ctx, a) if a.serve_requests > 0 else None
+    multi = multiview_comparison(ctx, a) if a.multiview_views > 0 else None
+    sweep = served_sweep(ctx, a) if a.serve_sweep else None
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(spec, cam_d, W, H, B, spp, a.cpu_spp, a.cpu_runs)
@@ -297,7 +423,10 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "served": served,
+        "multiview": multi,
     }
+    if sweep is not None:
+        out["served_sweep"] = sweep
     print(json.dumps(out))
     scene.close()
     tscene.close()

@@ -151,6 +151,12 @@ trc_status trc_render_paths(trc_scene* scene, const trc_camera* cam, const trc_o
  * (deterministic); the reduction that follows trc_render_paths. */
 trc_status trc_film_from_paths(trc_ctx* ctx, const float* paths, int32_t spp, int32_t wh, float* film,
                                trc_stream stream);
+/* Adam step on n device floats (Adam::Step, Optimizer.h:95-122 of the
+ * reference's C++ examples): m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;
+ * x -= lr (m / (1-b1^step)) / (sqrt(v / (1-b2^step)) + eps).  step counts
+ * from 1; m and v start zeroed; non-finite entries of grad count as zero. */
+trc_status trc_adam_step(trc_ctx* ctx, float* x, const float* grad, float* m, float* v, uint64_t n, int32_t step,
+                         float lr, float beta1, float beta2, float eps, trc_stream stream);
 /* Diagnostic: work counters of the forward paths, counters[4] = (paths,
  * ray queries, BVH node visits, triangle tests).  Synchronous. */
 trc_status trc_render_stats(trc_scene* scene, const trc_camera* cam, const trc_options* opt, uint64_t* counters,
@@ -210,8 +216,11 @@ trc_status trc_render_batch(trc_ctx* ctx, const trc_job* jobs, int32_t n, int32_
  * uploads the job table once and allocates what the launch needs
  * (privatized gradient copies), so that trc_plan_launch only launches
  * kernels -- no allocation, no host synchronization -- and may be captured
- * into a hipGraph and replayed.  The jobs' buffers must stay valid for the
- * plan's lifetime.  No counterpart in the reference, which renders views one
+ * into a hipGraph and replayed.  The jobs' buffers and scenes must stay
+ * valid for the plan's lifetime.  The plan snapshots its scenes' device
+ * state: after trc_scene_set_vertices / trc_scene_set_material /
+ * trc_scene_configure on one of them, launching or tuning fails with
+ * TRC_ERR_STATE until the plan is created again.  No counterpart in the reference, which renders views one
  * at a time (renderer_multi_view.py:55-78). */
 typedef struct trc_plan trc_plan;
 trc_status trc_plan_create(trc_ctx* ctx, const trc_job* jobs, int32_t n, int32_t adjoint, trc_plan** out);

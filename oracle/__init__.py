@@ -282,3 +282,22 @@ def guide_weights(scene, seed, g_spp, g_eps):
     L.oracle_guide(scene.h, seed, g_spp, g_eps, _p(x0, ctypes.c_float), _p(x1, ctypes.c_float),
                    _p(poly, ctypes.c_float), ctypes.byref(n))
     return x0, x1, poly
+
+
+def adam_step(x, g, m, v, step, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-6):
+    """Adam::Step (LibTensorRay/Examples/InvRendering/Optimizer.h:95-122), in
+    float32 with one rounding per operation, NaN / inf gradients as zero
+    (renderer_single_view.py:82).  Returns new (x, m, v)."""
+    f = np.float32
+    x, g, m, v = (np.asarray(a, np.float32).copy() for a in (x, g, m, v))
+    g[~np.isfinite(g)] = 0.0
+    b1, b2, lr, eps = f(beta1), f(beta2), f(lr), f(eps)
+    m = b1 * m + (f(1.0) - b1) * g
+    v = b2 * v + (f(1.0) - b2) * (g * g)
+    # 1 - beta^step of the float32 beta (the reference keeps beta in a float)
+    bc1 = f(1.0 - float(b1) ** step)
+    bc2 = f(1.0 - float(b2) ** step)
+    mh = m / bc1
+    vh = v / bc2
+    x = x - (lr * mh) / (np.sqrt(vh) + eps)
+    return x.astype(np.float32), m.astype(np.float32), v.astype(np.float32)

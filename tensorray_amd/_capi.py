@@ -98,6 +98,8 @@ SIGNATURES = [
     ("trc_plan_launch", ctypes.c_int32, [vp, vp]),
     ("trc_plan_tune", ctypes.c_int32, [vp, c_u32_p, ctypes.c_int32, ctypes.c_int32, c_u32_p, vp]),
     ("trc_plan_destroy", ctypes.c_int32, [vp]),
+    ("trc_adam_step", ctypes.c_int32, [vp, vp, vp, vp, vp, ctypes.c_uint64, ctypes.c_int32, ctypes.c_float,
+                                       ctypes.c_float, ctypes.c_float, ctypes.c_float, vp]),
 ]
 
 _lib = None
@@ -161,6 +163,12 @@ class Context:
         arr = (trc_job * len(jobs))(*jobs)
         self.check(self.lib.trc_render_batch(self.h, arr, len(jobs), 1 if adjoint else 0, _stream_ptr(stream)),
                    "trc_render_batch")
+
+    def adam_step(self, x_ptr, grad_ptr, m_ptr, v_ptr, n, step, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-6,
+                  stream=None):
+        """One Adam step on n device floats (trc_adam_step)."""
+        self.check(self.lib.trc_adam_step(self.h, x_ptr, grad_ptr, m_ptr, v_ptr, int(n), int(step), lr, beta1, beta2,
+                                          eps, _stream_ptr(stream)), "trc_adam_step")
 
     def plan(self, jobs, adjoint):
         """A prepared batch (trc_plan_create) over `jobs`."""

@@ -2,6 +2,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -41,7 +43,7 @@ trc_status guarded(trc_ctx* ctx, F&& f) {
 }
 
 constexpr int kSpillDepth = 64;
-constexpr int kBlocksPerCU = 8;
+constexpr int kBlocksPerCU = 2 * kFwdWaves;  // 128-lane blocks: 2 waves each, 4 SIMDs
 
 std::map<std::pair<trc_ctx*, hipStream_t>, uint32_t*>& spill_map() {
   static std::map<std::pair<trc_ctx*, hipStream_t>, uint32_t*> m;
@@ -391,6 +393,7 @@ trc_status trc_scene_configure(trc_scene* s, trc_stream stream) {
     prepare_host(s);
     upload_scene(s, (hipStream_t)stream);
     s->configured = true;
+    ++s->version;
   });
 }
 
@@ -429,6 +432,7 @@ trc_status trc_scene_set_vertices(trc_scene* s, const float* pos, int32_t nv, in
       s->host.build_bvh();
       upload_scene(s, st);
     }
+    ++s->version;
   });
 }
 
@@ -445,6 +449,7 @@ trc_status trc_scene_set_material(trc_scene* s, int32_t mi, const float albedo[3
       m.k[k] = m.spec[k];
     }
     m.roughness = roughness;
+    ++s->version;
     if (s->configured) {
       hip_check(hipMemcpy((void*)(s->dev.mats + mi), &m, sizeof(Material), hipMemcpyHostToDevice), "hipMemcpy");
     }
@@ -612,10 +617,14 @@ static std::vector<JobDev> prepare_batch(trc_ctx* ctx, const trc_job* jobs, int3
     if (adjoint) {
       const uint64_t n3 = 3ull * (uint64_t)d.S.n_verts;
       const uint32_t k = grad_copies_for(n3);
+      // jobs that accumulate into the same dpos share one set of copies
+      bool shared = false;
+      for (int q = 0; q < i && !shared; ++q)
+        shared = jd[(size_t)q].dpos == d.dpos && jd[(size_t)q].gc.copies > 1;
       if (k > 1) {
         d.gc.copies = k;
         d.gc.n3 = (uint32_t)n3;
-        *grad_floats += n3 * k;
+        if (!shared) *grad_floats += n3 * k;
       }
     }
     *total += J.num_paths;
@@ -626,11 +635,19 @@ static std::vector<JobDev> prepare_batch(trc_ctx* ctx, const trc_job* jobs, int3
 
 static void assign_grad_copies(std::vector<JobDev>& jd, float* base) {
   uint64_t off = 0;
-  for (JobDev& d : jd)
-    if (d.gc.copies > 1) {
-      d.gc.base = base + off;
-      off += (uint64_t)d.gc.n3 * d.gc.copies;
+  for (size_t i = 0; i < jd.size(); ++i) {
+    JobDev& d = jd[i];
+    if (d.gc.copies <= 1) continue;
+    const JobDev* owner = nullptr;
+    for (size_t q = 0; q < i && !owner; ++q)
+      if (jd[q].dpos == d.dpos && jd[q].gc.base) owner = &jd[q];
+    if (owner) {
+      d.gc.base = owner->gc.base;
+      continue;
     }
+    d.gc.base = base + off;
+    off += (uint64_t)d.gc.n3 * d.gc.copies;
+  }
 }
 
 // Default tile: about four tiles per resident wave (waves take tiles
@@ -704,6 +721,8 @@ struct trc_plan {
   uint32_t* spill = nullptr;   // the plan's own traversal spill buffer
   LaunchCfg L{};
   uint64_t total = 0, tile = 0, ntiles = 0;
+  // the scenes the job table snapshots, with their versions at creation
+  std::vector<std::pair<const trc_scene*, uint64_t>> scenes;
 };
 
 namespace {
@@ -735,6 +754,11 @@ trc_status trc_plan_create(trc_ctx* ctx, const trc_job* jobs, int32_t n, int32_t
     const trc_scene* deepest;
     uint64_t gfloats;
     p->jd = prepare_batch(ctx, jobs, n, p->adjoint, &deepest, &p->total, &gfloats);
+    for (int i = 0; i < n; ++i) {
+      bool seen = false;
+      for (auto& sv : p->scenes) seen = seen || sv.first == jobs[i].scene;
+      if (!seen) p->scenes.emplace_back(jobs[i].scene, jobs[i].scene->version);
+    }
     try {
       hip_check(hipSetDevice(ctx->device), "hipSetDevice");
       hip_check(hipMalloc((void**)&p->dev_jobs, p->jd.size() * sizeof(JobDev)), "hipMalloc(plan jobs)");
@@ -770,9 +794,16 @@ trc_status trc_plan_set_tile(trc_plan* plan, uint32_t tile_paths) {
 
 uint32_t trc_plan_tile(const trc_plan* plan) { return plan ? (uint32_t)plan->tile : 0u; }
 
+static void plan_check_scenes(const trc_plan* p) {
+  for (auto& sv : p->scenes)
+    if (sv.first->version != sv.second)
+      throw Fail(TRC_ERR_STATE, "a scene of the plan changed since trc_plan_create: create the plan again");
+}
+
 trc_status trc_plan_launch(trc_plan* plan, trc_stream stream) {
   if (!plan) return TRC_ERR_INVALID;
   return guarded(plan->ctx, [&] {
+    plan_check_scenes(plan);
     hip_check(launch_render_batch(plan->dev_jobs, plan->jd.data(), (int)plan->jd.size(), plan->ntiles,
                                   (uint32_t)plan->tile, plan->adjoint, plan->L, (hipStream_t)stream),
               plan->adjoint ? "k_render_batch_adj" : "k_render_batch_fwd");
@@ -784,6 +815,7 @@ trc_status trc_plan_tune(trc_plan* plan, const uint32_t* candidates, int32_t n, 
   if (!plan) return TRC_ERR_INVALID;
   return guarded(plan->ctx, [&] {
     if (n <= 0 || !candidates || reps <= 0) throw Fail(TRC_ERR_INVALID, "no candidates to tune over");
+    plan_check_scenes(plan);
     for (int i = 0; i < n; ++i)
       if (candidates[i] == 0 || candidates[i] % 64 != 0)
         throw Fail(TRC_ERR_INVALID, "candidate tiles must be positive multiples of 64");
@@ -854,6 +886,20 @@ trc_status trc_render_adj_ckpt(trc_scene* s, const trc_camera* cam, const trc_op
                                           dtex, s->vn_adj, gc, (hipStream_t)stream);
     if (gc.base) hipFreeAsync(gc.base, (hipStream_t)stream);
     hip_check(e, "k_render_adj_ckpt");
+  });
+}
+
+trc_status trc_adam_step(trc_ctx* ctx, float* x, const float* grad, float* m, float* v, uint64_t n, int32_t step,
+                         float lr, float beta1, float beta2, float eps, trc_stream stream) {
+  if (!ctx) return TRC_ERR_INVALID;
+  return guarded(ctx, [&] {
+    if (ctx->device < 0) throw Fail(TRC_ERR_STATE, "host-only context cannot run kernels");
+    if (n && (!x || !grad || !m || !v)) throw Fail(TRC_ERR_INVALID, "NULL Adam buffer");
+    if (step < 1) throw Fail(TRC_ERR_INVALID, "Adam step count starts at 1");
+    // bias corrections 1 - beta^step, evaluated in double and rounded once
+    const float bc1 = (float)(1.0 - std::pow((double)beta1, (double)step));
+    const float bc2 = (float)(1.0 - std::pow((double)beta2, (double)step));
+    hip_check(launch_adam(x, grad, m, v, n, lr, beta1, beta2, eps, bc1, bc2, (hipStream_t)stream), "k_adam");
   });
 }
 

@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(kBlock) k_intersect(SceneDev S, LaunchCfg L, c
 }
 
 template <bool kCkpt>
-__global__ void __launch_bounds__(kBlock) k_render_paths(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kFwdWaves))) k_render_paths(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
                                                          uint64_t npaths, uint32_t tile, float* paths, CkptDev ck) {
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx(lds_stack, L);
@@ -100,7 +100,7 @@ __global__ void k_film_reduce(const float* paths, int spp, int wh, float* film) 
   film[2 * wh + pix] = sz * inv;
 }
 
-__global__ void __launch_bounds__(kBlock) k_render_adj(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdjWaves))) k_render_adj(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
                                                        uint64_t npaths, const float* dLdI, float* dpos,
                                                        float* dmat, float* adj_vn, float* dtex, GradCopies gc) {
   if (gc.base) dpos = gc.base + (size_t)(blockIdx.x % gc.copies) * gc.n3;
@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(kBlock) k_render_adj(SceneDev S, CameraDev C, 
 
 // Adjoint from a path checkpoint: the path is re-evaluated from its recorded
 // hits (no traversal), then swept backwards as in k_render_adj.
-__global__ void __launch_bounds__(kBlock) k_render_adj_ckpt(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdjWaves))) k_render_adj_ckpt(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
                                                             uint64_t npaths, CkptDev ck, const float* dLdI,
                                                             float* dpos, float* dmat, float* adj_vn, float* dtex,
                                                             GradCopies gc) {
@@ -186,9 +186,11 @@ __global__ void k_grad_reduce(float* base, uint32_t copies, uint32_t n3, float* 
 }
 
 uint32_t grad_copies_for(uint64_t n3) {
-  if (n3 == 0 || n3 > (4u << 20)) return 1;
-  uint64_t k = (16u << 20) / n3;  // at most 64 MB of copies
-  return (uint32_t)(k > 64 ? 64 : (k < 1 ? 1 : k));
+  // above ~64k vertices the paths spread over enough vertices that atomics
+  // rarely collide, and the copies would cost more to clear and reduce
+  if (n3 == 0 || n3 > (192u << 10)) return 1;
+  uint64_t k = (4u << 20) / n3;  // at most 16 MB of copies
+  return (uint32_t)(k > 64 ? 64 : (k < 2 ? 1 : k));
 }
 
 hipError_t launch_fill_zero(float* p, uint64_t n, hipStream_t st) {
@@ -232,7 +234,7 @@ __device__ inline int job_of_tile(const JobDev* jobs, int njobs, uint64_t t) {
   return __builtin_amdgcn_readfirstlane(lo);
 }
 
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) k_render_batch_fwd(const JobDev* jobs, int njobs, uint64_t ntiles,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kFwdWaves))) k_render_batch_fwd(const JobDev* jobs, int njobs, uint64_t ntiles,
                                                              uint32_t tile, LaunchCfg L) {
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx(lds_stack, L);
@@ -261,7 +263,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
 
 // Material gradients are summed per wave in LDS and flushed at the end of
 // each tile (one job per tile).
-__global__ void __launch_bounds__(kBlock) k_render_batch_adj(const JobDev* jobs, int njobs, uint64_t ntiles,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdjWaves))) k_render_batch_adj(const JobDev* jobs, int njobs, uint64_t ntiles,
                                                              uint32_t tile, LaunchCfg L) {
   __shared__ float mat_lds[(kBlock / 64) * kMaxMatGrad * kMatGradStride];
   const int lane = (int)(threadIdx.x & 63u);
@@ -323,9 +325,14 @@ hipError_t launch_render_batch(const JobDev* jobs_dev, const JobDev* jobs_host, 
   hipError_t e = hipGetLastError();
   for (int j = 0; j < njobs && e == hipSuccess; ++j) {
     const JobDev& J = jobs_host[j];
-    if (J.gc.base) e = launch_grad_reduce(J.gc, J.dpos, st);
+    bool first_use = J.gc.base != nullptr;  // copies shared by jobs with one dpos are reduced once
+    for (int q = 0; q < j && first_use; ++q) first_use = jobs_host[q].gc.base != J.gc.base;
+    if (first_use) e = launch_grad_reduce(J.gc, J.dpos, st);
     if (e != hipSuccess) break;
     if (!J.vn || !J.S.vf_off || J.S.n_verts == 0) continue;
+    bool vn_first = true;  // a scratch shared by several jobs is folded once
+    for (int q = 0; q < j && vn_first; ++q) vn_first = jobs_host[q].vn != J.vn;
+    if (!vn_first) continue;
     const int grid = (J.S.n_verts + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_vertex_normal_adj, dim3(grid), dim3(kBlock), 0, st, J.S, J.vn, J.dpos);
     e = hipGetLastError();

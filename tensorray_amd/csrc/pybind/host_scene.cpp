@@ -197,6 +197,14 @@ void Camera::init(const Vector3f& p, const Vector3f& t, const Vector3f& u, int w
   height = h;
   fov = f;
 }
+void Camera::update(const Camera& c) {
+  pos = c.pos;
+  target = c.target;
+  up = c.up;
+  width = c.width;
+  height = c.height;
+  fov = c.fov;
+}
 void Camera::resize(int w, int h) {
   width = w;
   height = h;

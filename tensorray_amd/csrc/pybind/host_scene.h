@@ -114,6 +114,9 @@ struct Camera {
   Camera() = default;
   Camera(const Vector3f& p, const Vector3f& t, const Vector3f& u, int w, int h, float f);
   void init(const Vector3f& p, const Vector3f& t, const Vector3f& u, int w, int h, float f);
+  // Camera::Update (Camera.h:110-113): takes over another camera's pose,
+  // resolution and fov.
+  void update(const Camera& c);
   void resize(int w, int h);
   trc_camera to_trc() const;
   Tensorf pos_tensor() const { return Tensorf::vec3(pos[0], pos[1], pos[2], false); }

@@ -114,7 +114,7 @@ PYBIND11_MODULE(TensorRay, m) {
       .def_readonly("height", &Camera::height)
       .def_property_readonly("posTensor", &Camera::pos_tensor)
       .def("init", &Camera::init)
-      .def("update", &Camera::init)
+      .def("update", &Camera::update, "camera"_a)
       .def("resize", &Camera::resize);
 
   py::class_<Scene>(m, "Scene")

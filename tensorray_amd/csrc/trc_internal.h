@@ -42,6 +42,7 @@ struct trc_scene {
   trc_ctx* ctx = nullptr;
   trc::SceneHost host;
   bool configured = false;
+  uint64_t version = 0;           // bumped by every change of the device scene (plans check it)
   std::vector<DevBuf> bufs;       // all device allocations of this scene
   trc::SceneDev dev{};
   trc::TexDesc* dev_texs = nullptr;
@@ -63,6 +64,14 @@ namespace trc {
 
 // launch configuration shared by the megakernels
 constexpr int kBlock = 128;
+// Waves per SIMD the path kernels are compiled for (register budget:
+// 512 / waves VGPRs).  Traversal is latency-bound, so the forward trades a
+// few spilled registers for 5 resident waves (96 VGPRs); the adjoint keeps
+// 3 (168 VGPRs).  Measured on the metric workload: forward 6.39 -> 5.09 ms
+// going from 4 to 5 waves, 6.38 ms at 6; adjoint 2.15 -> 1.86 ms going from
+// 2 to 3.
+constexpr int kFwdWaves = 5;
+constexpr int kAdjWaves = 3;
 constexpr int kStackLds = 16;
 constexpr int kLeafQueue = 8;                          // postponed leaves per lane (LDS)
 constexpr int kTraceLdsWords = kStackLds + kLeafQueue;  // LDS words per lane for traversal
@@ -106,6 +115,8 @@ struct GradCopies {
 // Number of copies for a mesh of `n3` gradient floats (1 = none).
 uint32_t grad_copies_for(uint64_t n3);
 hipError_t launch_fill_zero(float* p, uint64_t n, hipStream_t st);
+hipError_t launch_adam(float* x, const float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2,
+                       float eps, float bc1, float bc2, hipStream_t st);
 hipError_t launch_grad_reduce(const GradCopies& g, float* dpos, hipStream_t st);
 
 // One job of a batched launch (device copy of trc_job).

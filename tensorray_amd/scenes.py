@@ -166,6 +166,55 @@ def icosphere(subdiv, radius=1.0, center=(0.0, 0.0, 0.0)):
     return pos.astype(np.float32), np.array(f, np.uint32)
 
 
+def geodesic_sphere(freq, radius=1.0, center=(0.0, 0.0, 0.0)):
+    """Icosahedron with every face split into freq x freq triangles, projected
+    onto the sphere: 20 * freq^2 triangles (freq 100: 200,000)."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v0 = np.array([(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t),
+                   (0, -1, -t), (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)], np.float64)
+    f0 = np.array([(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+                   (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+                   (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)])
+    n = int(freq)
+    # lattice (i, j), i + j <= n, of one face: point = A + i/n (B - A) + j/n (C - A)
+    ii, jj = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    keep = ii + jj <= n
+    ii, jj = ii[keep], jj[keep]
+    lid = -np.ones((n + 1, n + 1), np.int64)
+    lid[ii, jj] = np.arange(len(ii))
+    a, b = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    up = a + b <= n - 1                      # (i,j) (i+1,j) (i,j+1)
+    dn = a + b <= n - 2                      # (i+1,j) (i+1,j+1) (i,j+1)
+    tri_up = np.stack([lid[a[up], b[up]], lid[a[up] + 1, b[up]], lid[a[up], b[up] + 1]], 1)
+    tri_dn = np.stack([lid[a[dn] + 1, b[dn]], lid[a[dn] + 1, b[dn] + 1], lid[a[dn], b[dn] + 1]], 1)
+    local = np.concatenate([tri_up, tri_dn])
+    pts, faces = [], []
+    for k, (A, B, C) in enumerate(f0):
+        P = v0[A] + (ii[:, None] / n) * (v0[B] - v0[A]) + (jj[:, None] / n) * (v0[C] - v0[A])
+        pts.append(P / np.linalg.norm(P, axis=1, keepdims=True))
+        faces.append(local + k * len(ii))
+    P = np.concatenate(pts)
+    F = np.concatenate(faces)
+    # merge the copies of points on shared icosahedron edges
+    key = np.round(P * 1e6).astype(np.int64)
+    _, first, inv = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    order = np.argsort(first)                # keep first-seen order
+    remap = np.empty(len(order), np.int64)
+    remap[order] = np.arange(len(order))
+    pos = P[np.sort(first)] * radius + np.asarray(center, np.float64)
+    idx = remap[inv.ravel()][F]
+    return pos.astype(np.float32), idx.astype(np.uint32)
+
+
+def sphere_uv(pos, center=(0.0, 0.0, 0.0)):
+    """Longitude / latitude texture coordinates of points around `center`."""
+    p = pos.astype(np.float64) - np.asarray(center, np.float64)
+    p /= np.linalg.norm(p, axis=1, keepdims=True)
+    u = 0.5 + np.arctan2(p[:, 2], p[:, 0]) / (2.0 * np.pi)
+    v = np.arccos(np.clip(p[:, 1], -1.0, 1.0)) / np.pi
+    return np.stack([u, v], 1).astype(np.float32)
+
+
 def _value_noise(p, seed, octave):
     fl = np.floor(p)
     i = fl.astype(np.int64)
@@ -222,11 +271,15 @@ def perlin_texture(w, h, seed, octaves=4, base=8.0):
             ix, iy = fx.astype(np.int64), fy.astype(np.int64)
             tx, ty = x - fx, y - fy
 
+            # gradient angles on the lattice points the octave touches
+            nx, ny = int(fx.max()) + 2, int(fy.max()) + 2
+            lx, ly = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+            r, _, _, _ = philox4x32(lx.ravel().astype(np.uint64), ly.ravel().astype(np.uint64),
+                                    np.full(lx.size, ch), np.full(lx.size, o), seed, 0x7E47)
+            ang = ((r >> np.uint32(8)).astype(np.float64) / 16777216.0 * 2.0 * np.pi).reshape(nx, ny)
+
             def grad(gx, gy, dx, dy):
-                r, _, _, _ = philox4x32(gx.ravel() & 0xFFFFFFFF, gy.ravel() & 0xFFFFFFFF,
-                                        np.full(gx.size, ch), np.full(gx.size, o), seed, 0x7E47)
-                a = (r >> np.uint32(8)).astype(np.float64) / 16777216.0 * 2.0 * np.pi
-                a = a.reshape(gx.shape)
+                a = ang[gx, gy]
                 return np.cos(a) * dx + np.sin(a) * dy
 
             n00 = grad(ix, iy, tx, ty)
@@ -252,12 +305,14 @@ CAMERA_DEFAULT = dict(pos=(0.0, 1.0, 3.0), target=(0.0, 0.0, 0.0), up=(0.0, 1.0,
 
 def sphere_scene(subdiv=2, amplitude=0.0, noise_seed=1, material=None, diff=True, smooth=False,
                  ground_albedo=(0.5, 0.5, 0.5), light_radiance=(10.0, 10.0, 10.0), texture_res=0,
-                 texture_seed=7):
+                 texture_seed=7, freq=0):
     """Icosphere (radius 1 at the origin, optionally displaced) on a ground quad
     at y = -1.05 under a 1x1 area light at y = 2 facing down.
 
     material: Material for the sphere (default Lambertian albedo 0.5).
     texture_res > 0 adds a seeded Perlin albedo texture on the ground.
+    freq > 0 uses a geodesic sphere of 20 * freq^2 triangles instead of the
+    subdivided icosphere.
     """
     spec = SceneSpec()
     spec.materials.append(material or Material(MAT_DIFFUSE, albedo=(0.5, 0.5, 0.5)))
@@ -267,7 +322,7 @@ def sphere_scene(subdiv=2, amplitude=0.0, noise_seed=1, material=None, diff=True
         gmat.texture = 0
     spec.materials.append(gmat)
     spec.materials.append(Material(MAT_DIFFUSE, albedo=(0.0, 0.0, 0.0)))
-    pos, idx = icosphere(subdiv)
+    pos, idx = geodesic_sphere(freq) if freq else icosphere(subdiv)
     if amplitude:
         pos = displace(pos, amplitude, noise_seed)
     flags = (MESH_DIFF if diff else 0) | (MESH_SMOOTH if smooth else 0)
@@ -329,3 +384,41 @@ def metric_scene(amplitude=0.05, noise_seed=1):
     (amplitude 0.05), GGX microfacet (roughness 0.3, F0 0.04, kd 0)."""
     mat = Material(MAT_MICROFACET, albedo=(0.0, 0.0, 0.0), spec=(0.04, 0.04, 0.04), roughness=0.3)
     return sphere_scene(subdiv=5, amplitude=amplitude, noise_seed=noise_seed, material=mat)
+
+
+def fibonacci_cameras(n, radius=3.0, seed=0, target=(0.0, 0.0, 0.0), fov=45.0):
+    """n camera positions on a sphere of `radius` around `target`: a
+    Fibonacci lattice rotated by a seeded random angle (numpy PCG64(seed))."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    phase = rng.uniform(0.0, 2.0 * np.pi)
+    ga = np.pi * (3.0 - 5.0 ** 0.5)
+    out = []
+    for k in range(n):
+        y = 1.0 - 2.0 * (k + 0.5) / n
+        r = (1.0 - y * y) ** 0.5
+        th = phase + ga * k
+        d = np.array([r * np.cos(th), y, r * np.sin(th)])
+        pos = np.asarray(target, np.float64) + radius * d
+        up = (0.0, 1.0, 0.0) if abs(y) < 0.99 else (1.0, 0.0, 0.0)
+        out.append(dict(pos=tuple(float(x) for x in pos), target=tuple(float(x) for x in target), up=up,
+                        fov=float(fov)))
+    return out
+
+
+def multiview_scene(freq=100, amplitude=0.05, noise_seed=1, texture_res=1024, texture_seed=7):
+    """Geodesic sphere of 20 * freq^2 triangles (freq 100: 200,000) displaced
+    by 3-octave value noise, Lambertian with a seeded Perlin albedo texture
+    (texture_res^2 RGB float32, longitude / latitude mapped), under a 1x1
+    area light at y = 2.  No ground, so that cameras all around see it."""
+    spec = SceneSpec()
+    spec.textures.append(perlin_texture(texture_res, texture_res, texture_seed))
+    spec.materials.append(Material(MAT_DIFFUSE, albedo=(1.0, 1.0, 1.0), texture=0))
+    spec.materials.append(Material(MAT_DIFFUSE, albedo=(0.0, 0.0, 0.0)))
+    pos, idx = geodesic_sphere(freq)
+    uv = sphere_uv(pos)
+    if amplitude:
+        pos = displace(pos, amplitude, noise_seed)
+    spec.meshes.append(Mesh(pos, idx, material=0, flags=MESH_DIFF, uv=uv, uv_idx=idx.copy()))
+    l, li, _ = quad_y(2.0, -0.5, 0.5, -0.5, 0.5, up=False)
+    spec.meshes.append(Mesh(l, li, material=1, emit=(10.0, 10.0, 10.0)))
+    return spec

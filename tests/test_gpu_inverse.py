@@ -1,0 +1,64 @@
+"""Inverse-rendering loop: one step is the composition of its parts (the
+gradient is the sum of every view's interior adjoint and boundary terms,
+the positions move by the oracle's Adam step, the scene is refit to them),
+and a few steps on a small problem reduce the image loss and the vertex
+error."""
+import numpy as np
+import pytest
+
+from tensorray_amd import _capi, inverse
+
+pytestmark = pytest.mark.gpu
+
+
+def _small(**kw):
+    s = inverse.Setup(freq=4, amplitude=0.15, views=3, res=24, spp=4, depth=2, sppe=2, sppse0=2, sppse1=0, sppe0=0,
+                      guiding=True, target_spp=16, lr=5e-3, steps=1)
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+def test_step_is_composition(gpu_ctx, oracle_mod):
+    import torch
+    s = _small()
+    prob = inverse.make_problem(gpu_ctx, s)
+    x0 = prob.x.cpu().numpy().copy()
+    # the gradient of step 1, assembled by hand
+    g = torch.zeros_like(prob.x)
+    W = H = s.res
+    for k, cam in enumerate(prob.cams):
+        opt = prob._options(1, k)
+        film = torch.empty(3, W * H, device="cuda")
+        ck = torch.empty(prob.ds.checkpoint_bytes(cam, opt), dtype=torch.uint8, device="cuda")
+        prob.ds.render_fwd_ckpt(cam, opt, film.data_ptr(), ck.data_ptr(), ck.numel())
+        dl = (film - prob.targets[k]) * (2.0 / (3 * W * H))
+        prob.ds.render_adj_ckpt(cam, opt, ck.data_ptr(), ck.numel(), dl.data_ptr(), g.data_ptr())
+        prob.ds.render_boundary(cam, opt, "primary", dl.data_ptr(), g.data_ptr())
+        prob.ds.render_boundary(cam, opt, "direct", dl.data_ptr(), g.data_ptr())
+    torch.cuda.synchronize()
+    rec = prob.step()
+    assert set(rec["ms"]) >= {"forward", "loss", "adjoint", "boundary", "adam", "refit", "total"}
+    gref = g.cpu().numpy()
+    scale = np.abs(gref).max()
+    assert scale > 0
+    assert np.abs(prob.grad.cpu().numpy() - gref).max() <= 1e-4 * scale
+    z = np.zeros_like(x0)
+    x1, _, _ = oracle_mod.adam_step(x0, prob.grad.cpu().numpy(), z, z, 1, lr=s.lr)
+    assert np.array_equal(prob.x.cpu().numpy(), x1)
+    assert np.array_equal(prob.ds.vertices(), x1)   # the scene was moved (and refit) to the new positions
+    prob.close()
+
+
+def test_loop_reduces_error(gpu_ctx):
+    """The same loop run through the oracle on the CPU (same seeds) drops
+    the loss from 0.59 to 0.20 and the vertex error from 0.0609 to 0.0577
+    in 10 steps; plain Adam on raw vertex positions also moves vertices
+    tangentially, so the vertex error falls slowly and needs a small lr."""
+    s = _small(freq=3, amplitude=0.3, views=4, res=32, spp=16, sppe=4, sppse0=4, lr=2e-3, target_spp=64)
+    prob = inverse.make_problem(gpu_ctx, s)
+    e0 = prob.vertex_error()
+    recs = [prob.step() for _ in range(10)]
+    assert recs[-1]["vertex_error"] < 0.99 * e0
+    assert np.mean([r["loss"] for r in recs[-3:]]) < 0.6 * recs[0]["loss"]
+    prob.close()

@@ -182,3 +182,25 @@ def test_render_options_defaults():
     assert o.spp_batch == 16 and o.sppe_batch == 4
     assert o.g_direct is False and o.g_direct_max_size == 100000 and o.g_direct_spp == 16
     assert o.checkpoint is False
+
+
+def test_multi_view_cameras():
+    """Camera generation of the multi-view helpers (multi_view.py:41-90 of
+    the reference) and Camera.update, which copies another camera's pose,
+    resolution and fov into the scene's sensor."""
+    from tensorray_amd.pyTensorRay import multi_view
+    a = multi_view.gen_camera_positions(12, 3.0, False)
+    b = multi_view.gen_camera_positions(12, 3.0, False)
+    assert a.shape == (12, 3) and np.array_equal(a, b)          # seeded by the batch size
+    np.testing.assert_allclose(np.linalg.norm(a, axis=1), 3.0, rtol=1e-12)
+    np.testing.assert_allclose(a[:, 1], 3.0 * np.linspace(0.99, -0.99, 12), rtol=1e-12)
+    h = multi_view.gen_camera_positions(8, 2.0, True)
+    assert (h[:, 2] > 0).all()
+    cams = multi_view.gen_cameras(a, [0, 0, 0], [0, 1, 0], 45.0, [64, 48])
+    assert len(cams) == 12 and (cams[0].width, cams[0].height) == (64, 48)
+    s = TR.Camera(TR.Vector3f(0, 0, 5), TR.Vector3f(0, 0, 0), TR.Vector3f(0, 1, 0), 8, 8, 30.0)
+    s.update(cams[3])
+    assert (s.width, s.height) == (64, 48)
+    p = torch.zeros(3)
+    s.posTensor.value_to_torch(get_ptr(p))
+    np.testing.assert_allclose(p.numpy(), a[3], rtol=1e-6)

@@ -453,3 +453,33 @@ def test_texture_gradient_matches_finite_differences(oracle_mod):
             assert abs(fd - dtex[k]) <= 1e-3 * max(abs(fd), abs(dtex[k])) + 1e-7, (int(pth), int(k), fd, dtex[k])
             checked += 1
     assert checked >= 10
+
+
+def test_adam_oracle_closed_forms(oracle_mod):
+    """Adam's first step moves each coordinate by lr * g / (|g| + eps)
+    (m and v start at zero, so the bias-corrected moments are g and g^2);
+    a constant gradient keeps that step size at every later step; a zero
+    gradient leaves x unchanged; NaN gradients count as zero."""
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(5))
+    x = rng.standard_normal(1000).astype(np.float32)
+    g = (rng.standard_normal(1000) * 10.0 ** rng.uniform(-4, 2, 1000)).astype(np.float32)
+    z = np.zeros_like(x)
+    x1, m1, v1 = oracle_mod.adam_step(x, g, z, z, 1, lr=1e-2, eps=1e-6)
+    g64 = g.astype(np.float64)
+    np.testing.assert_allclose(x - x1, 1e-2 * g64 / (np.abs(g64) + 1e-6), rtol=2e-5, atol=1e-9)
+    np.testing.assert_allclose(m1, 0.1 * g64, rtol=1e-6)
+    np.testing.assert_allclose(v1, 0.001 * g64 * g64, rtol=5e-5)  # 1 - 0.999f = 0.00099999 in float32
+    # constant gradient: mh = g, vh = g^2 at every step
+    xs, ms, vs = x.copy(), z.copy(), z.copy()
+    for t in range(1, 8):
+        x_prev = xs
+        xs, ms, vs = oracle_mod.adam_step(xs, g, ms, vs, t, lr=1e-2)
+        np.testing.assert_allclose(x_prev - xs, 1e-2 * g64 / (np.abs(g64) + 1e-6), rtol=5e-4, atol=1e-8)
+    # zero and NaN gradients
+    gz = g.copy()
+    gz[::3] = 0.0
+    gz[1::3] = np.nan
+    x2, m2, v2 = oracle_mod.adam_step(x, gz, z, z, 1)
+    assert np.array_equal(x2[::3], x[::3]) and np.array_equal(x2[1::3], x[1::3])
+    assert np.isfinite(m2).all() and np.isfinite(v2).all()
