@@ -270,7 +270,7 @@ struct BoundaryParams {
 // ---------------------------------------------------------------------------
 // Primary edges (BoundaryPrimary.cpp:36-181)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_boundary_primary(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBndWaves))) k_boundary_primary(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
                                                              EdgeSet E, uint64_t N, const float* dLdI, float* dpos) {
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
@@ -311,7 +311,7 @@ __global__ void __launch_bounds__(kBlock) k_boundary_primary(SceneDev S, CameraD
 // Pixel boundary of the box filter (BoundaryPixel.cpp:36-159, Edge.cpp:377-409,
 // Camera.cpp:433-467).  4 antithetic positions per random number.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_boundary_pixel(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBndWaves))) k_boundary_pixel(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
                                                            uint64_t N, const float* dLdI, float* dpos) {
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
@@ -470,7 +470,7 @@ __global__ void __launch_bounds__(kGuideBlock) k_guide_finish(const float* sec_c
     for (int i = threadIdx.x; i < n; i += kGuideBlock) G.cdf[i] = G.cdf[i] / integral;
 }
 
-__global__ void __launch_bounds__(kBlock) k_boundary_direct(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBndWaves))) k_boundary_direct(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
                                                             GuideDev G, uint64_t N, const float* dLdI, float* dpos) {
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
@@ -542,7 +542,7 @@ __global__ void __launch_bounds__(kBlock) k_boundary_direct(SceneDev S, CameraDe
 // Secondary edges, indirect illumination (BoundaryIndirect.cpp:38-192,
 // Edge.cpp:146-199)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_boundary_indirect(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBndWaves))) k_boundary_indirect(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
                                                               uint64_t N, const float* dLdI, float* dpos) {
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
