@@ -181,9 +181,13 @@ def served_sweep(ctx, a):
              "s20k": scenes.sphere_scene(subdiv=5, amplitude=0.05, noise_seed=1),
              "s200k": scenes.sphere_scene(freq=100, amplitude=0.05, noise_seed=3),
              "s1m": scenes.sphere_scene(freq=224, amplitude=0.05, noise_seed=4)}
-    srv = serving.Server(ctx, budget_bytes=64 << 30, batch_paths=8 << 20, chunk_paths=4 << 20)
+    srvs = {"fifo": serving.Server(ctx, budget_bytes=64 << 30, batch_paths=16 << 20, chunk_paths=16 << 20),
+            "srpt": serving.Server(ctx, budget_bytes=64 << 30, batch_paths=8 << 20, chunk_paths=8 << 20,
+                                   order="srpt")}
+    srv = srvs["fifo"]
     for k, v in specs.items():
         srv.add_scene(k, v)
+    srvs["srpt"].scenes = srv.scenes
     build_s = time.perf_counter() - t0
     names = list(specs)
     n = a.serve_requests
@@ -192,31 +196,37 @@ def served_sweep(ctx, a):
         return serving.make_requests(names, n, rate, seed=seed, res=(64, 1024), spp=(4, 64), depth=(2, 6))
 
     serving.run_plain(ctx, srv.scenes, reqs(0, 7)[:4], realtime=False)
-    srv.run(reqs(0, 7)[:4], realtime=False)
-    out = dict(scenes={k: v.num_triangles() for k, v in specs.items()}, scene_build_s=build_s)
-    for mode in ("plain", "served"):
+    for sv in srvs.values():
+        sv.run(reqs(0, 7)[:4], realtime=False)
+    out = dict(scenes={k: v.num_triangles() for k, v in specs.items()}, scene_build_s=build_s,
+               policies={"served_fifo": "oldest first, 16M paths per step", "served_srpt":
+                         "shortest remaining work first, 8M paths per step"})
+    modes = ("plain", "served_fifo", "served_srpt")
+
+    def run_mode(mode, rs, realtime):
+        if mode == "plain":
+            serving.run_plain(ctx, srv.scenes, rs, realtime=realtime)
+        else:
+            srvs[mode.split("_")[1]].run(rs, realtime=realtime)
+
+    for mode in modes:
         rs = reqs(0, 2024)
         t1 = time.perf_counter()
-        if mode == "plain":
-            serving.run_plain(ctx, srv.scenes, rs, realtime=False)
-        else:
-            srv.run(rs, realtime=False)
+        run_mode(mode, rs, False)
         wall = time.perf_counter() - t1
         paths = sum(r.paths for r in rs)
         out[mode] = dict(mpaths_per_s=paths / wall / 1e6, wall_s=wall, requests=n, paths=paths)
     cap = n / out["plain"]["wall_s"]
     out["plain_capacity_req_per_s"] = cap
-    out["served_over_plain"] = out["served"]["mpaths_per_s"] / out["plain"]["mpaths_per_s"]
+    for mode in modes[1:]:
+        out[mode + "_over_plain"] = out[mode]["mpaths_per_s"] / out["plain"]["mpaths_per_s"]
     sweep = []
     for load in (0.25, 0.5, 0.75, 1.0, 1.25):
         row = dict(load=load, rate_per_s=load * cap)
-        for mode in ("plain", "served"):
+        for mode in modes:
             rs = reqs(load * cap, 2025)
             t1 = time.perf_counter()
-            if mode == "plain":
-                serving.run_plain(ctx, srv.scenes, rs, realtime=True)
-            else:
-                srv.run(rs, realtime=True)
+            run_mode(mode, rs, True)
             wall = time.perf_counter() - t1
             lat = serving.percentiles([r.latency * 1e3 for r in rs])
             row[mode] = dict(p50_ms=lat["p50"], p99_ms=lat["p99"],

