@@ -1213,12 +1213,55 @@ static std::vector<EdgeRec> prim_edges(const Scene& s, const Cam& c) {
 }
 
 // Distribution1D over edge lengths (Distribution.cpp:45-58, Distribution.cu:37-48)
+// Blocked summation order of the guide's reductions over leaves (the order
+// the device uses; the reference leaves it to its reduction library): chunks
+// of kSumChunk consecutive values are summed sequentially, then the chunk
+// sums sequentially; an inclusive prefix is the sequential prefix of the
+// preceding chunk sums plus the sequential prefix within the chunk.  Up to
+// kSumChunk values this is the plain sequential order.
+constexpr size_t kSumChunk = 1024;
+static std::vector<float> chunk_offsets(const std::vector<float>& v, float* total) {
+  const size_t nc = (v.size() + kSumChunk - 1) / kSumChunk;
+  std::vector<float> off(nc);
+  float run = 0.0f;
+  for (size_t c = 0; c < nc; ++c) {
+    float acc = 0.0f;
+    for (size_t i = c * kSumChunk; i < std::min(v.size(), (c + 1) * kSumChunk); ++i) acc = acc + v[i];
+    off[c] = run;
+    run = run + acc;
+  }
+  *total = run;
+  return off;
+}
+static float blocked_sum(const std::vector<float>& v) {
+  float total;
+  chunk_offsets(v, &total);
+  return total;
+}
+struct Blocked {};
+
 struct Dist1 {
   std::vector<float> f, cdf;
   float integral = 0.0f;
   explicit Dist1(const std::vector<float>& fn) : f(fn), cdf(fn.size()) {
     float inv = 1.0f / (float)fn.size(), run = 0.0f;
     for (size_t i = 0; i < fn.size(); ++i) { run += fn[i]; cdf[i] = run * inv; }
+    integral = cdf.back();
+    if (integral > 0.0f)
+      for (float& c : cdf) c = c / integral;
+  }
+  // The same distribution with the CDF's prefix in the blocked order.
+  Dist1(const std::vector<float>& fn, Blocked) : f(fn), cdf(fn.size()) {
+    const float inv = 1.0f / (float)fn.size();
+    float total;
+    std::vector<float> off = chunk_offsets(fn, &total);
+    for (size_t c = 0; c < off.size(); ++c) {
+      float local = 0.0f;
+      for (size_t i = c * kSumChunk; i < std::min(fn.size(), (c + 1) * kSumChunk); ++i) {
+        local = local + fn[i];
+        cdf[i] = (off[c] + local) * inv;
+      }
+    }
     integral = cdf.back();
     if (integral > 0.0f)
       for (float& c : cdf) c = c / integral;
@@ -1581,17 +1624,15 @@ static std::unique_ptr<Guide> build_guide(const Scene& s, const std::vector<Edge
       g->poly[i] = acc;
     }
   });
-  float sum = 0.0f;
-  for (size_t i = 0; i < n; ++i) sum += g->poly[i];
+  const float sum = blocked_sum(g->poly);
   std::vector<float> f(n);
   for (size_t i = 0; i < n; ++i) {
     float w = sum > 0.0f ? g->poly[i] / sum : 0.0f;
     g->poly[i] = eps + (1.0f - eps) * w;
     f[i] = g->poly[i] * ((g->x1[i] - g->x0[i]) * 1.0f * 1.0f);
   }
-  g->fsum = 0.0f;
-  for (size_t i = 0; i < n; ++i) g->fsum += f[i];
-  g->dist.reset(new Dist1(f));
+  g->fsum = blocked_sum(f);
+  g->dist.reset(new Dist1(f, Blocked{}));
   return g;
 }
 

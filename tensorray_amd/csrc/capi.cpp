@@ -1003,7 +1003,8 @@ trc_status trc_render_boundary(trc_scene* s, const trc_camera* cam, const trc_op
         throw Fail(TRC_ERR_INVALID, "guiding: " + std::to_string(n) + " secondary edges exceed g_direct_max_size " +
                                         std::to_string(opt->g_direct_max_size));
       const size_t nv = (size_t)n * (size_t)opt->g_direct_spp;
-      const size_t need = (nv + 2 * (size_t)n + 1) * sizeof(float);
+      const size_t nparts = ((size_t)n + kSumChunk - 1) / kSumChunk + 1;
+      const size_t need = (nv + 2 * (size_t)n + 1 + nparts) * sizeof(float);
       if (s->guide_buf.bytes < need) {
         hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
         if (s->guide_buf.ptr) hipFree(s->guide_buf.ptr);
@@ -1017,6 +1018,7 @@ trc_status trc_render_boundary(trc_scene* s, const trc_camera* cam, const trc_op
       guide.poly = base + nv;
       guide.cdf = base + nv + n;
       guide.fsum = base + nv + 2 * (size_t)n;
+      guide.part = base + nv + 2 * (size_t)n + 1;
       guide.n = n;
       guide.spp = opt->g_direct_spp;
       guide.eps = opt->g_eps;

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <map>
 #include <stdexcept>
+#include <thread>
 
 namespace trc {
 
@@ -185,32 +186,57 @@ inline F3 face_n(const SceneHost& s, int32_t f) {
   if (f < 0) return F3{0.0f, 0.0f, 0.0f};
   return F3{s.shade[f].n[0], s.shade[f].n[1], s.shade[f].n[2]};
 }
+
+// The edges e with keep(e), in edge order.  Large lists are filtered in
+// contiguous chunks on up to 16 threads and concatenated in chunk order, so
+// the result is the same as the sequential filter.
+template <class Keep>
+void filter_edges(const std::vector<EdgeHost>& edges, std::vector<EdgeDev>& out, Keep keep) {
+  out.clear();
+  const size_t n = edges.size();
+  unsigned nt = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+  if (n < (size_t)1 << 16) nt = 1;
+  std::vector<std::vector<EdgeDev>> part(nt);
+  auto run = [&](unsigned t) {
+    const size_t lo = n * t / nt, hi = n * (t + 1) / nt;
+    for (size_t i = lo; i < hi; ++i) {
+      const EdgeHost& e = edges[i];
+      if (keep(e)) part[t].push_back(EdgeDev{e.v0, e.v1, e.v2, e.f0, e.f1, {0, 0, 0}});
+    }
+  };
+  if (nt == 1) {
+    run(0);
+  } else {
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t) th.emplace_back(run, t);
+    for (auto& x : th) x.join();
+  }
+  size_t total = 0;
+  for (auto& p : part) total += p.size();
+  out.reserve(total);
+  for (auto& p : part) out.insert(out.end(), p.begin(), p.end());
+}
 }  // namespace
 
 void select_sec_edges(const SceneHost& s, std::vector<EdgeDev>& out) {
-  out.clear();
-  for (const EdgeHost& e : s.edges) {
-    bool keep = e.f1 < 0;
-    if (!keep) {
-      F3 n0 = face_n(s, e.f0), n1 = face_n(s, e.f1);
-      F3 p0 = vget(s.pos, e.v0), p2 = vget(s.pos, e.v2);
-      bool noncoplanar = dot(n0, n1) < 1.0f - 1e-6f;
-      bool convex = dot(n1, normalize(p2 - p0)) < -kEdgeEpsilon;
-      keep = noncoplanar && convex;
-    }
-    if (keep) out.push_back(EdgeDev{e.v0, e.v1, e.v2, e.f0, e.f1, {0, 0, 0}});
-  }
+  filter_edges(s.edges, out, [&](const EdgeHost& e) {
+    if (e.f1 < 0) return true;
+    F3 n0 = face_n(s, e.f0), n1 = face_n(s, e.f1);
+    F3 p0 = vget(s.pos, e.v0), p2 = vget(s.pos, e.v2);
+    bool noncoplanar = dot(n0, n1) < 1.0f - 1e-6f;
+    bool convex = dot(n1, normalize(p2 - p0)) < -kEdgeEpsilon;
+    return noncoplanar && convex;
+  });
 }
 
 void select_prim_edges(const SceneHost& s, const float eye[3], std::vector<EdgeDev>& out) {
-  out.clear();
   const F3 c{eye[0], eye[1], eye[2]};
-  for (const EdgeHost& e : s.edges) {
+  filter_edges(s.edges, out, [&](const EdgeHost& e) {
     F3 d = c - vget(s.pos, e.v0);
     bool ff0 = dot(d, face_n(s, e.f0)) > 0.0f;
     bool ff1 = dot(d, face_n(s, e.f1)) > 0.0f;
-    if (e.f1 < 0 || ff0 != ff1) out.push_back(EdgeDev{e.v0, e.v1, e.v2, e.f0, e.f1, {0, 0, 0}});
-  }
+    return e.f1 < 0 || ff0 != ff1;
+  });
 }
 
 float edge_distribution(const SceneHost& s, const std::vector<EdgeDev>& e, std::vector<float>& len,

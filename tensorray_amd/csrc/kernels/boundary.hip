@@ -409,65 +409,85 @@ __global__ void __launch_bounds__(kBlock) k_guide_eval(SceneDev S, LaunchCfg L, 
 
 // Guide normalization (AQ_distrb.h:166-180): poly_i = sum_k vals in sample
 // order, w = poly / sum(poly), poly = eps + (1 - eps) w, f = poly * volume,
-// then the Distribution1D CDF of f (Distribution.cpp:45-60).  The sums run
-// sequentially in index order on thread 0 over LDS-staged chunks, so the
-// result does not depend on the launch shape.
-constexpr int kGuideBlock = 1024;
-constexpr int kGuideChunk = 8192;
-__global__ void __launch_bounds__(kGuideBlock) k_guide_finish(const float* sec_cdf, GuideDev G) {
-  __shared__ float chunk[kGuideChunk];
-  __shared__ float total;
-  const int n = G.n;
-  for (int i = threadIdx.x; i < n; i += kGuideBlock) {
+// then the Distribution1D CDF of f (Distribution.cpp:45-60).  The sums over
+// leaves use the blocked order of kSumChunk, so the result does not depend on
+// the launch shape and every pass is parallel.
+__global__ void k_guide_poly(GuideDev G) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < G.n; i += stride) {
     float acc = 0.0f;
     const float* v = G.vals + (size_t)i * (size_t)G.spp;
     for (int k = 0; k < G.spp; ++k) acc = acc + v[k];
     G.poly[i] = acc;
   }
-  __syncthreads();
-  float run = 0.0f;  // thread 0 only
-  for (int c0 = 0; c0 < n; c0 += kGuideChunk) {
-    const int m = min(kGuideChunk, n - c0);
-    for (int i = threadIdx.x; i < m; i += kGuideBlock) chunk[i] = G.poly[c0 + i];
-    __syncthreads();
-    if (threadIdx.x == 0)
-      for (int i = 0; i < m; ++i) run += chunk[i];
-    __syncthreads();
+}
+
+// part[c] = sequential sum of chunk c of v[0, n).
+__global__ void k_chunk_sums(const float* v, int n, float* part) {
+  const int nc = (n + kSumChunk - 1) / kSumChunk;
+  const int stride = gridDim.x * blockDim.x;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += stride) {
+    const int lo = c * kSumChunk, hi = min(n, lo + kSumChunk);
+    float acc = 0.0f;
+    for (int i = lo; i < hi; ++i) acc = acc + v[i];
+    part[c] = acc;
   }
-  if (threadIdx.x == 0) total = run;
-  __syncthreads();
-  const float sum = total;
-  for (int i = threadIdx.x; i < n; i += kGuideBlock) {
+}
+
+// One thread: part[c] becomes the exclusive prefix of the chunk sums,
+// part[nc] their total.
+__global__ void k_chunk_scan(float* part, int nc) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  float run = 0.0f;
+  for (int c = 0; c < nc; ++c) {
+    const float s = part[c];
+    part[c] = run;
+    run = run + s;
+  }
+  part[nc] = run;
+}
+
+// f_i = (eps + (1 - eps) poly_i / total) * (leaf's x extent), stored in cdf.
+__global__ void k_guide_weights(const float* sec_cdf, GuideDev G, const float* total) {
+  const float sum = *total;
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < G.n; i += stride) {
     const float w = sum > 0.0f ? G.poly[i] / sum : 0.0f;
     const float poly = G.eps + (1.0f - G.eps) * w;
     const float x0 = i > 0 ? sec_cdf[i - 1] : 0.0f;
     G.poly[i] = poly;
     G.cdf[i] = poly * ((sec_cdf[i] - x0) * 1.0f * 1.0f);
   }
-  __syncthreads();
-  const float inv = 1.0f / (float)n;
-  run = 0.0f;
-  for (int c0 = 0; c0 < n; c0 += kGuideChunk) {
-    const int m = min(kGuideChunk, n - c0);
-    for (int i = threadIdx.x; i < m; i += kGuideBlock) chunk[i] = G.cdf[c0 + i];
-    __syncthreads();
-    if (threadIdx.x == 0)
-      for (int i = 0; i < m; ++i) {
-        run += chunk[i];
-        chunk[i] = run * inv;
-      }
-    __syncthreads();
-    for (int i = threadIdx.x; i < m; i += kGuideBlock) G.cdf[c0 + i] = chunk[i];
-    __syncthreads();
+}
+
+// cdf_i = (blocked inclusive prefix of f) / n, from the chunk offsets.
+__global__ void k_chunk_prefix(float* v, int n, const float* part, float inv) {
+  const int nc = (n + kSumChunk - 1) / kSumChunk;
+  const int stride = gridDim.x * blockDim.x;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += stride) {
+    const int lo = c * kSumChunk, hi = min(n, lo + kSumChunk);
+    float local = 0.0f;
+    const float off = part[c];
+    for (int i = lo; i < hi; ++i) {
+      local = local + v[i];
+      v[i] = (off + local) * inv;
+    }
   }
-  if (threadIdx.x == 0) {
-    total = G.cdf[n - 1];
-    G.fsum[0] = run;
-  }
-  __syncthreads();
-  const float integral = total;
-  if (integral > 0.0f)
-    for (int i = threadIdx.x; i < n; i += kGuideBlock) G.cdf[i] = G.cdf[i] / integral;
+}
+
+// fsum = total of f; cdf_i /= cdf_{n-1} (the integral) when it is positive.
+__global__ void k_guide_normalize(GuideDev G, const float* total) {
+  const float integral = G.cdf[G.n - 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) G.fsum[0] = *total;
+  if (!(integral > 0.0f)) return;
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < G.n - 1; i += stride) G.cdf[i] = G.cdf[i] / integral;
+}
+
+// The last element is divided after every other thread has read it.
+__global__ void k_guide_normalize_last(GuideDev G) {
+  const float integral = G.cdf[G.n - 1];
+  if (integral > 0.0f) G.cdf[G.n - 1] = integral / integral;
 }
 
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBndWaves))) k_boundary_direct(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
@@ -636,7 +656,19 @@ hipError_t launch_guide_build(const SceneDev& S, const LaunchCfg& L, uint32_t se
   hipLaunchKernelGGL(k_guide_eval, dim3(Lg.grid), dim3(kBlock), 0, st, S, Lg, seed, g);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_guide_finish, dim3(1), dim3(kGuideBlock), 0, st, S.sec.cdf, g);
+  const int n = g.n, nc = (n + kSumChunk - 1) / kSumChunk;
+  const int gl = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;  // grid over leaves
+  const int gc = (nc + 63) / 64;                                     // grid over chunks
+  float* total = g.part + nc;
+  hipLaunchKernelGGL(k_guide_poly, dim3(gl), dim3(256), 0, st, g);
+  hipLaunchKernelGGL(k_chunk_sums, dim3(gc), dim3(64), 0, st, g.poly, n, g.part);
+  hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(64), 0, st, g.part, nc);
+  hipLaunchKernelGGL(k_guide_weights, dim3(gl), dim3(256), 0, st, S.sec.cdf, g, total);
+  hipLaunchKernelGGL(k_chunk_sums, dim3(gc), dim3(64), 0, st, g.cdf, n, g.part);
+  hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(64), 0, st, g.part, nc);
+  hipLaunchKernelGGL(k_chunk_prefix, dim3(gc), dim3(64), 0, st, g.cdf, n, g.part, 1.0f / (float)n);
+  hipLaunchKernelGGL(k_guide_normalize, dim3(gl), dim3(256), 0, st, g, total);
+  hipLaunchKernelGGL(k_guide_normalize_last, dim3(1), dim3(64), 0, st, g);
   return hipGetLastError();
 }
 

@@ -165,11 +165,19 @@ struct GuideDev {
   float* poly;  // [n] leaf weights after the eps mix
   float* cdf;   // [n] leaf CDF (weight * volume)
   float* fsum;  // [1] sum of weight * volume
+  float* part;  // [ceil(n / kSumChunk) + 1] chunk sums / offsets of the blocked reductions
   int32_t n, spp;
   float eps;
   int32_t enabled;
 };
-// Builds the guide on `st` (two kernels); g.enabled is ignored.
+// Reductions over the guide's leaves run in a fixed blocked order: chunks of
+// kSumChunk consecutive values summed sequentially, then the chunk sums
+// sequentially; a prefix is the sequential prefix of the preceding chunk
+// sums plus the sequential prefix within the chunk.  (The reference leaves
+// the order to its reduction library.)  Up to kSumChunk values it is the
+// plain sequential order.
+constexpr int kSumChunk = 1024;
+// Builds the guide on `st`; g.enabled is ignored.
 hipError_t launch_guide_build(const SceneDev& S, const LaunchCfg& L, uint32_t seed, const GuideDev& g,
                               hipStream_t st);
 // kind: 0 primary edges, 1 pixel boundary, 2 secondary direct, 3 secondary indirect

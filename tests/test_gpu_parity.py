@@ -245,11 +245,14 @@ def test_boundary_gradients_match_oracle(gpu_ctx, oracle_mod, kind):
     ds.close()
 
 
-def test_guided_direct_matches_oracle(gpu_ctx, oracle_mod):
+@pytest.mark.parametrize("subdiv", [2, 4])
+def test_guided_direct_matches_oracle(gpu_ctx, oracle_mod, subdiv):
     """Guided sampling of the direct boundary term (AQ_distrb.h): guide build
-    and guided samples against the oracle."""
+    and guided samples against the oracle; subdivision 4 has several
+    thousand secondary edges, so the guide's blocked reductions span
+    several chunks."""
     torch = _torch()
-    spec = scenes.sphere_scene(subdiv=2)
+    spec = scenes.sphere_scene(subdiv=subdiv)
     W = H = 48
     ds = _capi.DeviceScene(gpu_ctx, spec)
     info = ds.info()
