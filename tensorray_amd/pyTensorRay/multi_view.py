@@ -50,9 +50,9 @@ def gen_camera_positions(batch, radius, on_hemisphere):
 
 def render_save_multi_views(scene, cameras, options, integrator, out_dir):
     """Renders every camera through scene.cameras[0] and saves
-    sensor_<i>.pfm images (the reference writes .exr, which needs OpenCV)."""
+    sensor_<i>.exr images."""
     for i, camera in enumerate(cameras):
         scene.cameras[0].update(camera)
         img_tensor = integrator.renderC(scene, options)
         img = image_tensor_to_torch(img_tensor, scene.get_height(0), scene.get_width(0))
-        save_torch_image(os.path.join(out_dir, "sensor_{:d}.pfm".format(i)), img)
+        save_torch_image(os.path.join(out_dir, "sensor_{:d}.exr".format(i)), img)

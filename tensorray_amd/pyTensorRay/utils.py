@@ -3,6 +3,8 @@
 A TensorRay image is Shape([W*H], Vec3) stored as three planes [3][H*W];
 torch images are (H, W, 3).
 """
+import struct
+
 import numpy as np
 import torch
 
@@ -51,9 +53,94 @@ def save_pfm(filename, img):
         f.write(a.tobytes())
 
 
+def _exr_attr(name, typ, payload):
+    return name.encode() + b"\0" + typ.encode() + b"\0" + struct.pack("<i", len(payload)) + payload
+
+
+def save_exr(filename, img):
+    """(H, W, 3) image as an OpenEXR file the way the reference's SaveEXR
+    (Utils.cpp:299-369) stores it: single-part scanline image, channels B, G,
+    R as 16-bit half floats, no compression, increasing y.  Layout per the
+    OpenEXR file-format specification: magic 20000630, version 2, the header
+    attributes, one 64-bit offset per scanline, then per scanline its y, its
+    byte count and the channels' rows in channel-list order."""
+    a = np.asarray(img.detach().cpu() if torch.is_tensor(img) else img, np.float32)
+    h, w = a.shape[0], a.shape[1]
+    chans = b""
+    for name in (b"B", b"G", b"R"):  # alphabetical, as the format requires
+        chans += name + b"\0" + struct.pack("<iBBBBii", 1, 0, 0, 0, 0, 1, 1)
+    chans += b"\0"
+    box = struct.pack("<iiii", 0, 0, w - 1, h - 1)
+    header = (_exr_attr("channels", "chlist", chans) + _exr_attr("compression", "compression", b"\0")
+              + _exr_attr("dataWindow", "box2i", box) + _exr_attr("displayWindow", "box2i", box)
+              + _exr_attr("lineOrder", "lineOrder", b"\0")
+              + _exr_attr("pixelAspectRatio", "float", struct.pack("<f", 1.0))
+              + _exr_attr("screenWindowCenter", "v2f", struct.pack("<ff", 0.0, 0.0))
+              + _exr_attr("screenWindowWidth", "float", struct.pack("<f", 1.0)) + b"\0")
+    pre = struct.pack("<I", 20000630) + struct.pack("<I", 2) + header
+    half = a.astype("<f2")
+    row_bytes = 3 * 2 * w
+    start = len(pre) + 8 * h
+    offsets = struct.pack("<%dQ" % h, *[start + y * (8 + row_bytes) for y in range(h)])
+    with open(filename, "wb") as f:
+        f.write(pre)
+        f.write(offsets)
+        for y in range(h):
+            f.write(struct.pack("<ii", y, row_bytes))
+            f.write(half[y, :, 2].tobytes() + half[y, :, 1].tobytes() + half[y, :, 0].tobytes())
+
+
+def load_exr(filename):
+    """Reads what save_exr writes (uncompressed scanline, half or float
+    channels) into an (H, W, C) float32 array, channels in R, G, B order when
+    present, else in channel-list order."""
+    data = open(filename, "rb").read()
+    magic, version = struct.unpack_from("<II", data, 0)
+    if magic != 20000630:
+        raise ValueError("not an OpenEXR file")
+    if version & 0xFF != 2 or version & 0x1E00:
+        raise ValueError("only single-part scanline images are read")
+    pos, attrs = 8, {}
+    while data[pos] != 0:
+        end = data.index(b"\0", pos)
+        name = data[pos:end].decode()
+        tend = data.index(b"\0", end + 1)
+        typ = data[end + 1:tend].decode()
+        (size,) = struct.unpack_from("<i", data, tend + 1)
+        attrs[name] = (typ, data[tend + 5:tend + 5 + size])
+        pos = tend + 5 + size
+    pos += 1
+    if attrs["compression"][1] != b"\0":
+        raise ValueError("only uncompressed images are read")
+    x0, y0, x1, y1 = struct.unpack("<iiii", attrs["dataWindow"][1])
+    w, h = x1 - x0 + 1, y1 - y0 + 1
+    cl, chans, q = attrs["channels"][1], [], 0
+    while cl[q] != 0:
+        e = cl.index(b"\0", q)
+        (pt,) = struct.unpack_from("<i", cl, e + 1)
+        chans.append((cl[q:e].decode(), pt))
+        q = e + 17
+    offsets = struct.unpack_from("<%dQ" % h, data, pos)
+    out = np.zeros((h, w, len(chans)), np.float32)
+    for off in offsets:
+        y, nbytes = struct.unpack_from("<ii", data, off)
+        p2 = off + 8
+        for c, (_, pt) in enumerate(chans):
+            dt, sz = ("<f2", 2) if pt == 1 else ("<f4", 4)
+            out[y - y0, :, c] = np.frombuffer(data, dt, w, p2)
+            p2 += sz * w
+    names = [c[0] for c in chans]
+    if all(n in names for n in ("R", "G", "B")):
+        out = out[:, :, [names.index("R"), names.index("G"), names.index("B")]]
+    return out
+
+
 def save_torch_image(filename, img):
-    """(H, W, 3) image: .pfm (float) or .ppm (gamma 2.2, 8-bit)."""
-    if filename.endswith(".pfm"):
+    """(H, W, 3) image: .exr (half float, as the reference), .pfm (float) or
+    .ppm (gamma 2.2, 8-bit)."""
+    if filename.endswith(".exr"):
+        save_exr(filename, img)
+    elif filename.endswith(".pfm"):
         save_pfm(filename, img)
     elif filename.endswith(".ppm"):
         a = np.asarray(img.detach().cpu() if torch.is_tensor(img) else img, np.float32)
@@ -62,4 +149,4 @@ def save_torch_image(filename, img):
             f.write(b"P6\n%d %d\n255\n" % (a.shape[1], a.shape[0]))
             f.write(a.tobytes())
     else:
-        raise ValueError("supported image formats: .pfm, .ppm")
+        raise ValueError("supported image formats: .exr, .pfm, .ppm")

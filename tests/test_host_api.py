@@ -204,3 +204,31 @@ def test_multi_view_cameras():
     p = torch.zeros(3)
     s.posTensor.value_to_torch(get_ptr(p))
     np.testing.assert_allclose(p.numpy(), a[3], rtol=1e-6)
+
+
+def test_exr_writer(tmp_path):
+    """OpenEXR output as the reference's SaveEXR stores it (B, G, R halves,
+    uncompressed scanlines): fixed header bytes from the file-format
+    specification, one offset per scanline pointing at that scanline's y,
+    and pixel values equal to their IEEE half rounding."""
+    import struct
+    from tensorray_amd.pyTensorRay import utils
+    rng = np.random.default_rng(3)
+    img = (rng.uniform(0, 4, (5, 7, 3)) * [1.0, 0.5, 0.25]).astype(np.float32)
+    f = str(tmp_path / "a.exr")
+    utils.save_torch_image(f, torch.from_numpy(img))
+    data = open(f, "rb").read()
+    assert data[:8] == bytes([0x76, 0x2F, 0x31, 0x01, 2, 0, 0, 0])
+    chl = data.index(b"channels\0chlist\0")
+    (size,) = struct.unpack_from("<i", data, chl + 16)
+    assert size == 3 * (2 + 16) + 1
+    assert data[chl + 20:chl + 22] == b"B\0" and struct.unpack_from("<i", data, chl + 22)[0] == 1  # HALF
+    assert b"compression\0compression\0\x01\0\0\0\0" in data
+    hdr_end = data.index(b"screenWindowWidth\0float\0") + len(b"screenWindowWidth\0float\0") + 8 + 1
+    offs = struct.unpack_from("<5Q", data, hdr_end)
+    for y, off in enumerate(offs):
+        assert struct.unpack_from("<ii", data, off) == (y, 3 * 2 * 7)
+    assert len(data) == offs[-1] + 8 + 3 * 2 * 7
+    back = utils.load_exr(f)
+    assert back.shape == (5, 7, 3)
+    assert np.array_equal(back, img.astype(np.float16).astype(np.float32))
