@@ -300,8 +300,8 @@ def multiview_comparison(ctx, a):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): every box of this build has a single "
-              f"MI355X; see the sharding paragraph of DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): the integrator renders on a single "
+              f"MI355X; sharding paths or views over several GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
     import torch
     from tensorray_amd import _capi, scenes

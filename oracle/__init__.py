@@ -5,8 +5,8 @@ reference integrator with its own reverse-mode AD.  Only tests/,
 __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this
 package; the product (tensorray_amd) never does.
 
-Parity status of each oracle function (what pins it) is listed in
-tests/test_oracle.py and DESIGN.md.
+What pins each oracle function is stated in the docstrings of
+tests/test_oracle.py.
 """
 import ctypes
 import os
