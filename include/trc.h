@@ -157,6 +157,29 @@ trc_status trc_film_from_paths(trc_ctx* ctx, const float* paths, int32_t spp, in
  * from 1; m and v start zeroed; non-finite entries of grad count as zero. */
 trc_status trc_adam_step(trc_ctx* ctx, float* x, const float* grad, float* m, float* v, uint64_t n, int32_t step,
                          float lr, float beta1, float beta2, float eps, trc_stream stream);
+/* Uniform Adam step, the update of the reference's LargeSteps optimizer
+ * (pyTensorRay/optimizer.py:40-63; AdamUniform of Nicolet et al. 2021,
+ * "Large Steps in Inverse Rendering of Geometry"): moments as in
+ * trc_adam_step, then x -= lr / ((1-b1^step) sqrt(max(v) / (1-b2^step))) m,
+ * one scale for every coordinate (no update while max(v) = 0). */
+trc_status trc_adam_uniform_step(trc_ctx* ctx, float* x, const float* grad, float* m, float* v, uint64_t n,
+                                 int32_t step, float lr, float beta1, float beta2, trc_stream stream);
+/* Solver for M = I + lambda L, L the uniform graph Laplacian of a triangle
+ * mesh (L_ii = number of neighbours, L_ij = -1 per mesh edge) -- the
+ * matrix of largesteps' compute_matrix used by LargeSteps
+ * (pyTensorRay/optimizer.py:147-158).  Vectors are [nv][3] device floats;
+ * solve runs conjugate gradients on the three columns (x holds the initial
+ * guess) until |r| <= tol |b| per column or max_iter iterations, without
+ * host synchronization; status reads the iteration count and relative
+ * residuals back (synchronous).  Solves on one solver must not overlap. */
+typedef struct trc_lsolver trc_lsolver;
+trc_status trc_lsolver_create(trc_ctx* ctx, const uint32_t* faces /* host [nf][3] */, int32_t nf, int32_t nv,
+                              float lambda, trc_lsolver** out);
+trc_status trc_lsolver_apply(trc_lsolver* s, const float* x, float* y, trc_stream stream);  /* y = M x */
+trc_status trc_lsolver_solve(trc_lsolver* s, const float* b, float* x, int32_t max_iter, float tol,
+                             trc_stream stream);
+trc_status trc_lsolver_status(trc_lsolver* s, int32_t* iters, float rel_residual[3]);
+trc_status trc_lsolver_destroy(trc_lsolver* s);
 /* Diagnostic: work counters of the forward paths, counters[4] = (paths,
  * ray queries, BVH node visits, triangle tests).  Synchronous. */
 trc_status trc_render_stats(trc_scene* scene, const trc_camera* cam, const trc_options* opt, uint64_t* counters,

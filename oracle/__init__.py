@@ -301,3 +301,39 @@ def adam_step(x, g, m, v, step, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-6):
     vh = v / bc2
     x = x - (lr * mh) / (np.sqrt(vh) + eps)
     return x.astype(np.float32), m.astype(np.float32), v.astype(np.float32)
+
+
+def adam_uniform_step(x, g, m, v, step, lr=1e-3, beta1=0.9, beta2=0.999):
+    """Uniform Adam (pyTensorRay/optimizer.py:57-61, without its Laplacian
+    solve): moments as adam_step, then x -= lr / ((1 - b1^t) sqrt(max(v) /
+    (1 - b2^t))) m in float32, no update while max(v) = 0."""
+    f = np.float32
+    x, g, m, v = (np.asarray(a, np.float32).copy() for a in (x, g, m, v))
+    g[~np.isfinite(g)] = 0.0
+    b1, b2, lr = f(beta1), f(beta2), f(lr)
+    m = b1 * m + (f(1.0) - b1) * g
+    v = b2 * v + (f(1.0) - b2) * (g * g)
+    bc1 = f(1.0 - float(b1) ** step)
+    bc2 = f(1.0 - float(b2) ** step)
+    vmax = v.max() if v.size else f(0.0)
+    if vmax > 0:
+        scale = lr / (bc1 * np.sqrt(vmax / bc2))
+        x = x - scale * m
+    return x.astype(np.float32), m.astype(np.float32), v.astype(np.float32)
+
+
+def laplacian_system(faces, nv, lam):
+    """M = I + lam * L as a scipy CSR matrix (float64), L the uniform graph
+    Laplacian of the triangle mesh: L_ii = number of neighbours of i, L_ij =
+    -1 for every mesh edge (compute_matrix of the largesteps package used by
+    pyTensorRay/optimizer.py:152)."""
+    import scipy.sparse as sp
+    f = np.asarray(faces, np.int64).reshape(-1, 3)
+    e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+    e = np.concatenate([e, e[:, ::-1]])
+    e = e[e[:, 0] != e[:, 1]]
+    e = np.unique(e, axis=0)
+    A = sp.csr_matrix((np.ones(len(e)), (e[:, 0], e[:, 1])), shape=(nv, nv))
+    deg = np.asarray(A.sum(axis=1)).ravel()
+    L = sp.diags(deg) - A
+    return (sp.identity(nv) + lam * L).tocsr()

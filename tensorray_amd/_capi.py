@@ -100,6 +100,14 @@ SIGNATURES = [
     ("trc_plan_destroy", ctypes.c_int32, [vp]),
     ("trc_adam_step", ctypes.c_int32, [vp, vp, vp, vp, vp, ctypes.c_uint64, ctypes.c_int32, ctypes.c_float,
                                        ctypes.c_float, ctypes.c_float, ctypes.c_float, vp]),
+    ("trc_adam_uniform_step", ctypes.c_int32, [vp, vp, vp, vp, vp, ctypes.c_uint64, ctypes.c_int32, ctypes.c_float,
+                                               ctypes.c_float, ctypes.c_float, vp]),
+    ("trc_lsolver_create", ctypes.c_int32, [vp, c_u32_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                            ctypes.POINTER(vp)]),
+    ("trc_lsolver_apply", ctypes.c_int32, [vp, vp, vp, vp]),
+    ("trc_lsolver_solve", ctypes.c_int32, [vp, vp, vp, ctypes.c_int32, ctypes.c_float, vp]),
+    ("trc_lsolver_status", ctypes.c_int32, [vp, ctypes.POINTER(ctypes.c_int32), c_float_p]),
+    ("trc_lsolver_destroy", ctypes.c_int32, [vp]),
 ]
 
 _lib = None
@@ -170,6 +178,16 @@ class Context:
         self.check(self.lib.trc_adam_step(self.h, x_ptr, grad_ptr, m_ptr, v_ptr, int(n), int(step), lr, beta1, beta2,
                                           eps, _stream_ptr(stream)), "trc_adam_step")
 
+    def adam_uniform_step(self, x_ptr, grad_ptr, m_ptr, v_ptr, n, step, lr=1e-3, beta1=0.9, beta2=0.999,
+                          stream=None):
+        """One uniform-Adam step on n device floats (trc_adam_uniform_step)."""
+        self.check(self.lib.trc_adam_uniform_step(self.h, x_ptr, grad_ptr, m_ptr, v_ptr, int(n), int(step), lr, beta1,
+                                                  beta2, _stream_ptr(stream)), "trc_adam_uniform_step")
+
+    def lsolver(self, faces, nv, lam):
+        """Solver for I + lam * L of the mesh (trc_lsolver_create)."""
+        return LSolver(self, faces, nv, lam)
+
     def plan(self, jobs, adjoint):
         """A prepared batch (trc_plan_create) over `jobs`."""
         return Plan(self, jobs, adjoint)
@@ -178,6 +196,45 @@ class Context:
         if self.h is not None and self.h.value:
             self.lib.trc_context_destroy(self.h)
             self.h = None
+
+
+class LSolver:
+    """M = I + lam * L (uniform Laplacian of a triangle mesh) on the device:
+    apply (y = M x) and conjugate-gradient solve, on [nv][3] vectors."""
+
+    def __init__(self, ctx, faces, nv, lam):
+        self.ctx = ctx
+        f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
+        h = vp()
+        ctx.check(ctx.lib.trc_lsolver_create(ctx.h, _up(f), len(f), int(nv), float(lam), ctypes.byref(h)),
+                  "trc_lsolver_create")
+        self.h = h
+        self.nv = int(nv)
+
+    def apply(self, x_ptr, y_ptr, stream=None):
+        self.ctx.check(self.ctx.lib.trc_lsolver_apply(self.h, x_ptr, y_ptr, _stream_ptr(stream)), "trc_lsolver_apply")
+
+    def solve(self, b_ptr, x_ptr, max_iter=200, tol=1e-6, stream=None):
+        self.ctx.check(self.ctx.lib.trc_lsolver_solve(self.h, b_ptr, x_ptr, int(max_iter), float(tol),
+                                                      _stream_ptr(stream)), "trc_lsolver_solve")
+
+    def status(self):
+        """(iterations, relative residual per column) of the last solve."""
+        it = ctypes.c_int32(0)
+        rr = np.zeros(3, np.float32)
+        self.ctx.check(self.ctx.lib.trc_lsolver_status(self.h, ctypes.byref(it), _fp(rr)), "trc_lsolver_status")
+        return int(it.value), rr
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            self.ctx.lib.trc_lsolver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Plan:

@@ -924,6 +924,153 @@ trc_status trc_adam_step(trc_ctx* ctx, float* x, const float* grad, float* m, fl
   });
 }
 
+trc_status trc_adam_uniform_step(trc_ctx* ctx, float* x, const float* grad, float* m, float* v, uint64_t n,
+                                 int32_t step, float lr, float beta1, float beta2, trc_stream stream) {
+  if (!ctx) return TRC_ERR_INVALID;
+  return guarded(ctx, [&] {
+    if (ctx->device < 0) throw Fail(TRC_ERR_STATE, "host-only context cannot run kernels");
+    if (n && (!x || !grad || !m || !v)) throw Fail(TRC_ERR_INVALID, "NULL Adam buffer");
+    if (step < 1) throw Fail(TRC_ERR_INVALID, "Adam step count starts at 1");
+    hipStream_t st = (hipStream_t)stream;
+    const float bc1 = (float)(1.0 - std::pow((double)beta1, (double)step));
+    const float bc2 = (float)(1.0 - std::pow((double)beta2, (double)step));
+    unsigned int* vmax = nullptr;
+    hip_check(hipMallocAsync((void**)&vmax, sizeof(unsigned int), st), "hipMallocAsync");
+    hipError_t e = launch_adam_uniform(x, grad, m, v, n, lr, beta1, beta2, bc1, bc2, vmax, st);
+    hipFreeAsync(vmax, st);
+    hip_check(e, "k_adam_uniform");
+  });
+}
+
+// ---- Laplacian solver ----
+struct trc_lsolver {
+  trc_ctx* ctx = nullptr;
+  int n = 0;
+  int* row = nullptr;
+  int* col = nullptr;
+  float* val = nullptr;
+  float* work = nullptr;      // r, p, ap: 3 x [n][3]
+  CgScalars* scal = nullptr;
+  CsrDev dev() const { return CsrDev{n, row, col, val}; }
+};
+
+namespace {
+void lsolver_free(trc_lsolver* s) {
+  for (void* p : {(void*)s->row, (void*)s->col, (void*)s->val, (void*)s->work, (void*)s->scal})
+    if (p) hipFree(p);
+  s->row = s->col = nullptr;
+  s->val = s->work = nullptr;
+  s->scal = nullptr;
+}
+}  // namespace
+
+trc_status trc_lsolver_create(trc_ctx* ctx, const uint32_t* faces, int32_t nf, int32_t nv, float lambda,
+                              trc_lsolver** out) {
+  if (!ctx || !out) return TRC_ERR_INVALID;
+  *out = nullptr;
+  return guarded(ctx, [&] {
+    if (ctx->device < 0) throw Fail(TRC_ERR_STATE, "host-only context cannot run kernels");
+    if (nv <= 0 || nf < 0 || (nf > 0 && !faces)) throw Fail(TRC_ERR_INVALID, "bad mesh");
+    if (!(lambda >= 0.0f)) throw Fail(TRC_ERR_INVALID, "lambda must be >= 0");
+    // undirected edges of the triangles, then CSR rows with the diagonal
+    std::vector<std::pair<int, int>> e;
+    e.reserve(6 * (size_t)nf);
+    for (int32_t f = 0; f < nf; ++f)
+      for (int k = 0; k < 3; ++k) {
+        const uint32_t a = faces[3 * (size_t)f + k], b = faces[3 * (size_t)f + (k + 1) % 3];
+        if (a >= (uint32_t)nv || b >= (uint32_t)nv) throw Fail(TRC_ERR_INVALID, "face index out of range");
+        if (a == b) continue;
+        e.emplace_back((int)a, (int)b);
+        e.emplace_back((int)b, (int)a);
+      }
+    std::sort(e.begin(), e.end());
+    e.erase(std::unique(e.begin(), e.end()), e.end());
+    std::vector<int> row((size_t)nv + 1, 0), col;
+    std::vector<float> val;
+    col.reserve(e.size() + (size_t)nv);
+    val.reserve(e.size() + (size_t)nv);
+    size_t k = 0;
+    for (int i = 0; i < nv; ++i) {
+      size_t k0 = k;
+      while (k < e.size() && e[k].first == i) ++k;
+      const int deg = (int)(k - k0);
+      bool diag_done = false;
+      for (size_t q = k0; q < k; ++q) {
+        if (!diag_done && e[q].second > i) {
+          col.push_back(i);
+          val.push_back(1.0f + lambda * (float)deg);
+          diag_done = true;
+        }
+        col.push_back(e[q].second);
+        val.push_back(-lambda);
+      }
+      if (!diag_done) {
+        col.push_back(i);
+        val.push_back(1.0f + lambda * (float)deg);
+      }
+      row[(size_t)i + 1] = (int)col.size();
+    }
+    std::unique_ptr<trc_lsolver> s(new trc_lsolver);
+    s->ctx = ctx;
+    s->n = nv;
+    try {
+      hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+      hip_check(hipMalloc((void**)&s->row, row.size() * sizeof(int)), "hipMalloc(csr)");
+      hip_check(hipMalloc((void**)&s->col, col.size() * sizeof(int)), "hipMalloc(csr)");
+      hip_check(hipMalloc((void**)&s->val, val.size() * sizeof(float)), "hipMalloc(csr)");
+      hip_check(hipMalloc((void**)&s->work, 9 * (size_t)nv * sizeof(float)), "hipMalloc(cg)");
+      hip_check(hipMalloc((void**)&s->scal, sizeof(CgScalars)), "hipMalloc(cg)");
+      hip_check(hipMemcpy(s->row, row.data(), row.size() * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+      hip_check(hipMemcpy(s->col, col.data(), col.size() * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+      hip_check(hipMemcpy(s->val, val.data(), val.size() * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+    } catch (...) {
+      lsolver_free(s.get());
+      throw;
+    }
+    *out = s.release();
+  });
+}
+
+trc_status trc_lsolver_apply(trc_lsolver* s, const float* x, float* y, trc_stream stream) {
+  if (!s) return TRC_ERR_INVALID;
+  return guarded(s->ctx, [&] {
+    if (!x || !y || x == y) throw Fail(TRC_ERR_INVALID, "bad vectors");
+    hip_check(launch_csr_spmv3(s->dev(), x, y, (hipStream_t)stream), "k_csr_spmv3");
+  });
+}
+
+trc_status trc_lsolver_solve(trc_lsolver* s, const float* b, float* x, int32_t max_iter, float tol,
+                             trc_stream stream) {
+  if (!s) return TRC_ERR_INVALID;
+  return guarded(s->ctx, [&] {
+    if (!b || !x || b == x) throw Fail(TRC_ERR_INVALID, "bad vectors");
+    if (max_iter < 0 || !(tol > 0.0f && tol < 1.0f)) throw Fail(TRC_ERR_INVALID, "bad solver limits");
+    float* r = s->work;
+    float* p = r + 3 * (size_t)s->n;
+    float* ap = p + 3 * (size_t)s->n;
+    hip_check(launch_cg(s->dev(), b, x, r, p, ap, s->scal, max_iter, tol, (hipStream_t)stream), "k_cg");
+  });
+}
+
+trc_status trc_lsolver_status(trc_lsolver* s, int32_t* iters, float rel_residual[3]) {
+  if (!s) return TRC_ERR_INVALID;
+  return guarded(s->ctx, [&] {
+    CgScalars h;
+    hip_check(hipMemcpy(&h, s->scal, sizeof(h), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (iters) *iters = (int32_t)h.iters;
+    if (rel_residual)
+      for (int c = 0; c < 3; ++c) rel_residual[c] = h.rr0[c] > 0.0f ? std::sqrt(h.rr[c] / h.rr0[c]) : 0.0f;  // |r| / |b|
+  });
+}
+
+trc_status trc_lsolver_destroy(trc_lsolver* s) {
+  if (!s) return TRC_OK;
+  hipDeviceSynchronize();
+  lsolver_free(s);
+  delete s;
+  return TRC_OK;
+}
+
 trc_status trc_film_from_paths(trc_ctx* ctx, const float* paths, int32_t spp, int32_t wh, float* film,
                                trc_stream stream) {
   if (!ctx) return TRC_ERR_INVALID;

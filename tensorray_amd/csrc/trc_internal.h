@@ -121,6 +121,27 @@ uint32_t grad_copies_for(uint64_t n3);
 hipError_t launch_fill_zero(float* p, uint64_t n, hipStream_t st);
 hipError_t launch_adam(float* x, const float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2,
                        float eps, float bc1, float bc2, hipStream_t st);
+// vmax_bits: one device word of scratch
+hipError_t launch_adam_uniform(float* x, const float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2,
+                               float bc1, float bc2, unsigned int* vmax_bits, hipStream_t st);
+// Sparse symmetric matrix (CSR) and the device scalars of a 3-column
+// conjugate-gradient solve.
+struct CsrDev {
+  int n;
+  const int* row;    // [n + 1]
+  const int* col;    // [nnz]
+  const float* val;  // [nnz]
+};
+struct CgScalars {
+  float rr[3], rr_new[3], pap[3], rr0[3];  // rr0: |b|^2 per column
+  unsigned int active;  // bit c: column c not converged
+  unsigned int iters;
+};
+hipError_t launch_csr_spmv3(const CsrDev& M, const float* x, float* y, hipStream_t st);
+// M x = b for the three columns of [n][3] b, x (x: initial guess on entry);
+// r, p, ap: [n][3] work vectors; stops a column once |r| <= tol |b|.
+hipError_t launch_cg(const CsrDev& M, const float* b, float* x, float* r, float* p, float* ap, CgScalars* s,
+                     int max_iter, float tol, hipStream_t st);
 hipError_t launch_grad_reduce(const GradCopies& g, float* dpos, hipStream_t st);
 
 // One job of a batched launch (device copy of trc_job).

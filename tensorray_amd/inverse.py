@@ -47,6 +47,8 @@ class Setup:
     target_spp: int = 256
     lr: float = 1e-3
     steps: int = 100
+    optimizer: str = "adam"         # "adam" on the positions, or "largesteps" (uniform Adam on u = (I + lmbda L) x)
+    lmbda: float = 20.0             # largesteps: weight of the uniform Laplacian
 
 
 def shape_scene(pos, idx):
@@ -98,9 +100,22 @@ class InverseRendering:
         torch.cuda.synchronize()
         tds.close()
         self.x = torch.from_numpy(self.ds.vertices()).cuda().contiguous()   # [V][3]
-        self.m = torch.zeros_like(self.x)
-        self.v = torch.zeros_like(self.x)
         self.grad = torch.zeros_like(self.x)
+        if setup.optimizer == "adam":
+            self.m = torch.zeros_like(self.x)
+            self.v = torch.zeros_like(self.x)
+        elif setup.optimizer == "largesteps":
+            # LargeSteps (pyTensorRay/optimizer.py:147-180): the optimized variable
+            # is u = M x, M = I + lmbda L on the mesh; gradients map as M^-1 g and
+            # positions come back as x = M^-1 u
+            self.solver = ctx.lsolver(init_spec.meshes[0].idx, self.n_mesh, setup.lmbda)
+            self.u = torch.empty(self.n_mesh, 3, device="cuda")
+            self.solver.apply(self.x.data_ptr(), self.u.data_ptr(), self.stream)
+            self.gu = torch.zeros_like(self.u)
+            self.m = torch.zeros_like(self.u)
+            self.v = torch.zeros_like(self.u)
+        else:
+            raise ValueError(f"unknown optimizer {setup.optimizer!r}")
         opt0 = self._options(0, 0)
         self.ckpt = torch.empty(self.ds.checkpoint_bytes(self.cams[0], opt0), dtype=torch.uint8, device="cuda")
         self.film = torch.empty(3, W * H, device="cuda")
@@ -158,8 +173,15 @@ class InverseRendering:
                 if name != "adam":
                     acc[name] += a.elapsed_time(b)
         ev["adam"][0].record(st)
-        self.ctx.adam_step(self.x.data_ptr(), self.grad.data_ptr(), self.m.data_ptr(), self.v.data_ptr(),
-                           self.x.numel(), t, lr=self.s.lr, stream=st)
+        if self.s.optimizer == "adam":
+            self.ctx.adam_step(self.x.data_ptr(), self.grad.data_ptr(), self.m.data_ptr(), self.v.data_ptr(),
+                               self.x.numel(), t, lr=self.s.lr, stream=st)
+        else:
+            # mesh vertices are the first n_mesh rows of x and grad
+            self.solver.solve(self.grad.data_ptr(), self.gu.data_ptr(), max_iter=400, tol=1e-5, stream=st)
+            self.ctx.adam_uniform_step(self.u.data_ptr(), self.gu.data_ptr(), self.m.data_ptr(), self.v.data_ptr(),
+                                       self.u.numel(), t, lr=self.s.lr, stream=st)
+            self.solver.solve(self.u.data_ptr(), self.x.data_ptr(), max_iter=400, tol=1e-6, stream=st)
         ev["adam"][1].record(st)
         torch.cuda.synchronize()
         acc["adam"] = ev["adam"][0].elapsed_time(ev["adam"][1])
@@ -175,6 +197,8 @@ class InverseRendering:
         return rec
 
     def close(self):
+        if getattr(self, "solver", None) is not None:
+            self.solver.close()
         self.ds.close()
 
 

@@ -62,3 +62,35 @@ def test_loop_reduces_error(gpu_ctx):
     assert recs[-1]["vertex_error"] < 0.99 * e0
     assert np.mean([r["loss"] for r in recs[-3:]]) < 0.6 * recs[0]["loss"]
     prob.close()
+
+
+def test_largesteps_step_is_composition(gpu_ctx, oracle_mod):
+    """With optimizer="largesteps" one step maps the vertex gradient through
+    M^-1 (M = I + lmbda L of the mesh), takes a uniform-Adam step on u = M x
+    and sets x = M^-1 u; checked against scipy solves and the oracle's
+    uniform Adam."""
+    import scipy.sparse.linalg as spla
+    s = _small(optimizer="largesteps", lmbda=4.0, lr=1e-2)
+    prob = inverse.make_problem(gpu_ctx, s)
+    n = prob.n_mesh
+    x0 = prob.x.cpu().numpy()[:n].astype(np.float64)
+    rec = prob.step()
+    assert rec["ms"]["adam"] > 0
+    M = oracle_mod.laplacian_system(prob.ds.info() and inverse_faces(s), n, s.lmbda).tocsc()
+    g = prob.grad.cpu().numpy()[:n].astype(np.float64)
+    gu = np.stack([spla.spsolve(M, g[:, c]) for c in range(3)], 1)
+    u0 = (M @ x0).astype(np.float32)
+    z = np.zeros_like(u0)
+    u1, _, _ = oracle_mod.adam_uniform_step(u0, gu.astype(np.float32), z, z, 1, lr=s.lr)
+    x1 = np.stack([spla.spsolve(M, u1[:, c].astype(np.float64)) for c in range(3)], 1)
+    got = prob.x.cpu().numpy()[:n]
+    step = np.abs(x1 - x0).max()
+    assert step > 0
+    assert np.abs(got - x1).max() <= 2e-2 * step + 1e-6
+    assert np.array_equal(prob.ds.vertices()[:n], got)
+    prob.close()
+
+
+def inverse_faces(s):
+    from tensorray_amd import scenes
+    return scenes.geodesic_sphere(s.freq)[1]
