@@ -134,6 +134,14 @@ class InverseRendering:
         """Mean Euclidean distance of the mesh vertices to the target's."""
         return (self.x[:self.n_mesh] - self.target_pos).norm(dim=1).mean().item()
 
+    def radial_error(self):
+        """Mean distance along the target's radial direction (the direction
+        of its displacement): the shape error without the tangential drift
+        of vertices, which moves no surface."""
+        t = self.target_pos
+        n = t / t.norm(dim=1, keepdim=True)
+        return ((self.x[:self.n_mesh] - t) * n).sum(dim=1).abs().mean().item()
+
     def step(self):
         torch = self.torch
         st = self.stream
@@ -191,7 +199,7 @@ class InverseRendering:
         acc["refit"] = (time.perf_counter() - r0) * 1e3
         acc["total"] = (time.perf_counter() - t0) * 1e3
         self.steps_done = t
-        rec = dict(step=t, loss=loss.item(), vertex_error=self.vertex_error(),
+        rec = dict(step=t, loss=loss.item(), vertex_error=self.vertex_error(), radial_error=self.radial_error(),
                    ms={k: round(v, 3) for k, v in acc.items()})
         self.log.append(rec)
         return rec
@@ -230,9 +238,10 @@ def main(argv=None):
                                                                       t_s=round(time.perf_counter() - t0, 2))),
                                                      flush=True))
     setup_s = time.perf_counter() - t0
-    e0 = prob.vertex_error()
+    e0, r0 = prob.vertex_error(), prob.radial_error()
     print(json.dumps(dict(setup=dataclasses.asdict(s), triangles=int(prob.ds.info().num_triangles),
-                          vertices=prob.V, setup_s=round(setup_s, 2), initial_vertex_error=e0)), flush=True)
+                          vertices=prob.V, setup_s=round(setup_s, 2), initial_vertex_error=e0,
+                          initial_radial_error=r0)), flush=True)
     for _ in range(s.steps):
         print(json.dumps(prob.step()), flush=True)
     steps = prob.log
@@ -240,7 +249,8 @@ def main(argv=None):
     mean = {k: sum(r["ms"][k] for r in steps) / len(steps) for k in keys}
     paths = s.views * s.res * s.res * s.spp
     print(json.dumps(dict(summary=True, steps=len(steps), initial_vertex_error=e0,
-                          final_vertex_error=steps[-1]["vertex_error"], final_loss=steps[-1]["loss"],
+                          final_vertex_error=steps[-1]["vertex_error"], initial_radial_error=r0,
+                          final_radial_error=steps[-1]["radial_error"], final_loss=steps[-1]["loss"],
                           mean_ms=mean, paths_per_step=paths,
                           interior_mpaths_per_s=paths / ((mean["forward"] + mean["adjoint"]) * 1e-3) / 1e6)),
           flush=True)
