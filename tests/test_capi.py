@@ -136,3 +136,23 @@ def test_host_context_cannot_render(host_ctx):
     with pytest.raises(_capi.TRCError, match="TRC_ERR_STATE"):
         ds.render_fwd(cam, opt, 0)
     ds.close()
+
+
+def test_host_context_rejects_device_work(host_ctx):
+    """Batches, plans, optimizer steps and solvers need a device; on a
+    host-only context they fail with TRC_ERR_STATE before touching memory."""
+    ds = _capi.DeviceScene(host_ctx, scenes.config1_scene())
+    cam = _capi.make_camera(8, 8)
+    opt = _capi.make_options(spp=1)
+    job = ds.job(cam, opt, 0, 64, paths=1, ckpt=1)
+    with pytest.raises(_capi.TRCError, match="TRC_ERR_STATE"):
+        host_ctx.render_batch([job], False)
+    with pytest.raises(_capi.TRCError, match="TRC_ERR_STATE"):
+        host_ctx.plan([job], adjoint=False)
+    with pytest.raises(_capi.TRCError, match="TRC_ERR_STATE"):
+        host_ctx.adam_step(1, 1, 1, 1, 4, 1)
+    with pytest.raises(_capi.TRCError, match="TRC_ERR_STATE"):
+        host_ctx.adam_uniform_step(1, 1, 1, 1, 4, 1)
+    with pytest.raises(_capi.TRCError, match="TRC_ERR_STATE"):
+        host_ctx.lsolver(np.array([[0, 1, 2]]), 3, 1.0)
+    ds.close()
