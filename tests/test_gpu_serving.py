@@ -19,9 +19,10 @@ def _requests():
     return out
 
 
-def test_served_results_equal_standalone(gpu_ctx):
+@pytest.mark.parametrize("order", ["fifo", "srpt"])
+def test_served_results_equal_standalone(gpu_ctx, order):
     import torch
-    srv = serving.Server(gpu_ctx, budget_bytes=1 << 30, batch_paths=9000, chunk_paths=4000)
+    srv = serving.Server(gpu_ctx, budget_bytes=1 << 30, batch_paths=9000, chunk_paths=4000, order=order, aging=1e5)
     srv.add_scene("a", scenes.sphere_scene(subdiv=2))
     srv.add_scene("b", scenes.sphere_scene(subdiv=3, amplitude=0.05, smooth=True))
     reqs = _requests()
