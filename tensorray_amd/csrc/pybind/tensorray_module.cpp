@@ -70,6 +70,14 @@ PYBIND11_MODULE(TensorRay, m) {
       .def(py::init([](float v, bool rg) { return Tensorf::scalar(v, rg); }), "val"_a = 0.0f, "requiresGrad"_a = false)
       .def(py::init([](float x, float y, float z, bool rg) { return Tensorf::vec3(x, y, z, rg); }), "x"_a = 0.0f,
            "y"_a = 0.0f, "z"_a = 0.0f, "requiresGrad"_a = false)
+      // 4x4 matrix from its entries in row order (the reference's 16-value
+      // constructor, TensorRay.cpp:90-98; used for Shape.rotate_matrix)
+      .def(py::init([](float m00, float m01, float m02, float m03, float m10, float m11, float m12, float m13,
+                       float m20, float m21, float m22, float m23, float m30, float m31, float m32, float m33) {
+        Tensorf t;
+        t.host = {m00, m01, m02, m03, m10, m11, m12, m13, m20, m21, m22, m23, m30, m31, m32, m33};
+        return t;
+      }))
       .def(py::init([](FloatPtr p, const TensorShape& s) { return Tensorf::from_ptr(p.p, s); }))
       .def("size", &Tensorf::size)
       .def("requires_grad", [](const Tensorf& t) { return t.requires_grad; })

@@ -232,3 +232,26 @@ def test_exr_writer(tmp_path):
     back = utils.load_exr(f)
     assert back.shape == (5, 7, 3)
     assert np.array_equal(back, img.astype(np.float16).astype(np.float32))
+
+
+def test_multi_pose_helpers(scene_dir):
+    """Multi-pose helpers (multi_pose.py:40-82 of the reference): the first
+    random pose is the identity, every pose is a proper rotation, z-axis
+    poses step by 2 pi / n, and the 16-value Tensorf holds the 4x4 matrix."""
+    from tensorray_amd.pyTensorRay import multi_pose
+    rots = multi_pose.gen_z_axis_rotations(4)
+    assert np.allclose(rots[0], np.eye(3).ravel())
+    np.testing.assert_allclose(rots[1].reshape(3, 3), [[0, -1, 0], [1, 0, 0], [0, 0, 1]], atol=1e-12)
+    tensors = multi_pose.matrices_to_tensor(rots)
+    assert len(tensors) == 4 and tensors[1].size() == 16
+    r = multi_pose.gen_random_rotations(5, 3)
+    assert np.allclose(r[0], np.eye(3).ravel())
+    for m in r:
+        M = m.reshape(3, 3)
+        assert np.allclose(M @ M.T, np.eye(3), atol=1e-12) and np.isclose(np.linalg.det(M), 1.0)
+    s = TR.Scene()
+    s.load_file(str(scene_dir / "scene.xml"))
+    c = s.shapes[0].get_obj_center()
+    cv = torch.zeros(3)
+    c.value_to_torch(get_ptr(cv))
+    assert np.allclose(cv.numpy(), 0.0, atol=1e-6)   # the unit cube is centred at the origin
