@@ -3,6 +3,8 @@
 A TensorRay image is Shape([W*H], Vec3) stored as three planes [3][H*W];
 torch images are (H, W, 3).
 """
+import os
+import re
 import struct
 
 import numpy as np
@@ -150,3 +152,58 @@ def save_torch_image(filename, img):
             f.write(a.tobytes())
     else:
         raise ValueError("supported image formats: .exr, .pfm, .ppm")
+
+
+def load_pfm(filename):
+    """(H, W, 3) float32 image from a PFM file (colour, either byte order)."""
+    with open(filename, "rb") as f:
+        kind = f.readline().strip()
+        w, h = (int(x) for x in f.readline().split())
+        scale = float(f.readline())
+        a = np.frombuffer(f.read(), "<f4" if scale < 0 else ">f4")
+    if kind != b"PF":
+        raise ValueError("only colour PFM images are read")
+    return a.reshape(h, w, 3)[::-1].astype(np.float32)
+
+
+def load_torch_image(filename):
+    """torch (height, width, 3) float image from .exr or .pfm
+    (utils.py:83-89 of the reference, which reads through OpenCV)."""
+    if filename.endswith(".exr"):
+        return torch.from_numpy(np.ascontiguousarray(load_exr(filename)))
+    if filename.endswith(".pfm"):
+        return torch.from_numpy(np.ascontiguousarray(load_pfm(filename)))
+    raise ValueError("supported image formats: .exr, .pfm")
+
+
+def load_image_to_tensor(filename):
+    """TR.Tensorf image Shape([W*H], Vec3) from an image file."""
+    img = load_torch_image(filename)
+    return image_torch_to_tensor(img, img.shape[0], img.shape[1])
+
+
+def read_images(path, ext=".exr"):
+    """All images with extension `ext` in `path`, in natural (numeric-aware)
+    file-name order."""
+    def key(name):
+        return [int(c) if c.isdigit() else c for c in re.split("([0-9]+)", name)]
+    return [load_torch_image(os.path.join(path, f)) for f in sorted(os.listdir(path), key=key) if f.endswith(ext)]
+
+
+def uniform_laplacian(num_verts, edges):
+    """Uniform graph Laplacian L = D - A as a sparse torch tensor; edges:
+    [2][E] vertex pairs (each undirected edge once)."""
+    with torch.no_grad():
+        e0, e1 = edges[0].long(), edges[1].long()
+        idx = torch.cat([torch.stack([e0, e1]), torch.stack([e1, e0])], dim=1)
+        ones = torch.ones(idx.shape[1], dtype=torch.float32, device=edges.device)
+        A = torch.sparse_coo_tensor(idx, ones, (num_verts, num_verts)).coalesce()
+        deg = torch.sparse.sum(A, dim=0).to_dense()
+        d = torch.arange(num_verts, device=edges.device)
+        D = torch.sparse_coo_tensor(torch.stack([d, d]), deg, (num_verts, num_verts))
+        return (D - A).coalesce()
+
+
+def sparse_eye(size):
+    d = torch.arange(size)
+    return torch.sparse_coo_tensor(torch.stack([d, d]), torch.ones(size), (size, size))

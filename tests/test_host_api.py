@@ -255,3 +255,27 @@ def test_multi_pose_helpers(scene_dir):
     cv = torch.zeros(3)
     c.value_to_torch(get_ptr(cv))
     assert np.allclose(cv.numpy(), 0.0, atol=1e-6)   # the unit cube is centred at the origin
+
+
+def test_pytensorray_helpers(scene_dir):
+    """Image readers and the uniform Laplacian of pyTensorRay.utils, the
+    option constructors of pyTensorRay.fwd."""
+    from tensorray_amd.pyTensorRay import fwd, utils
+    rng = np.random.default_rng(1)
+    img = rng.uniform(0, 2, (4, 6, 3)).astype(np.float32)
+    utils.save_torch_image(str(scene_dir / "i_2.pfm"), torch.from_numpy(img))
+    utils.save_torch_image(str(scene_dir / "i_10.pfm"), torch.from_numpy(2 * img))
+    assert torch.equal(utils.load_torch_image(str(scene_dir / "i_2.pfm")), torch.from_numpy(img))
+    imgs = utils.read_images(str(scene_dir), ".pfm")
+    assert len(imgs) == 2 and torch.equal(imgs[1], torch.from_numpy(2 * img))  # numeric order: 2 before 10
+    edges = torch.tensor([[0, 1, 2, 0], [1, 2, 0, 3]])
+    L = utils.uniform_laplacian(4, edges).to_dense()
+    np.testing.assert_array_equal(L.numpy(), [[3, -1, -1, -1], [-1, 2, -1, 0], [-1, -1, 2, 0], [-1, 0, 0, 1]])
+    o = fwd.RenderOptions(7, 2, 4, 1, 1, 0, 0, g_direct=True, checkpoint=True)
+    b = fwd.RenderBatchOptions(spp_batch=2)
+    utils.update_render_batch_options(o, b)
+    g = fwd.GuidingOptions(g_direct_spp=8)
+    utils.update_guiding_options(o, g)
+    assert o.spp_batch == 2 and o.g_direct_spp == 8 and o.g_direct is False and o.checkpoint is True
+    s = fwd.Scene(str(scene_dir / "scene.xml"))
+    assert len(s.shapes) == 2
