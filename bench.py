@@ -407,12 +407,24 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    served = served_comparison(ctx, a) if a.serve_requests > 0 else None
-    multi = multiview_comparison(ctx, a) if a.multiview_views > 0 else None
-    sweep = served_sweep(ctx, a) if a.serve_sweep else None
+    def optional(section, fn):
+        """The side measurements never cost the headline line: a failure is
+        reported in its field."""
+        try:
+            return fn(ctx, a)
+        except Exception as e:  # noqa: BLE001
+            print(f"bench.py: {section} failed: {e!r}", file=sys.stderr)
+            return {"error": repr(e)}
+
+    served = optional("served", served_comparison) if a.serve_requests > 0 else None
+    multi = optional("multiview", multiview_comparison) if a.multiview_views > 0 else None
+    sweep = optional("served_sweep", served_sweep) if a.serve_sweep else None
     cpu = None
     if not a.no_cpu_baseline:
-        cpu = cpu_baseline(spec, cam_d, W, H, B, spp, a.cpu_spp, a.cpu_runs)
+        try:
+            cpu = cpu_baseline(spec, cam_d, W, H, B, spp, a.cpu_spp, a.cpu_runs)
+        except Exception as e:  # noqa: BLE001
+            print(f"bench.py: cpu baseline failed: {e!r}", file=sys.stderr)
     out = {
         "metric": "Mpaths/s fwd+adjoint (512², depth 4) & p99 latency, served vs plain, 1 GPU",
         "value": mpaths,
