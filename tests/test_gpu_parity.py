@@ -77,6 +77,8 @@ CASES = {
     "conductor": (lambda: scenes.sphere_scene(subdiv=3, amplitude=0.05, material=scenes.Material(
         scenes.MAT_ROUGHCONDUCTOR, albedo=(0.143, 0.374, 1.44), spec=(3.98, 2.38, 1.60), roughness=0.35)),
         48, 48, 4, 3),
+    # the deepest paths the path record holds (max_bounces 8) in the closed room
+    "deepest": (lambda: scenes.box_scene(subdiv=2), 24, 24, 2, 8),
 }
 
 
@@ -111,7 +113,7 @@ def _grad_close(g, r, rel=1e-3):
     return ok.all(), err.max() / scale
 
 
-@pytest.mark.parametrize("case", ["config1", "metric_small", "textured", "conductor", "smooth"])
+@pytest.mark.parametrize("case", ["config1", "metric_small", "textured", "conductor", "smooth", "deepest"])
 def test_adjoint_matches_oracle(gpu_ctx, oracle_mod, case):
     torch = _torch()
     make, W, H, spp, B = CASES[case]
@@ -281,7 +283,7 @@ def test_guided_direct_matches_oracle(gpu_ctx, oracle_mod, subdiv):
     ds.close()
 
 
-@pytest.mark.parametrize("case", ["config1", "metric_small", "textured", "smooth", "conductor"])
+@pytest.mark.parametrize("case", ["config1", "metric_small", "textured", "smooth", "conductor", "deepest"])
 def test_checkpointed_adjoint(gpu_ctx, oracle_mod, case):
     """trc_render_fwd_ckpt writes the same film as trc_render_fwd; the
     adjoint from the checkpoint (no traversal) matches the oracle's AD like
