@@ -134,6 +134,25 @@ class SceneSpec:
     def num_triangles(self):
         return sum(len(m.idx) for m in self.meshes)
 
+    def to_module(self, cam, W, H):
+        """The same scene as a `TensorRay.Scene` (the reference-style module),
+        built through its programmatic API, with one camera `cam` (a dict of
+        pos, target, up, fov) at W x H.  Shapes and BSDFs keep this spec's
+        order, so shapes[i] is meshes[i]."""
+        from tensorray_amd import TensorRay as TR
+        s = TR.Scene()
+        for m in self.materials:
+            tex = None if m.texture < 0 else self.textures[m.texture]
+            s.add_bsdf(m.type, list(m.albedo), list(m.spec), m.roughness, tex)
+        for mesh in self.meshes:
+            s.add_mesh(np.asarray(mesh.pos, np.float32), np.asarray(mesh.idx, np.uint32), mesh.material,
+                       None if mesh.uv is None else np.asarray(mesh.uv, np.float32),
+                       None if mesh.uv_idx is None else np.asarray(mesh.uv_idx, np.uint32),
+                       None if mesh.emit is None else list(mesh.emit), bool(mesh.flags & MESH_SMOOTH))
+        s.add_camera(TR.Camera(TR.Vector3f(*cam["pos"]), TR.Vector3f(*cam["target"]), TR.Vector3f(*cam["up"]),
+                               W, H, cam["fov"]))
+        return s
+
 
 # ---------------------------------------------------------------------------
 # Generators

@@ -15,18 +15,7 @@ TR = pytest.importorskip("tensorray_amd.TensorRay")
 
 def _tr_scene(spec, cam_d, W, H):
     """The scenes.SceneSpec built through the module's programmatic API."""
-    s = TR.Scene()
-    for m in spec.materials:
-        tex = None if m.texture < 0 else spec.textures[m.texture]
-        s.add_bsdf(m.type, list(m.albedo), list(m.spec), m.roughness, tex)
-    for mesh in spec.meshes:
-        s.add_mesh(np.asarray(mesh.pos, np.float32), np.asarray(mesh.idx, np.uint32), mesh.material,
-                   None if mesh.uv is None else np.asarray(mesh.uv, np.float32),
-                   None if mesh.uv_idx is None else np.asarray(mesh.uv_idx, np.uint32),
-                   None if mesh.emit is None else list(mesh.emit), bool(mesh.flags & scenes.MESH_SMOOTH))
-    s.add_camera(TR.Camera(TR.Vector3f(*cam_d["pos"]), TR.Vector3f(*cam_d["target"]), TR.Vector3f(*cam_d["up"]),
-                           W, H, cam_d["fov"]))
-    return s
+    return spec.to_module(cam_d, W, H)
 
 
 @pytest.fixture
