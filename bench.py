@@ -54,6 +54,8 @@ def parse():
                    help="serve over 1k / 20k / 200k / 1M-triangle scenes (slower set-up) instead of 1k / 20k")
     p.add_argument("--multiview-views", type=int, default=32,
                    help="views of the multi-view workload (0 skips it)")
+    p.add_argument("--boundary-spp", type=int, default=16,
+                   help="edge samples per pixel of each boundary class in the boundary section (0: skip it)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="after the timed region, copy a MIB-sized device buffer (reference kernel of known "
                         "traffic for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -297,6 +299,44 @@ def multiview_comparison(ctx, a):
     return out
 
 
+def boundary_terms(scene, cam, W, H, B, dldi, a):
+    """Edge-sample throughput of the boundary integrators on the metric
+    workload (512^2, depth 4, seed 1234): each class at `boundary_spp`
+    samples per pixel (W * H * boundary_spp samples), dL/dI = the plain
+    step's film minus the target.  Guided direct includes building its guide
+    (g_direct_spp = 16 samples per secondary edge)."""
+    import torch
+    from tensorray_amd import _capi
+    n = W * H * a.boundary_spp
+    info = scene.info()
+    dpos = torch.zeros(info.num_vertices, 3, device="cuda")
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    classes = [("primary", "primary", dict(sppe=a.boundary_spp)),
+               ("pixel", "pixel", dict(sppe0=a.boundary_spp)),
+               ("direct", "direct", dict(sppse0=a.boundary_spp)),
+               ("direct_guided", "direct", dict(sppse0=a.boundary_spp, g_direct=True)),
+               ("indirect", "indirect", dict(sppse1=a.boundary_spp))]
+    out = dict(samples_per_class=n, workload=f"{W}x{H}, {a.boundary_spp} edge samples per pixel per class, "
+                                             f"depth {B + 1}, metric scene")
+    for name, kind, kw in classes:
+        opt = _capi.make_options(seed=1234, max_bounces=B, **kw)
+        scene.render_boundary(cam, opt, kind, dldi.data_ptr(), dpos.data_ptr(), stream)
+        torch.cuda.synchronize()
+        ms, wall = [], []
+        for _ in range(a.steps):
+            t0 = time.perf_counter()
+            ev[0].record(stream)
+            scene.render_boundary(cam, opt, kind, dldi.data_ptr(), dpos.data_ptr(), stream)
+            ev[1].record(stream)
+            torch.cuda.synchronize()
+            wall.append((time.perf_counter() - t0) * 1e3)
+            ms.append(ev[0].elapsed_time(ev[1]))
+        t = sum(ms) / len(ms)
+        out[name] = dict(ms=t, wall_ms=sum(wall) / len(wall), msamples_per_s=n / t / 1e3)
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -416,6 +456,9 @@ def main():
             print(f"bench.py: {section} failed: {e!r}", file=sys.stderr)
             return {"error": repr(e)}
 
+    dldi = film - target
+    boundary = (optional("boundary", lambda c, aa: boundary_terms(scene, cam, W, H, B, dldi, aa))
+                if a.boundary_spp > 0 else None)
     served = optional("served", served_comparison) if a.serve_requests > 0 else None
     multi = optional("multiview", multiview_comparison) if a.multiview_views > 0 else None
     sweep = optional("served_sweep", served_sweep) if a.serve_sweep else None
@@ -446,6 +489,7 @@ def main():
         "cpu_baseline": cpu,
         "served": served,
         "multiview": multi,
+        "boundary": boundary,
     }
     if sweep is not None:
         out["served_sweep"] = sweep

@@ -137,6 +137,8 @@ void free_scene_bufs(trc_scene* s) {
   s->prim_buf = DevBuf{};
   if (s->guide_buf.ptr) hipFree(s->guide_buf.ptr);
   s->guide_buf = DevBuf{};
+  if (s->bq_buf.ptr) hipFree(s->bq_buf.ptr);
+  s->bq_buf = DevBuf{};
   s->vn_adj = nullptr;
   s->device_bytes = 0;
   s->dev_texs = nullptr;
@@ -1174,7 +1176,17 @@ trc_status trc_render_boundary(trc_scene* s, const trc_camera* cam, const trc_op
       hip_check(launch_guide_build(s->dev, Lg, opt->seed, guide, st), "k_guide");
     }
     LaunchCfg L = make_cfg(s->ctx, s, st, N);
-    hip_check(launch_boundary(kind, s->dev, C, opt->seed, opt->max_bounces, spp, L, prim, guide, dLdI, dpos, st),
+    const size_t qb = (size_t)boundary_queue_bytes(kind, (uint64_t)N);
+    if (s->bq_buf.bytes < qb) {
+      hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+      if (s->bq_buf.ptr) hipFree(s->bq_buf.ptr);
+      s->bq_buf.ptr = nullptr;
+      s->bq_buf.bytes = 0;
+      hip_check(hipMalloc(&s->bq_buf.ptr, qb), "hipMalloc(boundary queue)");
+      s->bq_buf.bytes = qb;
+    }
+    hip_check(launch_boundary(kind, s->dev, C, opt->seed, opt->max_bounces, spp, L, prim, guide, dLdI, dpos,
+                              s->bq_buf.ptr, st),
               "k_boundary");
     if (kind == 0) hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
   });

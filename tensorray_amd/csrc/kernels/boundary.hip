@@ -268,42 +268,143 @@ struct BoundaryParams {
 };
 
 // ---------------------------------------------------------------------------
+// Survivor queues.  The primary, direct and indirect terms run in two
+// kernels per chunk of samples: a select kernel draws the samples, traces
+// their first rays and keeps the few that can contribute; an eval kernel
+// traces the long radiance / importance paths of those only, so its waves
+// start full instead of carrying the lanes whose samples died at the first
+// test.  The queue is structure-of-arrays (word k of slot s at
+// w[k * cap + s]) and filled by wave-level ballot compaction: one atomic
+// per wave and consecutive slots for the surviving lanes.  Chunks are large
+// (4M samples, one chunk for a 512^2 x 16 request): every chunk ends in two
+// kernel tails, and at 1M samples per chunk those cost more than the
+// compaction saved.  Survivors of the 512^2 metric scene: 10 % of the
+// primary, 0.8 % of the direct and 13 % of the indirect samples.
+// ---------------------------------------------------------------------------
+struct BndQueue {
+  uint32_t* count;  // survivors of this chunk
+  uint32_t* w;      // kBndWords x cap words
+  uint32_t cap;
+};
+constexpr int kBndWords = 20;
+constexpr uint64_t kBndChunk = 1ull << 22;  // 335 MB of queue at most
+
+// Slot of the calling lane in the queue when `keep`, else -1.  Every active
+// lane of the wave calls it at the same point.
+__device__ inline int64_t wave_append(bool keep, uint32_t* count) {
+  const uint64_t act = __ballot(1);
+  const uint64_t m = __ballot(keep);
+  if (m == 0ull) return -1;
+  const int lane = (int)(threadIdx.x & 63u);
+  const int leader = __ffsll((unsigned long long)act) - 1;
+  uint32_t base = 0u;
+  if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+  base = (uint32_t)__shfl((int)base, leader);
+  if (!keep) return -1;
+  return (int64_t)base + (int64_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+struct QWriter {
+  const BndQueue& q;
+  uint32_t s;
+  int k = 0;
+  __device__ void f(float x) { q.w[(size_t)(k++) * q.cap + s] = __float_as_uint(x); }
+  __device__ void u(uint32_t x) { q.w[(size_t)(k++) * q.cap + s] = x; }
+  __device__ void i(int32_t x) { u((uint32_t)x); }
+  __device__ void f3(F3 v) { f(v.x); f(v.y); f(v.z); }
+  __device__ void u64(uint64_t x) { u((uint32_t)x); u((uint32_t)(x >> 32)); }
+};
+struct QReader {
+  const BndQueue& q;
+  uint32_t s;
+  int k = 0;
+  __device__ float f() { return __uint_as_float(q.w[(size_t)(k++) * q.cap + s]); }
+  __device__ uint32_t u() { return q.w[(size_t)(k++) * q.cap + s]; }
+  __device__ int32_t i() { return (int32_t)u(); }
+  __device__ F3 f3() { F3 v; v.x = f(); v.y = f(); v.z = f(); return v; }
+  __device__ uint64_t u64() { uint64_t lo = u(); uint64_t hi = u(); return lo | (hi << 32); }
+};
+
+// ---------------------------------------------------------------------------
 // Primary edges (BoundaryPrimary.cpp:36-181)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBndWaves))) k_boundary_primary(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
-                                                             EdgeSet E, uint64_t N, const float* dLdI, float* dpos) {
+// select: edge point, the ray away from the camera, its projection onto the
+// sensor, the edge point's visibility from the camera and baseVal.
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBndWaves))) k_bnd_primary_select(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
+                                                               EdgeSet E, uint64_t first, uint64_t end, BndQueue Q) {
+  __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
+  TraceCtx tc = make_trace_ctx_b(lds_stack, L);
+  const F3 eye = cam_eye(C);
+  const uint64_t stride = (uint64_t)L.grid * kBlock;
+  for (uint64_t i = first + (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < end; i += stride) {
+    bool keep = false;
+    Hit h;
+    SensorSample sd;
+    F3 p, rd, n;
+    float pdf, bv, rb;
+    int32_t ei;
+    do {
+      Rnd4 rnd = rng4(P.seed, kRngPrimaryEdge, i, 0u);
+      ei = sample_edge(E, rnd.a, &pdf);
+      const EdgeDev ed = E.e[ei];
+      EdgeGeo g = edge_geo(S, ed);
+      rb = rnd.b;
+      p = g.p0 + rnd.b * g.e1;
+      F3 edge = normalize(g.e1);
+      F3 edge2 = normalize(g.p2 - g.p0);
+      F3 dir = normalize(p - eye);
+      h = trace_ray<false>(S, tc, p, dir, kShadowEpsilon, 1e32f);
+      if (h.tri < 0) break;
+      F3 hitP = p + h.t * dir;
+      sd = sensor_sample(C, hitP);
+      if (!sd.valid) break;
+      rd = camera_dir(C, sd.qx * (float)C.width, sd.qy * (float)C.height);
+      float tmax = length(p - eye) - kShadowEpsilon;
+      if (trace_ray<true>(S, tc, eye, rd, kShadowEpsilon, tmax).tri >= 0) break;
+      Its its = post_intersect(S, h.tri, h.u, h.v);
+      bv = base_val_camera(its.p, its.ng, rd, p, eye, edge, edge2, &n, false);
+      // K = rad * bv / pdf vanishes or is not finite for every radiance
+      keep = bv != 0.0f && isfinite(bv);
+    } while (false);
+    const int64_t s = wave_append(keep, Q.count);
+    if (s >= 0) {
+      QWriter w{Q, (uint32_t)s};
+      w.u64(i); w.i(h.tri); w.f(h.u); w.f(h.v); w.f3(rd); w.f(sd.value); w.i(sd.pixel);
+      w.f(bv); w.f(pdf); w.f3(n); w.i(ei); w.f(rb); w.f3(p);
+    }
+  }
+}
+
+// eval: the radiance seen through the sensor point and the scatter.
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBndWaves))) k_bnd_primary_eval(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
+                                                             EdgeSet E, BndQueue Q, const float* dLdI, float* dpos) {
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
   const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
   const F3 eye = cam_eye(C);
-  const uint64_t stride = (uint64_t)L.grid * kBlock;
-  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
-    Rnd4 rnd = rng4(P.seed, kRngPrimaryEdge, i, 0u);
-    float pdf;
-    int32_t ei = sample_edge(E, rnd.a, &pdf);
-    const EdgeDev ed = E.e[ei];
-    EdgeGeo g = edge_geo(S, ed);
-    F3 p = g.p0 + rnd.b * g.e1;
-    F3 edge = normalize(g.e1);
-    F3 edge2 = normalize(g.p2 - g.p0);
-    F3 dir = normalize(p - eye);
-    Hit h = trace_ray<false>(S, tc, p, dir, kShadowEpsilon, 1e32f);
-    if (h.tri < 0) continue;
-    F3 hitP = p + h.t * dir;
-    SensorSample sd = sensor_sample(C, hitP);
-    if (!sd.valid) continue;
-    F3 rd = camera_dir(C, sd.qx * (float)C.width, sd.qy * (float)C.height);
-    float tmax = length(p - eye) - kShadowEpsilon;
-    if (trace_ray<true>(S, tc, eye, rd, kShadowEpsilon, tmax).tri >= 0) continue;
-    Its its = post_intersect(S, h.tri, h.u, h.v);
-    F3 n;
-    float bv = base_val_camera(its.p, its.ng, rd, p, eye, edge, edge2, &n, false);
-    F3 rad = boundary_radiance(S, tc, its, rd, sd.value, P.seed, kRngPrimaryEdge, i, P.max_bounces);
+  const uint32_t n_q = *Q.count;
+  const uint32_t stride = (uint32_t)L.grid * kBlock;
+  for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < n_q; s += stride) {
+    QReader r{Q, s};
+    const uint64_t i = r.u64();
+    const int32_t tri = r.i();
+    const float u = r.f(), v = r.f();
+    const F3 rd = r.f3();
+    const float value = r.f();
+    const int32_t pixel = r.i();
+    const float bv = r.f(), pdf = r.f();
+    const F3 n = r.f3();
+    const int32_t ei = r.i();
+    const float rb = r.f();
+    const F3 p = r.f3();
+    Its its = post_intersect(S, tri, u, v);
+    F3 rad = boundary_radiance(S, tc, its, rd, value, P.seed, kRngPrimaryEdge, i, P.max_bounces);
     F3 K = rad * bv / pdf;
     if (!(length(K) > 0.0f) || !finite3(K)) continue;
-    float w = dot(dLdI_at(dLdI, WH, sd.pixel), K) / (float)P.spp;
+    float w = dot(dLdI_at(dLdI, WH, pixel), K) / (float)P.spp;
     if (w == 0.0f) continue;
-    scatter_xdn(S, h.tri, eye, p, n, w, &ed, rnd.b, ed.f0, -1, 0.0f, 0.0f, dpos);
+    const EdgeDev ed = E.e[ei];
+    scatter_xdn(S, tri, eye, p, n, w, &ed, rb, ed.f0, -1, 0.0f, 0.0f, dpos);
   }
 }
 
@@ -490,56 +591,96 @@ __global__ void k_guide_normalize_last(GuideDev G) {
   if (integral > 0.0f) G.cdf[G.n - 1] = integral / integral;
 }
 
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBndWaves))) k_boundary_direct(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
-                                                            GuideDev G, uint64_t N, const float* dLdI, float* dpos) {
+// select: edge point and light point, the two rays from the edge point,
+// the emitted radiance and baseVal.
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBndWaves))) k_bnd_direct_select(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
+                                                              GuideDev G, uint64_t first, uint64_t end, BndQueue Q) {
+  __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
+  TraceCtx tc = make_trace_ctx_b(lds_stack, L);
+  const EdgeSet& E = S.sec;
+  const uint64_t stride = (uint64_t)L.grid * kBlock;
+  for (uint64_t i = first + (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < end; i += stride) {
+    bool keep = false;
+    Hit h0, h;
+    F3 K0, rdir, p, n;
+    float t;
+    int32_t ei;
+    do {
+      Rnd4 rnd = rng4(P.seed, kRngDirectEdge, i, 0u);
+      float pdf_b = 1.0f;
+      if (G.enabled) {
+        // AdaptiveQuadratureDistribution::sample (AQ_distrb.h:182-197)
+        const int32_t leaf = cdf_search(G.cdf, G.n, rnd.d);
+        const float x0 = leaf > 0 ? E.cdf[leaf - 1] : 0.0f, x1 = E.cdf[leaf];
+        rnd.a = x0 + rnd.a * (x1 - x0);
+        pdf_b = G.poly[leaf] / G.fsum[0];
+      }
+      float pdf_e;
+      ei = sample_edge(E, rnd.a, &pdf_e);
+      const EdgeDev ed = E.e[ei];
+      EdgeGeo g = edge_geo(S, ed);
+      t = cdf_reuse(E.cdf, ei, rnd.a);
+      p = g.p0 + t * g.e1;
+      F3 edge = normalize(g.e1);
+      F3 edge2 = normalize(g.p2 - g.p0);
+      LightSample ls = light_sample(S, rnd.b, rnd.c);
+      F3 ev = ls.p - p;
+      float d2 = len2(ev), dist = sqrtf(d2);
+      F3 en = ev / dist;
+      float cosT = dot(ls.n, -en);
+      float c0 = dot(en, g.n0), c1 = dot(en, g.n1);
+      bool valid0 = fabsf(c0) > kEdgeEpsilon;
+      bool valid1 = (c0 > kEdgeEpsilon && c1 < -kEdgeEpsilon) || (c0 < -kEdgeEpsilon && c1 > kEdgeEpsilon);
+      bool right = (g.boundary && valid0) || (!g.boundary && valid1);
+      if (!(right && cosT > kEpsilon && dist > kShadowEpsilon)) break;
+      float pdf = pdf_e * ls.pdf * d2 / cosT * pdf_b;
+      rdir = normalize(p - ls.p);
+      h0 = trace_ray<false>(S, tc, p, -rdir, kShadowEpsilon, 1e32f);
+      if (h0.tri < 0) break;
+      h = trace_ray<false>(S, tc, p, rdir, kShadowEpsilon, 1e32f);
+      if (h.tri < 0) break;
+      F3 hitP = p + h0.t * (-rdir);
+      if (!(length(hitP - ls.p) < kShadowEpsilon)) break;
+      F3 Le{0.0f, 0.0f, 0.0f};
+      const TriShade& sh0 = S.shade[h0.tri];
+      if (sh0.emitter >= 0) Le = emitted(S, sh0.emitter, F3{sh0.n[0], sh0.n[1], sh0.n[2]}, rdir);
+      Its its = post_intersect(S, h.tri, h.u, h.v);
+      float bv = base_val_secondary(its.p, ls.p, ls.n, rdir, p, edge, edge2, &n);
+      K0 = Le * bv / pdf;
+      // every contribution K0 * importance then vanishes or is not finite
+      keep = finite3(K0) && (K0.x != 0.0f || K0.y != 0.0f || K0.z != 0.0f);
+    } while (false);
+    const int64_t s = wave_append(keep, Q.count);
+    if (s >= 0) {
+      QWriter w{Q, (uint32_t)s};
+      w.u64(i); w.f3(K0); w.i(h.tri); w.f(h.u); w.f(h.v); w.f3(rdir); w.i(h0.tri); w.f3(p); w.f3(n); w.i(ei);
+      w.f(t);
+    }
+  }
+}
+
+// eval: EvalImportance from the sensor-side hit for B bounces and the scatter.
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBndWaves))) k_bnd_direct_eval(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
+                                                            BndQueue Q, const float* dLdI, float* dpos) {
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
   const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
   const EdgeSet& E = S.sec;
-  const uint64_t stride = (uint64_t)L.grid * kBlock;
-  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
-    Rnd4 rnd = rng4(P.seed, kRngDirectEdge, i, 0u);
-    float pdf_b = 1.0f;
-    if (G.enabled) {
-      // AdaptiveQuadratureDistribution::sample (AQ_distrb.h:182-197)
-      const int32_t leaf = cdf_search(G.cdf, G.n, rnd.d);
-      const float x0 = leaf > 0 ? E.cdf[leaf - 1] : 0.0f, x1 = E.cdf[leaf];
-      rnd.a = x0 + rnd.a * (x1 - x0);
-      pdf_b = G.poly[leaf] / G.fsum[0];
-    }
-    float pdf_e;
-    int32_t ei = sample_edge(E, rnd.a, &pdf_e);
-    const EdgeDev ed = E.e[ei];
-    EdgeGeo g = edge_geo(S, ed);
-    float t = cdf_reuse(E.cdf, ei, rnd.a);
-    F3 p = g.p0 + t * g.e1;
-    F3 edge = normalize(g.e1);
-    F3 edge2 = normalize(g.p2 - g.p0);
-    LightSample ls = light_sample(S, rnd.b, rnd.c);
-    F3 ev = ls.p - p;
-    float d2 = len2(ev), dist = sqrtf(d2);
-    F3 en = ev / dist;
-    float cosT = dot(ls.n, -en);
-    float c0 = dot(en, g.n0), c1 = dot(en, g.n1);
-    bool valid0 = fabsf(c0) > kEdgeEpsilon;
-    bool valid1 = (c0 > kEdgeEpsilon && c1 < -kEdgeEpsilon) || (c0 < -kEdgeEpsilon && c1 > kEdgeEpsilon);
-    bool right = (g.boundary && valid0) || (!g.boundary && valid1);
-    if (!(right && cosT > kEpsilon && dist > kShadowEpsilon)) continue;
-    float pdf = pdf_e * ls.pdf * d2 / cosT * pdf_b;
-    F3 rdir = normalize(p - ls.p);
-    Hit h0 = trace_ray<false>(S, tc, p, -rdir, kShadowEpsilon, 1e32f);
-    if (h0.tri < 0) continue;
-    Hit h = trace_ray<false>(S, tc, p, rdir, kShadowEpsilon, 1e32f);
-    if (h.tri < 0) continue;
-    F3 hitP = p + h0.t * (-rdir);
-    if (!(length(hitP - ls.p) < kShadowEpsilon)) continue;
-    F3 Le{0.0f, 0.0f, 0.0f};
-    const TriShade& sh0 = S.shade[h0.tri];
-    if (sh0.emitter >= 0) Le = emitted(S, sh0.emitter, F3{sh0.n[0], sh0.n[1], sh0.n[2]}, rdir);
-    Its its = post_intersect(S, h.tri, h.u, h.v);
-    F3 n;
-    float bv = base_val_secondary(its.p, ls.p, ls.n, rdir, p, edge, edge2, &n);
-    F3 K0 = Le * bv / pdf;
+  const uint32_t n_q = *Q.count;
+  const uint32_t stride = (uint32_t)L.grid * kBlock;
+  for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < n_q; s += stride) {
+    QReader r{Q, s};
+    const uint64_t i = r.u64();
+    const F3 K0 = r.f3();
+    const int32_t tri = r.i();
+    const float hu = r.f(), hv = r.f();
+    const F3 rdir = r.f3();
+    const int32_t tri0 = r.i();
+    const F3 p = r.f3();
+    const F3 n = r.f3();
+    const int32_t ei = r.i();
+    const float t = r.f();
+    Its its = post_intersect(S, tri, hu, hv);
     // sensor side: EvalImportance for B bounces, throughput 1
     float w = 0.0f;
     Its cur = its;
@@ -554,7 +695,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
     }
     w = w / (float)P.spp;
     if (w == 0.0f || !isfinite(w)) continue;
-    scatter_xdn(S, h0.tri, its.p, p, n, w, &ed, t, ed.f0, h.tri, h.u, h.v, dpos);
+    const EdgeDev ed = E.e[ei];
+    scatter_xdn(S, tri0, its.p, p, n, w, &ed, t, ed.f0, tri, hu, hv, dpos);
   }
 }
 
@@ -562,62 +704,105 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
 // Secondary edges, indirect illumination (BoundaryIndirect.cpp:38-192,
 // Edge.cpp:146-199)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBndWaves))) k_boundary_indirect(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
-                                                              uint64_t N, const float* dLdI, float* dpos) {
+// select: edge point and direction, the two rays from the edge point and
+// baseVal / pdf.
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBndWaves))) k_bnd_indirect_select(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
+                                                                uint64_t first, uint64_t end, BndQueue Q) {
+  __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
+  TraceCtx tc = make_trace_ctx_b(lds_stack, L);
+  const EdgeSet& E = S.sec;
+  const uint64_t stride = (uint64_t)L.grid * kBlock;
+  for (uint64_t i = first + (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < end; i += stride) {
+    bool keep = false;
+    Hit he, hs;
+    F3 dir, p, n;
+    float v0, t;
+    int32_t ei;
+    do {
+      Rnd4 rnd = rng4(P.seed, kRngIndirectEdge, i, 0u);
+      float pdf_e;
+      ei = sample_edge(E, rnd.a, &pdf_e);
+      const EdgeDev ed = E.e[ei];
+      EdgeGeo g = edge_geo(S, ed);
+      t = cdf_reuse(E.cdf, ei, rnd.a);
+      p = g.p0 + t * g.e1;
+      F3 edge = normalize(g.e1);
+      F3 edge2 = normalize(g.p2 - g.p0);
+      float pdf_d;
+      if (g.boundary) {
+        F3 nrm = cross(g.n0, edge);
+        F3 ft, fb;
+        coordinate_system(nrm, &ft, &fb);
+        float z = minf(1.0f - 2.0f * rnd.b, 1.0f - kEdgeEpsilon);
+        float rr = sqrtf(maxf(0.0f, 1.0f - z * z));
+        float sp, cp;
+        tr_sincos(2.0f * kPi * rnd.c, &sp, &cp);
+        dir = ((rr * cp) * ft + (rr * sp) * fb) + z * nrm;
+        pdf_d = 0.25f / kPi;
+      } else {
+        float phi0 = tr_acos(dot(g.n0, g.n1));
+        pdf_d = 0.25f / phi0;
+        F3 z = normalize(g.n0 + g.n1);
+        F3 y = normalize(cross(g.n0, z));
+        F3 x = cross(y, z);
+        float phi = (rnd.b - 0.5f) * phi0;
+        phi = minf(maxf(phi, -0.5f * phi0 + kEdgeEpsilon), 0.5f * phi0 - kEdgeEpsilon);
+        float sp, cp;
+        tr_sincos(phi, &sp, &cp);
+        F3 x1 = x * cp + z * sp;
+        bool m0 = rnd.c > 0.5f;
+        float bb = 4.0f * rnd.c - (m0 ? 3.0f : 1.0f);
+        float aa = sqrtf(1.0f - bb * bb) * (m0 ? -1.0f : 1.0f);
+        dir = x1 * aa + y * bb;
+      }
+      float pdf = pdf_e * pdf_d;
+      he = trace_ray<false>(S, tc, p, -dir, kShadowEpsilon, 1e32f);
+      if (he.tri < 0) break;
+      hs = trace_ray<false>(S, tc, p, dir, kShadowEpsilon, 1e32f);
+      if (hs.tri < 0) break;
+      Its ite = post_intersect(S, he.tri, he.u, he.v);
+      Its its = post_intersect(S, hs.tri, hs.u, hs.v);
+      float bv = base_val_secondary(its.p, ite.p, ite.ng, dir, p, edge, edge2, &n);
+      v0 = bv / pdf;
+      // with max_bounces <= 1 no path length fits; a zero or non-finite v0
+      // makes every contribution vanish or not finite
+      keep = P.max_bounces > 1 && v0 != 0.0f && isfinite(v0);
+    } while (false);
+    const int64_t s = wave_append(keep, Q.count);
+    if (s >= 0) {
+      QWriter w{Q, (uint32_t)s};
+      w.u64(i); w.i(he.tri); w.f(he.u); w.f(he.v); w.i(hs.tri); w.f(hs.u); w.f(hs.v); w.f3(dir); w.f(v0);
+      w.f3(p); w.f3(n); w.i(ei); w.f(t);
+    }
+  }
+}
+
+// eval: the emitter-side radiance after 1..B-1 bounces, the sensor-side
+// importance, their combination by total length and the scatter.
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBndWaves))) k_bnd_indirect_eval(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
+                                                              BndQueue Q, const float* dLdI, float* dpos) {
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
   const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
   const EdgeSet& E = S.sec;
   const int B = P.max_bounces;
-  const uint64_t stride = (uint64_t)L.grid * kBlock;
-  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
-    Rnd4 rnd = rng4(P.seed, kRngIndirectEdge, i, 0u);
-    float pdf_e;
-    int32_t ei = sample_edge(E, rnd.a, &pdf_e);
-    const EdgeDev ed = E.e[ei];
-    EdgeGeo g = edge_geo(S, ed);
-    float t = cdf_reuse(E.cdf, ei, rnd.a);
-    F3 p = g.p0 + t * g.e1;
-    F3 edge = normalize(g.e1);
-    F3 edge2 = normalize(g.p2 - g.p0);
-    F3 dir;
-    float pdf_d;
-    if (g.boundary) {
-      F3 nrm = cross(g.n0, edge);
-      F3 ft, fb;
-      coordinate_system(nrm, &ft, &fb);
-      float z = minf(1.0f - 2.0f * rnd.b, 1.0f - kEdgeEpsilon);
-      float rr = sqrtf(maxf(0.0f, 1.0f - z * z));
-      float sp, cp;
-      tr_sincos(2.0f * kPi * rnd.c, &sp, &cp);
-      dir = ((rr * cp) * ft + (rr * sp) * fb) + z * nrm;
-      pdf_d = 0.25f / kPi;
-    } else {
-      float phi0 = tr_acos(dot(g.n0, g.n1));
-      pdf_d = 0.25f / phi0;
-      F3 z = normalize(g.n0 + g.n1);
-      F3 y = normalize(cross(g.n0, z));
-      F3 x = cross(y, z);
-      float phi = (rnd.b - 0.5f) * phi0;
-      phi = minf(maxf(phi, -0.5f * phi0 + kEdgeEpsilon), 0.5f * phi0 - kEdgeEpsilon);
-      float sp, cp;
-      tr_sincos(phi, &sp, &cp);
-      F3 x1 = x * cp + z * sp;
-      bool m0 = rnd.c > 0.5f;
-      float bb = 4.0f * rnd.c - (m0 ? 3.0f : 1.0f);
-      float aa = sqrtf(1.0f - bb * bb) * (m0 ? -1.0f : 1.0f);
-      dir = x1 * aa + y * bb;
-    }
-    float pdf = pdf_e * pdf_d;
-    Hit he = trace_ray<false>(S, tc, p, -dir, kShadowEpsilon, 1e32f);
-    if (he.tri < 0) continue;
-    Hit hs = trace_ray<false>(S, tc, p, dir, kShadowEpsilon, 1e32f);
-    if (hs.tri < 0) continue;
-    Its ite = post_intersect(S, he.tri, he.u, he.v);
-    Its its = post_intersect(S, hs.tri, hs.u, hs.v);
-    F3 n;
-    float bv = base_val_secondary(its.p, ite.p, ite.ng, dir, p, edge, edge2, &n);
-    float v0 = bv / pdf;
+  const uint32_t n_q = *Q.count;
+  const uint32_t stride = (uint32_t)L.grid * kBlock;
+  for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < n_q; s += stride) {
+    QReader r{Q, s};
+    const uint64_t i = r.u64();
+    const int32_t tri_e = r.i();
+    const float ue = r.f(), ve = r.f();
+    const int32_t tri_s = r.i();
+    const float us = r.f(), vs = r.f();
+    const F3 dir = r.f3();
+    const float v0 = r.f();
+    const F3 p = r.f3();
+    const F3 n = r.f3();
+    const int32_t ei = r.i();
+    const float t = r.f();
+    Its ite = post_intersect(S, tri_e, ue, ve);
+    Its its = post_intersect(S, tri_s, us, vs);
     // emitter side: cumulative radiance after 1..B-1 bounces
     F3 cum[kMaxBounces];
     F3 per[kMaxBounces];
@@ -646,7 +831,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
     }
     w = w / (float)P.spp;
     if (w == 0.0f || !isfinite(w)) continue;
-    scatter_xdn(S, he.tri, its.p, p, n, w, &ed, t, ed.f0, hs.tri, hs.u, hs.v, dpos);
+    const EdgeDev ed = E.e[ei];
+    scatter_xdn(S, tri_e, its.p, p, n, w, &ed, t, ed.f0, tri_s, us, vs, dpos);
   }
 }
 
@@ -672,25 +858,51 @@ hipError_t launch_guide_build(const SceneDev& S, const LaunchCfg& L, uint32_t se
   return hipGetLastError();
 }
 
+uint64_t boundary_queue_bytes(int kind, uint64_t n_samples) {
+  if (kind == 1 || n_samples == 0) return 0;
+  const uint64_t nchunk = (n_samples + kBndChunk - 1) / kBndChunk;
+  const uint64_t cap = n_samples < kBndChunk ? n_samples : kBndChunk;
+  return ((uint64_t)kBndWords * cap + nchunk) * sizeof(uint32_t);
+}
+
 hipError_t launch_boundary(int kind, const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                            const LaunchCfg& L, const EdgeSet& prim, const GuideDev& guide, const float* dLdI,
-                           float* dpos, hipStream_t st) {
+                           float* dpos, void* queue, hipStream_t st) {
   BoundaryParams P{seed, max_bounces, spp, 0u};
   uint64_t N = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
-  if (kind == 0) {
-    const LaunchCfg Lk = fit_grid(k_boundary_primary, L);
-    hipLaunchKernelGGL(k_boundary_primary, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, prim, N, dLdI, dpos);
-  } else if (kind == 1) {
+  if (N == 0) return hipSuccess;
+  if (kind == 1) {
     const LaunchCfg Lk = fit_grid(k_boundary_pixel, L);
     hipLaunchKernelGGL(k_boundary_pixel, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, N, dLdI, dpos);
-  } else if (kind == 2) {
-    const LaunchCfg Lk = fit_grid(k_boundary_direct, L);
-    hipLaunchKernelGGL(k_boundary_direct, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, guide, N, dLdI, dpos);
-  } else {
-    const LaunchCfg Lk = fit_grid(k_boundary_indirect, L);
-    hipLaunchKernelGGL(k_boundary_indirect, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, N, dLdI, dpos);
+    return hipGetLastError();
   }
-  return hipGetLastError();
+  // select / eval per chunk of kBndChunk samples over one survivor queue;
+  // the chunks' counters are cleared by a kernel (no memset node when captured)
+  if (!queue) return hipErrorInvalidValue;
+  const uint64_t nchunk = (N + kBndChunk - 1) / kBndChunk;
+  const uint32_t cap = (uint32_t)(N < kBndChunk ? N : kBndChunk);
+  uint32_t* buf = (uint32_t*)queue;
+  uint32_t* counts = buf + (size_t)kBndWords * cap;
+  hipError_t e = launch_fill_zero((float*)counts, nchunk, st);
+  for (uint64_t c = 0; c < nchunk && e == hipSuccess; ++c) {
+    const uint64_t first = c * kBndChunk, end = first + kBndChunk < N ? first + kBndChunk : N;
+    const BndQueue Q{counts + c, buf, cap};
+    if (kind == 0) {
+      const LaunchCfg Ls = fit_grid(k_bnd_primary_select, L), Le = fit_grid(k_bnd_primary_eval, L);
+      hipLaunchKernelGGL(k_bnd_primary_select, dim3(Ls.grid), dim3(kBlock), 0, st, S, C, P, Ls, prim, first, end, Q);
+      hipLaunchKernelGGL(k_bnd_primary_eval, dim3(Le.grid), dim3(kBlock), 0, st, S, C, P, Le, prim, Q, dLdI, dpos);
+    } else if (kind == 2) {
+      const LaunchCfg Ls = fit_grid(k_bnd_direct_select, L), Le = fit_grid(k_bnd_direct_eval, L);
+      hipLaunchKernelGGL(k_bnd_direct_select, dim3(Ls.grid), dim3(kBlock), 0, st, S, C, P, Ls, guide, first, end, Q);
+      hipLaunchKernelGGL(k_bnd_direct_eval, dim3(Le.grid), dim3(kBlock), 0, st, S, C, P, Le, Q, dLdI, dpos);
+    } else {
+      const LaunchCfg Ls = fit_grid(k_bnd_indirect_select, L), Le = fit_grid(k_bnd_indirect_eval, L);
+      hipLaunchKernelGGL(k_bnd_indirect_select, dim3(Ls.grid), dim3(kBlock), 0, st, S, C, P, Ls, first, end, Q);
+      hipLaunchKernelGGL(k_bnd_indirect_eval, dim3(Le.grid), dim3(kBlock), 0, st, S, C, P, Le, Q, dLdI, dpos);
+    }
+    e = hipGetLastError();
+  }
+  return e;
 }
 
 }  // namespace trc

@@ -55,6 +55,8 @@ struct trc_scene {
   // guiding of the direct boundary term: per-sample values, leaf weights,
   // leaf CDF and the weight integral
   DevBuf guide_buf;
+  // survivor queue of the primary / direct / indirect boundary kernels
+  DevBuf bq_buf;
   // adjoint of the vertex normals [V][3] (kept zero between calls), present
   // when a differentiable mesh is smooth-shaded
   float* vn_adj = nullptr;
@@ -201,10 +203,14 @@ constexpr int kSumChunk = 1024;
 // Builds the guide on `st`; g.enabled is ignored.
 hipError_t launch_guide_build(const SceneDev& S, const LaunchCfg& L, uint32_t seed, const GuideDev& g,
                               hipStream_t st);
-// kind: 0 primary edges, 1 pixel boundary, 2 secondary direct, 3 secondary indirect
+// Bytes of the survivor queue launch_boundary needs for `n_samples` samples
+// of `kind` (0 for the pixel boundary, which runs in one pass).
+uint64_t boundary_queue_bytes(int kind, uint64_t n_samples);
+// kind: 0 primary edges, 1 pixel boundary, 2 secondary direct, 3 secondary
+// indirect; queue: boundary_queue_bytes(kind, spp * W * H) bytes of scratch.
 hipError_t launch_boundary(int kind, const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                            const LaunchCfg& L, const EdgeSet& prim, const GuideDev& guide, const float* dLdI,
-                           float* dpos, hipStream_t st);
+                           float* dpos, void* queue, hipStream_t st);
 // vn_adj: [V][3] zeroed scratch for smooth-shaded differentiable meshes or
 // null; it is folded into dpos and cleared again by a second kernel.
 hipError_t launch_render_adj(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,

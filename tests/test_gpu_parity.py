@@ -247,6 +247,35 @@ def test_boundary_gradients_match_oracle(gpu_ctx, oracle_mod, kind):
     ds.close()
 
 
+@pytest.mark.parametrize("kind", ["primary", "direct", "indirect"])
+def test_boundary_multi_chunk_matches_oracle(gpu_ctx, oracle_mod, kind):
+    """More samples than one survivor-queue chunk (2^22): 256 x 256 x 65 =
+    4,259,840 edge samples run as two select / eval chunks that share the
+    queue; the second chunk's sample indices continue the first's."""
+    torch = _torch()
+    make, cam_d, _, _, _, B = BOUNDARY_CASES[kind]
+    W = H = 256
+    spp = 65
+    assert W * H * spp > (1 << 22)
+    spec = make()
+    ds = _capi.DeviceScene(gpu_ctx, spec)
+    info = ds.info()
+    cam = _capi.make_camera(W, H, **cam_d)
+    opt = _capi.make_options(seed=31, max_bounces=B, spp=1, sppe=spp, sppse0=spp, sppse1=spp)
+    dl = np.random.default_rng(2).uniform(0.2, 1.0, (3, W * H)).astype(np.float32)
+    dpos = torch.zeros(info.num_vertices, 3, device="cuda")
+    ds.render_boundary(cam, opt, kind, torch.from_numpy(dl).cuda().data_ptr(), dpos.data_ptr())
+    torch.cuda.synchronize()
+    S = oracle_mod.Scene(spec)
+    ref = oracle_mod.boundary_adj(S, cam_d, W, H, 31, B, kind, spp, dl)
+    got = dpos.cpu().numpy().astype(np.float64)
+    mask = spec.diff_vertex_mask()
+    assert np.abs(ref[mask]).max() > 0
+    ok, e = _grad_close(got[mask], ref[mask])
+    assert ok, e
+    ds.close()
+
+
 @pytest.mark.parametrize("subdiv", [2, 4])
 def test_guided_direct_matches_oracle(gpu_ctx, oracle_mod, subdiv):
     """Guided sampling of the direct boundary term (AQ_distrb.h): guide build
