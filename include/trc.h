@@ -74,6 +74,14 @@ typedef struct trc_options {
   int32_t g_direct_spp;      /* guide samples per leaf, 1..4096 */
   float g_direct_thold;      /* accepted and ignored, as in the reference */
   float g_eps;               /* uniform mix in [0, 1]; the reference uses 0.01 */
+  /* Guide reuse.  No counterpart in the reference, which rebuilds the guide
+   * in every RenderD.  The guide's leaf weights depend on the geometry and
+   * the lights only, not on the camera, so the views of one scene can share
+   * one guide: with g_keep != 0 the guide is built from g_seed (not seed),
+   * kept on the scene, and reused by later calls with the same g_seed,
+   * g_direct_spp and g_eps until the scene changes.  0 = rebuild per call. */
+  int32_t g_keep;
+  uint32_t g_seed;
 } trc_options;
 
 typedef struct trc_scene_info {

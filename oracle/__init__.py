@@ -245,14 +245,15 @@ def boundary_adj(scene, cam, w, h, seed, max_bounces, kind, spp, dLdI, first=0, 
                  guide=None):
     """Gradient [V,3] (float64) of one boundary estimator (primary, pixel,
     direct, indirect) for samples [first, first + count) of spp * w * h.
-    guide = (g_spp, g_eps) enables guiding of the direct term."""
+    guide = (g_spp, g_eps) enables guiding of the direct term, with the
+    guide built from `seed`; (g_spp, g_eps, g_seed) builds it from g_seed."""
     L = lib()
     if not hasattr(L, "_bnd_decl"):
         P = ctypes.POINTER
-        L.oracle_boundary_adj.argtypes = [ctypes.c_void_p, P(ctypes.c_float), ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
-                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_float), ctypes.c_uint64,
-                                          ctypes.c_uint64, P(ctypes.c_double), ctypes.c_int, ctypes.c_int,
-                                          ctypes.c_int, ctypes.c_float]
+        L.oracle_boundary_adj_gs.argtypes = [ctypes.c_void_p, P(ctypes.c_float), ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_float),
+                                             ctypes.c_uint64, ctypes.c_uint64, P(ctypes.c_double), ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint32]
         L.oracle_guide.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_float, P(ctypes.c_float),
                                    P(ctypes.c_float), P(ctypes.c_float), P(ctypes.c_int)]
         L._bnd_decl = True
@@ -261,9 +262,11 @@ def boundary_adj(scene, cam, w, h, seed, max_bounces, kind, spp, dLdI, first=0, 
         count = spp * w * h
     dpos = np.zeros((scene.V, 3), np.float64)
     c = scene._cam(cam)
-    L.oracle_boundary_adj(scene.h, _p(c, ctypes.c_float), w, h, seed, max_bounces, _boundary_kinds()[kind], spp,
-                          _p(dLdI, ctypes.c_float), first, count, _p(dpos, ctypes.c_double), threads or default_threads(),
-                          1 if guide else 0, int(guide[0]) if guide else 0, float(guide[1]) if guide else 0.0)
+    g_seed = int(guide[2]) if guide and len(guide) > 2 else seed
+    L.oracle_boundary_adj_gs(scene.h, _p(c, ctypes.c_float), w, h, seed, max_bounces, _boundary_kinds()[kind], spp,
+                             _p(dLdI, ctypes.c_float), first, count, _p(dpos, ctypes.c_double),
+                             threads or default_threads(), 1 if guide else 0, int(guide[0]) if guide else 0,
+                             float(guide[1]) if guide else 0.0, g_seed)
     return dpos
 
 

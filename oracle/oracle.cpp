@@ -2040,9 +2040,9 @@ void oracle_path_twin(void* sp, const float* cam, int w, int h, uint32_t seed, i
 // Boundary gradients: kind 0 primary edges, 1 pixel boundary, 2 secondary
 // direct, 3 secondary indirect.  dpos [V][3] += d/dV of the boundary terms
 // for samples [first, first + count) of spp * W * H.
-void oracle_boundary_adj(void* sp, const float* cam, int w, int h, uint32_t seed, int max_bounces, int kind, int spp,
-                         const float* dLdI, uint64_t first, uint64_t count, double* dpos, int threads, int g_direct,
-                         int g_spp, float g_eps) {
+void oracle_boundary_adj_gs(void* sp, const float* cam, int w, int h, uint32_t seed, int max_bounces, int kind,
+                            int spp, const float* dLdI, uint64_t first, uint64_t count, double* dpos, int threads,
+                            int g_direct, int g_spp, float g_eps, uint32_t g_seed) {
   const Scene& s = *(Scene*)sp;
   Cam c = make_camera(cam, cam + 3, cam + 6, w, h, cam[9]);
   std::vector<EdgeRec> E;
@@ -2052,7 +2052,7 @@ void oracle_boundary_adj(void* sp, const float* cam, int w, int h, uint32_t seed
   std::unique_ptr<Dist1> D;
   if (kind != 1) D.reset(new Dist1(edge_lengths(E)));
   std::unique_ptr<Guide> G;
-  if (kind == 2 && g_direct) G = build_guide(s, E, *D, seed, g_spp, g_eps, threads);
+  if (kind == 2 && g_direct) G = build_guide(s, E, *D, g_seed, g_spp, g_eps, threads);
   const uint64_t N = (uint64_t)spp * w * h;
   int nt = std::max(1, threads);
   std::vector<std::vector<double>> tp(nt, std::vector<double>(3 * (size_t)s.V, 0.0));
@@ -2072,6 +2072,14 @@ void oracle_boundary_adj(void* sp, const float* cam, int w, int h, uint32_t seed
   });
   for (int t = 0; t < nt; ++t)
     for (size_t k = 0; k < tp[t].size(); ++k) dpos[k] += tp[t][k];
+}
+
+// The guide built from the samples' own seed (the reference's behaviour).
+void oracle_boundary_adj(void* sp, const float* cam, int w, int h, uint32_t seed, int max_bounces, int kind, int spp,
+                         const float* dLdI, uint64_t first, uint64_t count, double* dpos, int threads, int g_direct,
+                         int g_spp, float g_eps) {
+  oracle_boundary_adj_gs(sp, cam, w, h, seed, max_bounces, kind, spp, dLdI, first, count, dpos, threads, g_direct,
+                         g_spp, g_eps, seed);
 }
 
 // Guiding weights of the direct boundary term: poly [E_sec] (after the eps

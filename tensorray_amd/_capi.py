@@ -33,7 +33,8 @@ class trc_options(ctypes.Structure):
                 ("sppe", ctypes.c_int32), ("sppse0", ctypes.c_int32), ("sppse1", ctypes.c_int32),
                 ("sppe0", ctypes.c_int32), ("quiet", ctypes.c_int32), ("g_direct", ctypes.c_int32),
                 ("g_direct_depth", ctypes.c_int32), ("g_direct_max_size", ctypes.c_int32),
-                ("g_direct_spp", ctypes.c_int32), ("g_direct_thold", ctypes.c_float), ("g_eps", ctypes.c_float)]
+                ("g_direct_spp", ctypes.c_int32), ("g_direct_thold", ctypes.c_float), ("g_eps", ctypes.c_float),
+                ("g_keep", ctypes.c_int32), ("g_seed", ctypes.c_uint32)]
 
 
 class trc_job(ctypes.Structure):
@@ -297,9 +298,14 @@ def make_camera(width, height, pos=(0.0, 1.0, 3.0), target=(0.0, 0.0, 0.0), up=(
 
 
 def make_options(seed=1234, max_bounces=1, spp=1, sppe=0, sppse0=0, sppse1=0, sppe0=0, quiet=1, g_direct=False,
-                 g_direct_depth=0, g_direct_max_size=100000, g_direct_spp=16, g_direct_thold=0.01, g_eps=0.01):
-    """RenderOptions with the reference's defaults (Integrator.h:47-78)."""
+                 g_direct_depth=0, g_direct_max_size=100000, g_direct_spp=16, g_direct_thold=0.01, g_eps=0.01,
+                 g_keep=False, g_seed=0):
+    """RenderOptions with the reference's defaults (Integrator.h:47-78).
+    g_keep / g_seed: build the direct-term guide from g_seed and keep it on
+    the scene for later calls (views) until the scene changes."""
     o = trc_options()
+    o.g_keep = 1 if g_keep else 0
+    o.g_seed = g_seed
     o.g_direct = 1 if g_direct else 0
     o.g_direct_depth = g_direct_depth
     o.g_direct_max_size = g_direct_max_size

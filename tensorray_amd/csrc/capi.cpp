@@ -137,6 +137,7 @@ void free_scene_bufs(trc_scene* s) {
   s->prim_buf = DevBuf{};
   if (s->guide_buf.ptr) hipFree(s->guide_buf.ptr);
   s->guide_buf = DevBuf{};
+  s->guide_key.valid = false;
   if (s->bq_buf.ptr) hipFree(s->bq_buf.ptr);
   s->bq_buf = DevBuf{};
   s->vn_adj = nullptr;
@@ -1154,11 +1155,17 @@ trc_status trc_render_boundary(trc_scene* s, const trc_camera* cam, const trc_op
       const size_t nv = (size_t)n * (size_t)opt->g_direct_spp;
       const size_t nparts = ((size_t)n + kSumChunk - 1) / kSumChunk + 1;
       const size_t need = (nv + 2 * (size_t)n + 1 + nparts) * sizeof(float);
+      const uint32_t gseed = opt->g_keep ? opt->g_seed : opt->seed;
+      const trc_scene::GuideKey key{true, s->version, gseed, opt->g_direct_spp, n, opt->g_eps};
+      const trc_scene::GuideKey& k = s->guide_key;
+      const bool reuse = opt->g_keep && k.valid && k.version == key.version && k.seed == key.seed &&
+                         k.spp == key.spp && k.n == key.n && k.eps == key.eps && s->guide_buf.bytes >= need;
       if (s->guide_buf.bytes < need) {
         hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
         if (s->guide_buf.ptr) hipFree(s->guide_buf.ptr);
         s->guide_buf.ptr = nullptr;
         s->guide_buf.bytes = 0;
+        s->guide_key.valid = false;
         hip_check(hipMalloc(&s->guide_buf.ptr, need), "hipMalloc(guide)");
         s->guide_buf.bytes = need;
       }
@@ -1172,8 +1179,12 @@ trc_status trc_render_boundary(trc_scene* s, const trc_camera* cam, const trc_op
       guide.spp = opt->g_direct_spp;
       guide.eps = opt->g_eps;
       guide.enabled = 1;
-      LaunchCfg Lg = make_cfg(s->ctx, s, st, (long long)nv);
-      hip_check(launch_guide_build(s->dev, Lg, opt->seed, guide, st), "k_guide");
+      if (!reuse) {
+        s->guide_key.valid = false;
+        LaunchCfg Lg = make_cfg(s->ctx, s, st, (long long)nv);
+        hip_check(launch_guide_build(s->dev, Lg, gseed, guide, st), "k_guide");
+        if (opt->g_keep) s->guide_key = key;
+      }
     }
     LaunchCfg L = make_cfg(s->ctx, s, st, N);
     const size_t qb = (size_t)boundary_queue_bytes(kind, (uint64_t)N);

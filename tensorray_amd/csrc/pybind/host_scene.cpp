@@ -366,6 +366,8 @@ trc_options RenderOptions::to_trc() const {
   t.g_direct_spp = g_direct_spp;
   t.g_direct_thold = g_direct_thold;
   t.g_eps = g_eps;
+  t.g_keep = g_keep ? 1 : 0;
+  t.g_seed = g_seed;
   return t;
 }
 

@@ -169,6 +169,9 @@ struct RenderOptions {
   bool g_direct = false;
   int g_direct_depth = 0, g_direct_max_size = 100000, g_direct_spp = 16;
   float g_direct_thold = 0.01f, g_eps = 0.01f;
+  // guide reuse across views (trc_options.g_keep / g_seed); off as in the reference
+  bool g_keep = false;
+  uint32_t g_seed = 0;
   RenderOptions() = default;
   RenderOptions(int s, int b, int sp, int se, int s0, int s1, int e0)
       : seed(s), max_bounces(b), spp(sp), sppe(se), sppse0(s0), sppse1(s1), sppe0(e0),

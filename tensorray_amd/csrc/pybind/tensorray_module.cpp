@@ -211,7 +211,9 @@ PYBIND11_MODULE(TensorRay, m) {
       .def_readwrite("g_direct_max_size", &RenderOptions::g_direct_max_size)
       .def_readwrite("g_direct_spp", &RenderOptions::g_direct_spp)
       .def_readwrite("g_direct_thold", &RenderOptions::g_direct_thold)
-      .def_readwrite("g_eps", &RenderOptions::g_eps);
+      .def_readwrite("g_eps", &RenderOptions::g_eps)
+      .def_readwrite("g_keep", &RenderOptions::g_keep)
+      .def_readwrite("g_seed", &RenderOptions::g_seed);
 
   py::class_<Integrator>(m, "Integrator");
   py::class_<PathTracer, Integrator>(m, "PathTracer")

@@ -55,6 +55,14 @@ struct trc_scene {
   // guiding of the direct boundary term: per-sample values, leaf weights,
   // leaf CDF and the weight integral
   DevBuf guide_buf;
+  // the guide in guide_buf was kept (trc_options.g_keep) for this key
+  struct GuideKey {
+    bool valid = false;
+    uint64_t version = 0;
+    uint32_t seed = 0;
+    int32_t spp = 0, n = 0;
+    float eps = 0.0f;
+  } guide_key;
   // survivor queue of the primary / direct / indirect boundary kernels
   DevBuf bq_buf;
   // adjoint of the vertex normals [V][3] (kept zero between calls), present

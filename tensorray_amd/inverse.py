@@ -44,6 +44,7 @@ class Setup:
     sppse1: int = 0                 # indirect secondary-edge samples per pixel
     sppe0: int = 0                  # pixel-boundary samples per pixel
     guiding: bool = True            # guiding of the direct secondary-edge term
+    share_guide: bool = True        # one guide per step for all views (trc_options.g_keep); 0: one per view
     target_spp: int = 256
     lr: float = 1e-3
     steps: int = 100
@@ -125,10 +126,13 @@ class InverseRendering:
 
     def _options(self, step, view):
         s = self.s
-        # one guiding leaf per secondary edge: room for every edge of the mesh
+        # one guiding leaf per secondary edge: room for every edge of the mesh.
+        # The guide does not depend on the camera: with share_guide the views
+        # of a step share one guide, built from the step's seed by the first.
         return _capi.make_options(seed=1234 + 1000 * step + view, max_bounces=s.depth - 1, spp=s.spp, sppe=s.sppe,
                                   sppse0=s.sppse0, sppse1=s.sppse1, sppe0=s.sppe0, g_direct=s.guiding,
-                                  g_direct_max_size=self.max_edges)
+                                  g_direct_max_size=self.max_edges, g_keep=s.share_guide,
+                                  g_seed=1234 + 1000 * step)
 
     def vertex_error(self):
         """Mean Euclidean distance of the mesh vertices to the target's."""

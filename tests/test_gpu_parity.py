@@ -312,6 +312,44 @@ def test_guided_direct_matches_oracle(gpu_ctx, oracle_mod, subdiv):
     ds.close()
 
 
+def test_guide_kept_across_views(gpu_ctx, oracle_mod):
+    """g_keep: the direct-term guide is built from g_seed, kept on the scene
+    and shared by later calls with other seeds and cameras (the views of one
+    optimisation step), then rebuilt once the vertices move."""
+    torch = _torch()
+    spec = scenes.sphere_scene(subdiv=2)
+    W = H = 48
+    ds = _capi.DeviceScene(gpu_ctx, spec)
+    info = ds.info()
+    S = oracle_mod.Scene(spec)
+    mask = spec.diff_vertex_mask()
+    dl = np.random.default_rng(2).uniform(0.2, 1.0, (3, W * H)).astype(np.float32)
+    dld = torch.from_numpy(dl).cuda()
+    dpos = torch.zeros(info.num_vertices, 3, device="cuda")
+    side = dict(pos=(1.5, 1.2, 2.5), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=45.0)
+
+    def check(scene_ref, cam_d, seed):
+        opt = _capi.make_options(seed=seed, max_bounces=2, spp=1, sppse0=8, g_direct=True, g_direct_spp=8,
+                                 g_eps=0.1, g_keep=True, g_seed=5)
+        dpos.zero_()
+        ds.render_boundary(_capi.make_camera(W, H, **cam_d), opt, "direct", dld.data_ptr(), dpos.data_ptr())
+        torch.cuda.synchronize()
+        ref = oracle_mod.boundary_adj(scene_ref, cam_d, W, H, seed, 2, "direct", 8, dl, guide=(8, 0.1, 5))
+        own = oracle_mod.boundary_adj(scene_ref, cam_d, W, H, seed, 2, "direct", 8, dl, guide=(8, 0.1))
+        assert not np.allclose(ref[mask], own[mask])  # the guide seed matters
+        ok, e = _grad_close(dpos.cpu().numpy().astype(np.float64)[mask], ref[mask])
+        assert ok, (seed, e)
+
+    check(S, scenes.CAMERA_DEFAULT, 31)   # builds the guide from g_seed 5
+    check(S, side, 77)                    # reuses it
+    moved = scenes.sphere_scene(subdiv=2)
+    moved.meshes[0].pos = (np.asarray(moved.meshes[0].pos, np.float64) * 1.02).astype(np.float32)
+    allpos = np.ascontiguousarray(np.concatenate([np.asarray(m.pos, np.float32) for m in moved.meshes]))
+    ds.set_vertices(allpos.ctypes.data, len(allpos), True)
+    check(oracle_mod.Scene(moved), side, 77)   # the scene changed: rebuilt
+    ds.close()
+
+
 @pytest.mark.parametrize("case", ["config1", "metric_small", "textured", "smooth", "conductor", "deepest"])
 def test_checkpointed_adjoint(gpu_ctx, oracle_mod, case):
     """trc_render_fwd_ckpt writes the same film as trc_render_fwd; the
