@@ -200,6 +200,21 @@ trc_status trc_render_stats(trc_scene* scene, const trc_camera* cam, const trc_o
  * Samples per pixel 0 skips the term.  Their forward value is zero. */
 trc_status trc_render_boundary(trc_scene* scene, const trc_camera* cam, const trc_options* opt, int32_t kind,
                                const float* dLdI, float* dpos, trc_stream stream);
+/* Forward-mode derivative image of one integrator along a scalar parameter
+ * theta (RenderD with RenderOptions.export_deriv, Integrator.h:106-121, and
+ * GradiantImageHandler::AccumulateDeriv, Integrator.cpp:338-373):
+ * dimg [3][H*W] += d(image)/d(theta).  The parameter enters through its
+ * tangents: tpos [V][3] = d(vertex position)/d(theta) and tmat [M][8] =
+ * d(material slot)/d(theta) (NULL: none; used by the interior term only),
+ * both device memory.  kind -1 = the interior path tracer (opt->spp;
+ * smooth-shaded differentiable meshes included through the vertex-normal
+ * tangents), 0..3 = the boundary terms of trc_render_boundary.  Texels
+ * carry no tangent.  Non-finite derivatives are dropped, as the reference
+ * filters its derivative images.  The sum over all five integrators is the
+ * derivative image the reference's validation compares with finite
+ * differences (validate_gpu.py:69-101). */
+trc_status trc_render_deriv(trc_scene* scene, const trc_camera* cam, const trc_options* opt, int32_t kind,
+                            const float* tpos, const float* tmat, float* dimg, trc_stream stream);
 /* Path checkpoint (adjoint without re-tracing).  The checkpoint holds, per
  * path, the outcome of every ray query -- hit triangle and barycentrics of
  * each vertex and the NEE visibility bits -- in

@@ -84,6 +84,8 @@ SIGNATURES = [
                                           ctypes.POINTER(ctypes.c_uint64), vp]),
     ("trc_render_boundary", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options),
                                              ctypes.c_int32, vp, vp, vp]),
+    ("trc_render_deriv", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options),
+                                          ctypes.c_int32, vp, vp, vp, vp]),
     ("trc_render_adj", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options), vp, vp, vp,
                                         vp, vp]),
     ("trc_checkpoint_bytes", ctypes.c_int64, [ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options)]),
@@ -461,6 +463,15 @@ class DeviceScene:
         k = self.BOUNDARY_KINDS[kind] if isinstance(kind, str) else int(kind)
         self.ctx.check(self.ctx.lib.trc_render_boundary(self.h, ctypes.byref(cam), ctypes.byref(opt), k, dldi_ptr,
                                                         dpos_ptr, _stream_ptr(stream)), "trc_render_boundary")
+
+    DERIV_KINDS = {"interior": -1, "primary": 0, "pixel": 1, "direct": 2, "indirect": 3}
+
+    def render_deriv(self, cam, opt, kind, tpos_ptr, tmat_ptr, dimg_ptr, stream=None):
+        """dimg [3][H*W] += derivative image of integrator `kind` along the
+        tangents tpos [V][3] (and tmat [M][8] or None) -- device pointers."""
+        k = self.DERIV_KINDS[kind] if isinstance(kind, str) else int(kind)
+        self.ctx.check(self.ctx.lib.trc_render_deriv(self.h, ctypes.byref(cam), ctypes.byref(opt), k, tpos_ptr,
+                                                     tmat_ptr, dimg_ptr, _stream_ptr(stream)), "trc_render_deriv")
 
     def close(self):
         if self.h is not None and self.h.value:

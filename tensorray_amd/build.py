@@ -63,7 +63,9 @@ def build_lib(jobs=8):
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
+    # --no-undefined: a missing definition fails the build, not the first dlopen
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp] + objs +
+         ["-lpthread"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -1104,13 +1104,13 @@ trc_status trc_render_stats(trc_scene* s, const trc_camera* cam, const trc_optio
   });
 }
 
-trc_status trc_render_boundary(trc_scene* s, const trc_camera* cam, const trc_options* opt, int32_t kind,
-                               const float* dLdI, float* dpos, trc_stream stream) {
-  if (!s) return TRC_ERR_INVALID;
-  return guarded(s->ctx, [&] {
+// One boundary integrator: gradient mode (dLdI, dpos) or, with dimg, the
+// derivative image along tpos.  Runs inside guarded().
+static void boundary_call(trc_scene* s, const trc_camera* cam, const trc_options* opt, int32_t kind,
+                          const float* dLdI, float* dpos, const float* tpos, float* dimg, trc_stream stream) {
+  {
     check_render_common(s, opt);
     CameraDev C = to_cam(cam);
-    if (!dLdI || !dpos) throw Fail(TRC_ERR_INVALID, "dLdI / dpos is NULL");
     if (kind < 0 || kind > 3) throw Fail(TRC_ERR_INVALID, "unknown boundary kind");
     const int spp = kind == 0 ? opt->sppe : kind == 1 ? opt->sppe0 : kind == 2 ? opt->sppse0 : opt->sppse1;
     if (spp < 0) throw Fail(TRC_ERR_INVALID, "negative boundary spp");
@@ -1196,10 +1196,45 @@ trc_status trc_render_boundary(trc_scene* s, const trc_camera* cam, const trc_op
       hip_check(hipMalloc(&s->bq_buf.ptr, qb), "hipMalloc(boundary queue)");
       s->bq_buf.bytes = qb;
     }
-    hip_check(launch_boundary(kind, s->dev, C, opt->seed, opt->max_bounces, spp, L, prim, guide, dLdI, dpos,
-                              s->bq_buf.ptr, st),
+    hip_check(launch_boundary(kind, s->dev, C, opt->seed, opt->max_bounces, spp, L, prim, guide, dLdI, dpos, tpos,
+                              dimg, s->bq_buf.ptr, st),
               "k_boundary");
     if (kind == 0) hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  }
+}
+
+trc_status trc_render_boundary(trc_scene* s, const trc_camera* cam, const trc_options* opt, int32_t kind,
+                               const float* dLdI, float* dpos, trc_stream stream) {
+  if (!s) return TRC_ERR_INVALID;
+  return guarded(s->ctx, [&] {
+    if (!dLdI || !dpos) throw Fail(TRC_ERR_INVALID, "dLdI / dpos is NULL");
+    boundary_call(s, cam, opt, kind, dLdI, dpos, nullptr, nullptr, stream);
+  });
+}
+
+trc_status trc_render_deriv(trc_scene* s, const trc_camera* cam, const trc_options* opt, int32_t kind,
+                            const float* tpos, const float* tmat, float* dimg, trc_stream stream) {
+  if (!s) return TRC_ERR_INVALID;
+  return guarded(s->ctx, [&] {
+    if (!tpos || !dimg) throw Fail(TRC_ERR_INVALID, "tpos / dimg is NULL");
+    if (kind < -1 || kind > 3) throw Fail(TRC_ERR_INVALID, "unknown integrator kind");
+    if (kind >= 0) {
+      boundary_call(s, cam, opt, kind, nullptr, nullptr, tpos, dimg, stream);
+      return;
+    }
+    check_render(s, opt);
+    CameraDev C = to_cam(cam);
+    hipStream_t st = (hipStream_t)stream;
+    const long long np = (long long)opt->spp * C.width * C.height;
+    LaunchCfg L = make_cfg(s->ctx, s, st, np);
+    // vertex-normal tangents only where a differentiable mesh is smooth
+    float* tvn = nullptr;
+    if (s->vn_adj)
+      hip_check(hipMallocAsync((void**)&tvn, 3 * (size_t)s->dev.n_verts * sizeof(float), st), "hipMallocAsync(tvn)");
+    const hipError_t e = launch_render_deriv(s->dev, C, opt->seed, opt->max_bounces, opt->spp, L, tpos, tmat, tvn,
+                                             dimg, st);
+    if (tvn) hipFreeAsync(tvn, st);
+    hip_check(e, "k_render_deriv");
   });
 }
 

@@ -272,19 +272,60 @@ struct FrameAdj {
   F3 t, b, ns;
 };
 
-// Texture gradient (BSDF::DiffTexture, BSDF.cpp:350-356): the nearest-texel
-// lookup scatters its adjoint into one texel.
-__device__ inline void tex_scatter(const SceneDev& S, const Material& m, const Its& s, F3 g, float* dtex) {
-  if (!dtex) return;
-  const TexDesc& t = S.texs[m.tex];
-  float* d = dtex + t.grad_off + 3 * tex_index(t, s.tu, s.tv);
-  if (g.x != 0.0f) atomicAdd(&d[0], g.x);
-  if (g.y != 0.0f) atomicAdd(&d[1], g.y);
-  if (g.z != 0.0f) atomicAdd(&d[2], g.z);
-}
+// Where the backward sweep sends the adjoints of the scene parameters.
+// GradSink: float atomics into the gradient buffers -- vertex positions
+// dpos [V][3], the block's material accumulators, texels (BSDF::DiffTexture,
+// BSDF.cpp:350-356: the nearest-texel lookup scatters into one texel) and
+// the vertex-normal adjoints of smooth-shaded meshes.
+struct GradSink {
+  float* dpos;
+  float* mat_grad;
+  float* adj_vn;
+  float* dtex;
+  __device__ bool has_vn() const { return adj_vn != nullptr; }
+  __device__ void pos(uint32_t v, F3 a) {
+    if (a.x != 0.0f) atomicAdd(&dpos[3 * v], a.x);
+    if (a.y != 0.0f) atomicAdd(&dpos[3 * v + 1], a.y);
+    if (a.z != 0.0f) atomicAdd(&dpos[3 * v + 2], a.z);
+  }
+  __device__ void mat(int i, float g) { atomicAdd(&mat_grad[i], g); }
+  __device__ void tex(const SceneDev& S, const Material& m, const Its& s, F3 g) {
+    if (!dtex) return;
+    const TexDesc& t = S.texs[m.tex];
+    float* d = dtex + t.grad_off + 3 * tex_index(t, s.tu, s.tv);
+    if (g.x != 0.0f) atomicAdd(&d[0], g.x);
+    if (g.y != 0.0f) atomicAdd(&d[1], g.y);
+    if (g.z != 0.0f) atomicAdd(&d[2], g.z);
+  }
+  __device__ void vn(uint32_t v, F3 c) {
+    if (c.x != 0.0f) atomicAdd(&adj_vn[3 * v], c.x);
+    if (c.y != 0.0f) atomicAdd(&adj_vn[3 * v + 1], c.y);
+    if (c.z != 0.0f) atomicAdd(&adj_vn[3 * v + 2], c.z);
+  }
+};
 
+// TangentSink: forward-mode derivative of one path's contribution along a
+// parameter tangent -- tpos [V][3] = d position / d theta, tmat [M][8] =
+// d material slot / d theta (or null), tvn [V][3] = d vertex normal / d theta
+// (or null): every adjoint is dotted with its tangent and summed into acc.
+// Texels carry no tangent.
+struct TangentSink {
+  const float* tpos;
+  const float* tmat;
+  const float* tvn;
+  float acc;
+  __device__ bool has_vn() const { return tvn != nullptr; }
+  __device__ void pos(uint32_t v, F3 a) { acc += dot(a, load3(tpos, v)); }
+  __device__ void mat(int i, float g) {
+    if (tmat) acc += g * tmat[i];
+  }
+  __device__ void tex(const SceneDev&, const Material&, const Its&, F3) {}
+  __device__ void vn(uint32_t v, F3 c) { acc += dot(c, load3(tvn, v)); }
+};
+
+template <class Sink>
 __device__ inline void bsdf_eval_vjp(const SceneDev& S, const Its& s, F3 wo_w, F3 wi_w, F3 adj_f,
-                                     F3& adj_wo_w, F3& adj_wi_w, FrameAdj& fa, float* mat_grad, float* dtex) {
+                                     F3& adj_wo_w, F3& adj_wi_w, FrameAdj& fa, Sink& sk) {
   const Material& m = S.mats[s.mat];
   F3 wo = to_local(s, wo_w), wi = to_local(s, wi_w);
   F3 awo{0.0f, 0.0f, 0.0f}, awi{0.0f, 0.0f, 0.0f};
@@ -293,11 +334,11 @@ __device__ inline void bsdf_eval_vjp(const SceneDev& S, const Its& s, F3 wo_w, F
     if (inner == 0.0f) return;
     F3 ga = adj_f * inner;
     if (m.tex >= 0) {
-      tex_scatter(S, m, s, ga, dtex);
+      sk.tex(S, m, s, ga);
     } else if (s.mat < kMaxMatGrad) {
-      atomicAdd(&mat_grad[s.mat * kMatGradStride + 1], ga.x);
-      atomicAdd(&mat_grad[s.mat * kMatGradStride + 2], ga.y);
-      atomicAdd(&mat_grad[s.mat * kMatGradStride + 3], ga.z);
+      sk.mat(s.mat * kMatGradStride + 1, ga.x);
+      sk.mat(s.mat * kMatGradStride + 2, ga.y);
+      sk.mat(s.mat * kMatGradStride + 3, ga.z);
     }
     return;  // no directional dependence
   }
@@ -308,10 +349,10 @@ __device__ inline void bsdf_eval_vjp(const SceneDev& S, const Its& s, F3 wo_w, F
     roughconductor_inner_vjp(wo, wi, m.roughness, F3{m.eta[0], m.eta[1], m.eta[2]}, F3{m.k[0], m.k[1], m.k[2]},
                              adj_f, &awo, &awi, &ar, &aeta, &ak);
     if (s.mat < kMaxMatGrad) {
-      float* g = &mat_grad[s.mat * kMatGradStride];
-      atomicAdd(&g[0], ar);
-      atomicAdd(&g[1], aeta.x); atomicAdd(&g[2], aeta.y); atomicAdd(&g[3], aeta.z);
-      atomicAdd(&g[4], ak.x); atomicAdd(&g[5], ak.y); atomicAdd(&g[6], ak.z);
+      const int gb = s.mat * kMatGradStride;
+      sk.mat(gb, ar);
+      sk.mat(gb + 1, aeta.x); sk.mat(gb + 2, aeta.y); sk.mat(gb + 3, aeta.z);
+      sk.mat(gb + 4, ak.x); sk.mat(gb + 5, ak.y); sk.mat(gb + 6, ak.z);
     }
   } else {
     F3 kd = mat_albedo(S, m, s);
@@ -319,14 +360,14 @@ __device__ inline void bsdf_eval_vjp(const SceneDev& S, const Its& s, F3 wo_w, F
     float ar;
     F3 akd, aF0;
     microfacet_inner_vjp(wo, wi, m.roughness, kd, F0, adj_f, &awo, &awi, &ar, &akd, &aF0);
-    if (m.tex >= 0) tex_scatter(S, m, s, akd, dtex);
+    if (m.tex >= 0) sk.tex(S, m, s, akd);
     if (s.mat < kMaxMatGrad) {
-      float* g = &mat_grad[s.mat * kMatGradStride];
-      atomicAdd(&g[0], ar);
+      const int gb = s.mat * kMatGradStride;
+      sk.mat(gb, ar);
       if (m.tex < 0) {
-        atomicAdd(&g[1], akd.x); atomicAdd(&g[2], akd.y); atomicAdd(&g[3], akd.z);
+        sk.mat(gb + 1, akd.x); sk.mat(gb + 2, akd.y); sk.mat(gb + 3, akd.z);
       }
-      atomicAdd(&g[4], aF0.x); atomicAdd(&g[5], aF0.y); atomicAdd(&g[6], aF0.z);
+      sk.mat(gb + 4, aF0.x); sk.mat(gb + 5, aF0.y); sk.mat(gb + 6, aF0.z);
     }
   }
   // local = (v.t, v.b, v.ns)
@@ -342,17 +383,14 @@ __device__ inline F3 bsdf_value(const SceneDev& S, const Its& s, F3 wo_w, F3 wi_
   return bsdf_eval(S, s, wo_w, wi_w);
 }
 
-__device__ inline void tri_flush(const SceneDev& S, int32_t tri, const TriAdj& a, float* dpos) {
+template <class Sink>
+__device__ inline void tri_flush(const SceneDev& S, int32_t tri, const TriAdj& a, Sink& sk) {
   if (tri < 0) return;
   if (!(S.shade[tri].flags & kTriDiff)) return;
   uint32_t v[3] = {S.idx[3 * tri], S.idx[3 * tri + 1], S.idx[3 * tri + 2]};
   F3 aa[3] = {a.a0, a.a1, a.a2};
 #pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    if (aa[j].x != 0.0f) atomicAdd(&dpos[3 * v[j]], aa[j].x);
-    if (aa[j].y != 0.0f) atomicAdd(&dpos[3 * v[j] + 1], aa[j].y);
-    if (aa[j].z != 0.0f) atomicAdd(&dpos[3 * v[j] + 2], aa[j].z);
-  }
+  for (int j = 0; j < 3; ++j) sk.pos(v[j], aa[j]);
 }
 
 __device__ inline void tri_verts(const SceneDev& S, int32_t tri, F3& p0, F3& p1, F3& p2) {
@@ -361,9 +399,11 @@ __device__ inline void tri_verts(const SceneDev& S, int32_t tri, F3& p0, F3& p1,
   p2 = load3(S.pos, S.idx[3 * tri + 2]);
 }
 
-// Backward sweep over a recorded path.  g = dL/dI of the pixel times 1/spp.
+// Backward sweep over a recorded path.  g = dL/dI of the pixel times 1/spp;
+// the parameter adjoints go to `sk` (GradSink or TangentSink).
+template <class Sink>
 __device__ void path_backward(const SceneDev& S, const CameraDev& C, const PathParams& P, const PathRec& R,
-                              F3 g, float* dpos, float* mat_grad, float* adj_vn, float* dtex) {
+                              F3 g, Sink& sk) {
   const F3 zero{0.0f, 0.0f, 0.0f};
   const int K = R.K;
   if (K < 0) return;
@@ -424,7 +464,7 @@ __device__ void path_backward(const SceneDev& S, const CameraDev& C, const PathP
         float sv = dot(wn, f);
         float adj_absSh = sv * GL, adj_GL = sv * absSh, adj_JL = sv * absSh * GL;
         F3 adj_wl = zero;
-        bsdf_eval_vjp(S, its, wo, wl, adj_f, adj_wo, adj_wl, fa, mat_grad, dtex);
+        bsdf_eval_vjp(S, its, wo, wl, adj_f, adj_wo, adj_wl, fa, sk);
         float adj_dsh = adj_absSh * sgnf(dsh);
         fa.ns += adj_dsh * wl;
         adj_wl += adj_dsh * its.ns;
@@ -439,7 +479,7 @@ __device__ void path_backward(const SceneDev& S, const CameraDev& C, const PathP
           tri_adj_zero(la);
           la.a0 += lbw * adj_wr; la.a1 += lbu * adj_wr; la.a2 += lbv * adj_wr;
           face_vjp(lg, adj_nL, adj_JL / lg.area, la);
-          tri_flush(S, lt, la, dpos);
+          tri_flush(S, lt, la, sk);
         }
       }
     }
@@ -466,7 +506,7 @@ __device__ void path_backward(const SceneDev& S, const CameraDev& C, const PathP
       float sv = dot(gT, f) * inv_pp;
       float adj_absc = sv * G, adj_G = sv * absc, adj_J = sv * absc * G;
       F3 adj_d = zero;
-      bsdf_eval_vjp(S, its, wo, d, adj_f, adj_wo, adj_d, fa, mat_grad, dtex);
+      bsdf_eval_vjp(S, its, wo, d, adj_f, adj_wo, adj_d, fa, sk);
       float adj_dc = adj_absc * sgnf(dc);
       fa.ns += adj_dc * d;
       adj_d += adj_dc * its.ns;
@@ -515,17 +555,13 @@ __device__ void path_backward(const SceneDev& S, const CameraDev& C, const PathP
     const uint32_t tflags = S.shade[ti].flags;
     if (!(tflags & kTriSmooth)) {
       adj_ng += adj_ns;  // flat shading: ns = ng
-    } else if (adj_vn && (tflags & kTriDiff)) {
+    } else if (sk.has_vn() && (tflags & kTriDiff)) {
       // ns = w n0 + u n1 + v n2 (vertex normals, not renormalized)
       const uint32_t k0 = S.idx[3 * ti], k1 = S.idx[3 * ti + 1], k2 = S.idx[3 * ti + 2];
       const F3 c[3] = {w * adj_ns, u * adj_ns, v * adj_ns};
       const uint32_t ks[3] = {k0, k1, k2};
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        if (c[j].x != 0.0f) atomicAdd(&adj_vn[3 * ks[j]], c[j].x);
-        if (c[j].y != 0.0f) atomicAdd(&adj_vn[3 * ks[j] + 1], c[j].y);
-        if (c[j].z != 0.0f) atomicAdd(&adj_vn[3 * ks[j] + 2], c[j].z);
-      }
+      for (int j = 0; j < 3; ++j) sk.vn(ks[j], c[j]);
     }
     A_cur.a0 += w * adj_x; A_cur.a1 += u * adj_x; A_cur.a2 += v * adj_x;
     face_vjp(fg, adj_ng, adj_area, A_cur);
@@ -534,12 +570,12 @@ __device__ void path_backward(const SceneDev& S, const CameraDev& C, const PathP
       A_prev.a0 += ww * adj_xprev; A_prev.a1 += uu * adj_xprev; A_prev.a2 += vv * adj_xprev;
     }
     // triangle i+1 is complete
-    tri_flush(S, tri_next, A_next, dpos);
+    tri_flush(S, tri_next, A_next, sk);
     tri_next = ti;
     A_next = A_cur;
     A_cur = A_prev;
   }
-  tri_flush(S, tri_next, A_next, dpos);
+  tri_flush(S, tri_next, A_next, sk);
 }
 
 // Adjoint of the smooth-shading vertex normals (TriangleMesh::
@@ -567,8 +603,35 @@ __device__ inline void vertex_normal_backward(const SceneDev& S, uint32_t vtx, f
     TriAdj ta;
     tri_adj_zero(ta);
     face_vjp(face_geom(p0, p1, p2), adj_a, 0.0f, ta);
-    tri_flush(S, t, ta, dpos);
+    GradSink sk{dpos, nullptr, nullptr, nullptr};
+    tri_flush(S, t, ta, sk);
   }
+}
+
+// Tangent of the smooth-shading vertex normal of `vtx` along tpos (forward
+// mode of n_v = normalize(a / c), a = sum of the unit normals of the c
+// incident faces, each n_f = normalize(cross(p1 - p0, p2 - p0))).
+__device__ inline F3 vertex_normal_tangent(const SceneDev& S, uint32_t vtx, const float* tpos) {
+  const uint32_t b = S.vf_off[vtx], e = S.vf_off[vtx + 1];
+  const F3 zero{0.0f, 0.0f, 0.0f};
+  if (b == e) return zero;
+  F3 a = zero, da = zero;
+  for (uint32_t j = b; j < e; ++j) {
+    const int32_t t = (int32_t)S.vf_list[j];
+    F3 p0, p1, p2;
+    tri_verts(S, t, p0, p1, p2);
+    const F3 t0 = load3(tpos, S.idx[3 * t]), t1 = load3(tpos, S.idx[3 * t + 1]), t2 = load3(tpos, S.idx[3 * t + 2]);
+    const FaceGeom g = face_geom(p0, p1, p2);
+    const F3 dc = cross(t1 - t0, g.e1) + cross(g.e0, t2 - t0);
+    const TriShade& sh = S.shade[t];
+    a = a + F3{sh.n[0], sh.n[1], sh.n[2]};
+    da = da + (dc - g.n * dot(g.n, dc)) / g.len;
+  }
+  const float c = (float)(e - b);
+  const F3 q = a / c, dq = da / c;
+  const float l = length(q);
+  const F3 nv = q / l;
+  return (dq - nv * dot(nv, dq)) / l;
 }
 
 }  // namespace trc

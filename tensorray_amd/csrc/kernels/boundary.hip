@@ -225,12 +225,11 @@ __device__ inline XdnAdj x_dot_n_vjp(F3 P0, F3 P1, F3 P2, F3 org, F3 through, F3
   return a;
 }
 
-__device__ inline void vtx_add(const SceneDev& S, uint32_t v, int32_t tri, F3 g, float* dpos) {
+template <class Sink>
+__device__ inline void vtx_add(const SceneDev& S, uint32_t v, int32_t tri, F3 g, Sink& sk) {
   if (!(S.shade[tri].flags & kTriDiff)) return;
   if (!finite3(g)) return;
-  if (g.x != 0.0f) atomicAdd(&dpos[3 * v], g.x);
-  if (g.y != 0.0f) atomicAdd(&dpos[3 * v + 1], g.y);
-  if (g.z != 0.0f) atomicAdd(&dpos[3 * v + 2], g.z);
+  sk.pos(v, g);
 }
 
 __device__ inline F3 dLdI_at(const float* dLdI, uint32_t WH, int32_t pix) {
@@ -239,25 +238,35 @@ __device__ inline F3 dLdI_at(const float* dLdI, uint32_t WH, int32_t pix) {
 
 // Scatter of one sample: triangle of the RayIntersectAD (tri), edge point
 // (through = q0 + t (q1 - q0) when the edge is given) and material-form
-// origin (otri, ou, ov) when given.
+// origin (otri, ou, ov) when given.  GradSink: gradients of w * xDotN;
+// TangentSink with w = 1: the derivative of xDotN along the tangents.
+template <class Sink>
 __device__ void scatter_xdn(const SceneDev& S, int32_t tri, F3 org, F3 through, F3 n, float w, const EdgeDev* edge,
-                            float et, int32_t ef0, int32_t otri, float ou, float ov, float* dpos) {
+                            float et, int32_t ef0, int32_t otri, float ou, float ov, Sink& sk) {
   F3 P0, P1, P2;
   tri_verts(S, tri, P0, P1, P2);
   XdnAdj a = x_dot_n_vjp(P0, P1, P2, org, through, n, w);
-  vtx_add(S, S.idx[3 * tri], tri, a.P0, dpos);
-  vtx_add(S, S.idx[3 * tri + 1], tri, a.P1, dpos);
-  vtx_add(S, S.idx[3 * tri + 2], tri, a.P2, dpos);
+  vtx_add(S, S.idx[3 * tri], tri, a.P0, sk);
+  vtx_add(S, S.idx[3 * tri + 1], tri, a.P1, sk);
+  vtx_add(S, S.idx[3 * tri + 2], tri, a.P2, sk);
   if (edge) {
-    vtx_add(S, edge->v0, ef0, (1.0f - et) * a.through, dpos);
-    vtx_add(S, edge->v1, ef0, et * a.through, dpos);
+    vtx_add(S, edge->v0, ef0, (1.0f - et) * a.through, sk);
+    vtx_add(S, edge->v1, ef0, et * a.through, sk);
   }
   if (otri >= 0) {
     float ow = 1.0f - ou - ov;
-    vtx_add(S, S.idx[3 * otri], otri, ow * a.org, dpos);
-    vtx_add(S, S.idx[3 * otri + 1], otri, ou * a.org, dpos);
-    vtx_add(S, S.idx[3 * otri + 2], otri, ov * a.org, dpos);
+    vtx_add(S, S.idx[3 * otri], otri, ow * a.org, sk);
+    vtx_add(S, S.idx[3 * otri + 1], otri, ou * a.org, sk);
+    vtx_add(S, S.idx[3 * otri + 2], otri, ov * a.org, sk);
   }
+}
+
+// Derivative-image contribution v at a pixel (non-finite values dropped).
+__device__ inline void add_deriv(float* dimg, uint32_t WH, int32_t pix, F3 v) {
+  if (!finite3(v)) return;
+  if (v.x != 0.0f) atomicAdd(&dimg[pix], v.x);
+  if (v.y != 0.0f) atomicAdd(&dimg[WH + pix], v.y);
+  if (v.z != 0.0f) atomicAdd(&dimg[2 * WH + pix], v.z);
 }
 
 struct BoundaryParams {
@@ -376,8 +385,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
 }
 
 // eval: the radiance seen through the sensor point and the scatter.
+// kTangent: the derivative image along tpos goes to dimg instead.
+template <bool kTangent>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBndWaves))) k_bnd_primary_eval(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
-                                                             EdgeSet E, BndQueue Q, const float* dLdI, float* dpos) {
+                                                             EdgeSet E, BndQueue Q, const float* dLdI, float* dpos,
+                                                             const float* tpos, float* dimg) {
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
   const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
@@ -401,10 +413,17 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
     F3 rad = boundary_radiance(S, tc, its, rd, value, P.seed, kRngPrimaryEdge, i, P.max_bounces);
     F3 K = rad * bv / pdf;
     if (!(length(K) > 0.0f) || !finite3(K)) continue;
-    float w = dot(dLdI_at(dLdI, WH, pixel), K) / (float)P.spp;
-    if (w == 0.0f) continue;
     const EdgeDev ed = E.e[ei];
-    scatter_xdn(S, tri, eye, p, n, w, &ed, rb, ed.f0, -1, 0.0f, 0.0f, dpos);
+    if constexpr (kTangent) {
+      TangentSink sk{tpos, nullptr, nullptr, 0.0f};
+      scatter_xdn(S, tri, eye, p, n, 1.0f, &ed, rb, ed.f0, -1, 0.0f, 0.0f, sk);
+      add_deriv(dimg, WH, pixel, K * (sk.acc / (float)P.spp));
+    } else {
+      float w = dot(dLdI_at(dLdI, WH, pixel), K) / (float)P.spp;
+      if (w == 0.0f) continue;
+      GradSink sk{dpos, nullptr, nullptr, nullptr};
+      scatter_xdn(S, tri, eye, p, n, w, &ed, rb, ed.f0, -1, 0.0f, 0.0f, sk);
+    }
   }
 }
 
@@ -412,8 +431,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
 // Pixel boundary of the box filter (BoundaryPixel.cpp:36-159, Edge.cpp:377-409,
 // Camera.cpp:433-467).  4 antithetic positions per random number.
 // ---------------------------------------------------------------------------
+template <bool kTangent>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBndWaves))) k_boundary_pixel(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
-                                                           uint64_t N, const float* dLdI, float* dpos) {
+                                                           uint64_t N, const float* dLdI, float* dpos,
+                                                           const float* tpos, float* dimg) {
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
   const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
@@ -456,9 +477,16 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
     F3 rad = boundary_radiance(S, tc, its, rd, sd.value, P.seed, kRngPixelEdge, j, P.max_bounces);
     F3 K = rad * bv / pdf;
     if (!finite3(K)) continue;
-    float w = dot(dLdI_at(dLdI, WH, (int32_t)pix), K) / (float)P.spp;
-    if (w == 0.0f) continue;
-    scatter_xdn(S, h.tri, eye, p0, n, w, nullptr, 0.0f, -1, -1, 0.0f, 0.0f, dpos);
+    if constexpr (kTangent) {
+      TangentSink sk{tpos, nullptr, nullptr, 0.0f};
+      scatter_xdn(S, h.tri, eye, p0, n, 1.0f, nullptr, 0.0f, -1, -1, 0.0f, 0.0f, sk);
+      add_deriv(dimg, WH, (int32_t)pix, K * (sk.acc / (float)P.spp));
+    } else {
+      float w = dot(dLdI_at(dLdI, WH, (int32_t)pix), K) / (float)P.spp;
+      if (w == 0.0f) continue;
+      GradSink sk{dpos, nullptr, nullptr, nullptr};
+      scatter_xdn(S, h.tri, eye, p0, n, w, nullptr, 0.0f, -1, -1, 0.0f, 0.0f, sk);
+    }
   }
 }
 
@@ -660,8 +688,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
 }
 
 // eval: EvalImportance from the sensor-side hit for B bounces and the scatter.
+// kTangent: the derivative image along tpos goes to dimg instead.
+template <bool kTangent>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBndWaves))) k_bnd_direct_eval(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
-                                                            BndQueue Q, const float* dLdI, float* dpos) {
+                                                            BndQueue Q, const float* dLdI, float* dpos,
+                                                            const float* tpos, float* dimg) {
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
   const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
@@ -681,6 +712,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
     const int32_t ei = r.i();
     const float t = r.f();
     Its its = post_intersect(S, tri, hu, hv);
+    const EdgeDev ed = E.e[ei];
+    float ds = 0.0f;  // tangent mode: derivative of xDotN
+    if constexpr (kTangent) {
+      TangentSink sk{tpos, nullptr, nullptr, 0.0f};
+      scatter_xdn(S, tri0, its.p, p, n, 1.0f, &ed, t, ed.f0, tri, hu, hv, sk);
+      ds = sk.acc / (float)P.spp;
+      if (ds == 0.0f || !isfinite(ds)) continue;
+    }
     // sensor side: EvalImportance for B bounces, throughput 1
     float w = 0.0f;
     Its cur = its;
@@ -689,14 +728,19 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
       ImpStep st = importance_step(S, C, tc, cur, cdir, thr, rng4(P.seed, kRngDirectEdge, i, kSensorBlock0 + (uint32_t)b));
       if (st.contrib) {
         F3 v = K0 * st.value;
-        if (length(v) > 0.0f) w += dot(dLdI_at(dLdI, WH, st.pixel), v);
+        if (length(v) > 0.0f) {
+          if constexpr (kTangent) add_deriv(dimg, WH, st.pixel, v * ds);
+          else w += dot(dLdI_at(dLdI, WH, st.pixel), v);
+        }
       }
       if (!st.next) break;
     }
-    w = w / (float)P.spp;
-    if (w == 0.0f || !isfinite(w)) continue;
-    const EdgeDev ed = E.e[ei];
-    scatter_xdn(S, tri0, its.p, p, n, w, &ed, t, ed.f0, tri, hu, hv, dpos);
+    if constexpr (!kTangent) {
+      w = w / (float)P.spp;
+      if (w == 0.0f || !isfinite(w)) continue;
+      GradSink sk{dpos, nullptr, nullptr, nullptr};
+      scatter_xdn(S, tri0, its.p, p, n, w, &ed, t, ed.f0, tri, hu, hv, sk);
+    }
   }
 }
 
@@ -779,8 +823,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
 
 // eval: the emitter-side radiance after 1..B-1 bounces, the sensor-side
 // importance, their combination by total length and the scatter.
+// kTangent: the derivative image along tpos goes to dimg instead.
+template <bool kTangent>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBndWaves))) k_bnd_indirect_eval(SceneDev S, CameraDev C, BoundaryParams P, LaunchCfg L,
-                                                              BndQueue Q, const float* dLdI, float* dpos) {
+                                                              BndQueue Q, const float* dLdI, float* dpos,
+                                                              const float* tpos, float* dimg) {
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
   const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
@@ -803,6 +850,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
     const float t = r.f();
     Its ite = post_intersect(S, tri_e, ue, ve);
     Its its = post_intersect(S, tri_s, us, vs);
+    const EdgeDev ed = E.e[ei];
+    float ds = 0.0f;  // tangent mode: derivative of xDotN
+    if constexpr (kTangent) {
+      TangentSink sk{tpos, nullptr, nullptr, 0.0f};
+      scatter_xdn(S, tri_e, its.p, p, n, 1.0f, &ed, t, ed.f0, tri_s, us, vs, sk);
+      ds = sk.acc / (float)P.spp;
+      if (ds == 0.0f || !isfinite(ds)) continue;
+    }
     // emitter side: cumulative radiance after 1..B-1 bounces
     F3 cum[kMaxBounces];
     F3 per[kMaxBounces];
@@ -825,14 +880,19 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
       int ri = B - b - 2;
       if (st.contrib && ri >= 0) {
         F3 v = (v0 * cum[ri]) * st.value;
-        if (length(v) > 0.0f) w += dot(dLdI_at(dLdI, WH, st.pixel), v);
+        if (length(v) > 0.0f) {
+          if constexpr (kTangent) add_deriv(dimg, WH, st.pixel, v * ds);
+          else w += dot(dLdI_at(dLdI, WH, st.pixel), v);
+        }
       }
       if (!st.next) break;
     }
-    w = w / (float)P.spp;
-    if (w == 0.0f || !isfinite(w)) continue;
-    const EdgeDev ed = E.e[ei];
-    scatter_xdn(S, tri_e, its.p, p, n, w, &ed, t, ed.f0, tri_s, us, vs, dpos);
+    if constexpr (!kTangent) {
+      w = w / (float)P.spp;
+      if (w == 0.0f || !isfinite(w)) continue;
+      GradSink sk{dpos, nullptr, nullptr, nullptr};
+      scatter_xdn(S, tri_e, its.p, p, n, w, &ed, t, ed.f0, tri_s, us, vs, sk);
+    }
   }
 }
 
@@ -865,15 +925,15 @@ uint64_t boundary_queue_bytes(int kind, uint64_t n_samples) {
   return ((uint64_t)kBndWords * cap + nchunk) * sizeof(uint32_t);
 }
 
-hipError_t launch_boundary(int kind, const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
-                           const LaunchCfg& L, const EdgeSet& prim, const GuideDev& guide, const float* dLdI,
-                           float* dpos, void* queue, hipStream_t st) {
-  BoundaryParams P{seed, max_bounces, spp, 0u};
-  uint64_t N = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
-  if (N == 0) return hipSuccess;
+template <bool kTangent>
+static hipError_t launch_boundary_t(int kind, const SceneDev& S, const CameraDev& C, const BoundaryParams& P,
+                                    uint64_t N, const LaunchCfg& L, const EdgeSet& prim, const GuideDev& guide,
+                                    const float* dLdI, float* dpos, const float* tpos, float* dimg, void* queue,
+                                    hipStream_t st) {
   if (kind == 1) {
-    const LaunchCfg Lk = fit_grid(k_boundary_pixel, L);
-    hipLaunchKernelGGL(k_boundary_pixel, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, N, dLdI, dpos);
+    const LaunchCfg Lk = fit_grid(k_boundary_pixel<kTangent>, L);
+    hipLaunchKernelGGL(k_boundary_pixel<kTangent>, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, N, dLdI, dpos,
+                       tpos, dimg);
     return hipGetLastError();
   }
   // select / eval per chunk of kBndChunk samples over one survivor queue;
@@ -888,21 +948,35 @@ hipError_t launch_boundary(int kind, const SceneDev& S, const CameraDev& C, uint
     const uint64_t first = c * kBndChunk, end = first + kBndChunk < N ? first + kBndChunk : N;
     const BndQueue Q{counts + c, buf, cap};
     if (kind == 0) {
-      const LaunchCfg Ls = fit_grid(k_bnd_primary_select, L), Le = fit_grid(k_bnd_primary_eval, L);
+      const LaunchCfg Ls = fit_grid(k_bnd_primary_select, L), Le = fit_grid(k_bnd_primary_eval<kTangent>, L);
       hipLaunchKernelGGL(k_bnd_primary_select, dim3(Ls.grid), dim3(kBlock), 0, st, S, C, P, Ls, prim, first, end, Q);
-      hipLaunchKernelGGL(k_bnd_primary_eval, dim3(Le.grid), dim3(kBlock), 0, st, S, C, P, Le, prim, Q, dLdI, dpos);
+      hipLaunchKernelGGL(k_bnd_primary_eval<kTangent>, dim3(Le.grid), dim3(kBlock), 0, st, S, C, P, Le, prim, Q,
+                         dLdI, dpos, tpos, dimg);
     } else if (kind == 2) {
-      const LaunchCfg Ls = fit_grid(k_bnd_direct_select, L), Le = fit_grid(k_bnd_direct_eval, L);
+      const LaunchCfg Ls = fit_grid(k_bnd_direct_select, L), Le = fit_grid(k_bnd_direct_eval<kTangent>, L);
       hipLaunchKernelGGL(k_bnd_direct_select, dim3(Ls.grid), dim3(kBlock), 0, st, S, C, P, Ls, guide, first, end, Q);
-      hipLaunchKernelGGL(k_bnd_direct_eval, dim3(Le.grid), dim3(kBlock), 0, st, S, C, P, Le, Q, dLdI, dpos);
+      hipLaunchKernelGGL(k_bnd_direct_eval<kTangent>, dim3(Le.grid), dim3(kBlock), 0, st, S, C, P, Le, Q, dLdI, dpos,
+                         tpos, dimg);
     } else {
-      const LaunchCfg Ls = fit_grid(k_bnd_indirect_select, L), Le = fit_grid(k_bnd_indirect_eval, L);
+      const LaunchCfg Ls = fit_grid(k_bnd_indirect_select, L), Le = fit_grid(k_bnd_indirect_eval<kTangent>, L);
       hipLaunchKernelGGL(k_bnd_indirect_select, dim3(Ls.grid), dim3(kBlock), 0, st, S, C, P, Ls, first, end, Q);
-      hipLaunchKernelGGL(k_bnd_indirect_eval, dim3(Le.grid), dim3(kBlock), 0, st, S, C, P, Le, Q, dLdI, dpos);
+      hipLaunchKernelGGL(k_bnd_indirect_eval<kTangent>, dim3(Le.grid), dim3(kBlock), 0, st, S, C, P, Le, Q, dLdI,
+                         dpos, tpos, dimg);
     }
     e = hipGetLastError();
   }
   return e;
+}
+
+hipError_t launch_boundary(int kind, const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
+                           const LaunchCfg& L, const EdgeSet& prim, const GuideDev& guide, const float* dLdI,
+                           float* dpos, const float* tpos, float* dimg, void* queue, hipStream_t st) {
+  const BoundaryParams P{seed, max_bounces, spp, 0u};
+  const uint64_t N = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
+  if (N == 0) return hipSuccess;
+  if (dimg)
+    return launch_boundary_t<true>(kind, S, C, P, N, L, prim, guide, nullptr, nullptr, tpos, dimg, queue, st);
+  return launch_boundary_t<false>(kind, S, C, P, N, L, prim, guide, dLdI, dpos, nullptr, nullptr, queue, st);
 }
 
 }  // namespace trc

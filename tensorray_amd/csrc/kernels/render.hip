@@ -119,7 +119,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAd
     if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) continue;
     PathRec R;
     path_forward<true>(S, C, P, p, tc, &R);
-    path_backward(S, C, P, R, g, dpos, mat_grad, adj_vn, dtex);
+    GradSink sk{dpos, mat_grad, adj_vn, dtex};
+    path_backward(S, C, P, R, g, sk);
   }
   __syncthreads();
   if (dmat) {
@@ -151,7 +152,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAd
     if ((src.meta & 0xFFu) == 0u) continue;  // camera ray missed: no gradient
     PathRec R;
     path_forward<true>(S, C, P, p, src, &R);
-    path_backward(S, C, P, R, g, dpos, mat_grad, adj_vn, dtex);
+    GradSink sk{dpos, mat_grad, adj_vn, dtex};
+    path_backward(S, C, P, R, g, sk);
   }
   __syncthreads();
   if (dmat) {
@@ -159,6 +161,45 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAd
     for (int i = threadIdx.x; i < n; i += kBlock)
       if (mat_grad[i] != 0.0f) atomicAdd(&dmat[i], mat_grad[i]);
   }
+}
+
+// Derivative image of the interior integrator along one parameter tangent
+// (the reference's export_deriv: GradiantImageHandler::AccumulateDeriv,
+// Integrator.cpp:338-373, the forward-mode derivative of every path's
+// contribution): each path is replayed once and swept backwards three
+// times, with the seeds e_r, e_g, e_b / spp, into a TangentSink; the sums
+// are the derivatives of the path's RGB contribution and go to
+// dimg[c][pixel].  Non-finite derivatives are dropped, as the reference
+// filters its derivative images (Integrator.cpp:364-367).
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdjWaves))) k_render_deriv(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
+                                                         uint64_t npaths, const float* tpos, const float* tmat,
+                                                         const float* tvn, float* dimg) {
+  __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
+  TraceCtx tc = make_trace_ctx(lds_stack, L);
+  const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
+  const float inv_spp = 1.0f / (float)P.spp;
+  const uint64_t stride = (uint64_t)L.grid * kBlock;
+  for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < npaths; p += stride) {
+    const uint32_t pix = (uint32_t)(p % WH);
+    PathRec R;
+    path_forward<true>(S, C, P, p, tc, &R);
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+      const F3 g{c == 0 ? inv_spp : 0.0f, c == 1 ? inv_spp : 0.0f, c == 2 ? inv_spp : 0.0f};
+      TangentSink sk{tpos, tmat, tvn, 0.0f};
+      path_backward(S, C, P, R, g, sk);
+      if (sk.acc != 0.0f && isfinite(sk.acc)) atomicAdd(&dimg[(size_t)c * WH + pix], sk.acc);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_vertex_normal_tangent(SceneDev S, const float* tpos, float* tvn) {
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  if (v >= (uint32_t)S.n_verts) return;
+  const F3 d = vertex_normal_tangent(S, v, tpos);
+  tvn[3 * v] = d.x;
+  tvn[3 * v + 1] = d.y;
+  tvn[3 * v + 2] = d.z;
 }
 
 __global__ void __launch_bounds__(kBlock) k_vertex_normal_adj(SceneDev S, float* adj_vn, float* dpos) {
@@ -301,7 +342,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAd
       if ((src.meta & 0xFFu) == 0u) continue;
       PathRec R;
       path_forward<true>(S, C, P, p, src, &R);
-      path_backward(S, C, P, R, g, dpos, mat_grad, vn, dtex);
+      GradSink sk{dpos, mat_grad, vn, dtex};
+      path_backward(S, C, P, R, g, sk);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -448,6 +490,22 @@ hipError_t launch_render_adj(const SceneDev& S, const CameraDev& C, uint32_t see
   if (e != hipSuccess || !vn_adj || S.n_verts == 0) return e;
   const int grid = (S.n_verts + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_vertex_normal_adj, dim3(grid), dim3(kBlock), 0, st, S, vn_adj, dpos);
+  return hipGetLastError();
+}
+
+hipError_t launch_render_deriv(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
+                               const LaunchCfg& L, const float* tpos, const float* tmat, float* tvn, float* dimg,
+                               hipStream_t st) {
+  const PathParams P{seed, max_bounces, spp, 0u};
+  const uint64_t np = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
+  if (np == 0) return hipSuccess;
+  if (tvn) {
+    const unsigned nb = (unsigned)((S.n_verts + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_vertex_normal_tangent, dim3(nb), dim3(kBlock), 0, st, S, tpos, tvn);
+  }
+  const LaunchCfg Lk = fit_grid(k_render_deriv, L);
+  hipLaunchKernelGGL(k_render_deriv, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, np, tpos, tmat,
+                     (const float*)tvn, dimg);
   return hipGetLastError();
 }
 

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <set>
 #include <sstream>
 #include <unordered_map>
 
@@ -28,9 +29,23 @@ void env_create(int device) {
   env().rnd_seed = 0;
 }
 
-// Releases the device context and empties the parameter pool.
+// Scenes holding a device scene of the current context.
+static std::set<Scene*>& live_scenes() {
+  static std::set<Scene*> s;
+  return s;
+}
+
+// Releases the device context and empties the parameter pool.  Scenes that
+// outlive it lose their device state first (they must be configured again
+// under a new context); destroying it afterwards would touch the freed one.
 void env_release() {
   param_pool().clear();
+  for (Scene* s : live_scenes())
+    if (s->sc) {
+      trc_scene_destroy(s->sc);
+      s->sc = nullptr;
+    }
+  live_scenes().clear();
   if (env().ctx) trc_context_destroy(env().ctx);
   env().ctx = nullptr;
 }
@@ -92,7 +107,8 @@ void Tensorf::value_to(float* dst) const {
   if (dev) {
     hip_ok(hipMemcpy(dst, dev->p, (size_t)dev->n * sizeof(float), hipMemcpyDefault), "hipMemcpy");
   } else if (!host.empty()) {
-    any_copy(dst, host.data(), host.size() * sizeof(float));
+    const std::vector<float> v = resolved();
+    any_copy(dst, v.data(), v.size() * sizeof(float));
   }
 }
 
@@ -118,6 +134,62 @@ Tensorf Tensorf::vec3(float x, float y, float z, bool rg) {
   t.shape = TensorShape({1}, kVec3);
   t.host = {x, y, z};
   t.requires_grad = rg;
+  return t;
+}
+
+static const Param* scalar_param(int i) {
+  if (i < 0 || i >= (int)param_pool().size()) return nullptr;
+  const Param& p = param_pool()[i];
+  return p.kind == kParamScalar && p.value.size() == 1 ? &p : nullptr;
+}
+
+std::vector<float> Tensorf::resolved() const {
+  if (const Param* p = scalar_param(lin_param)) {
+    const float x = p->value[0];
+    return {lin_dir[0] * x, lin_dir[1] * x, lin_dir[2] * x};
+  }
+  if (const Param* p = scalar_param(param)) return p->value;
+  return host;
+}
+
+int register_scalar_param(float v) {
+  Param p;
+  p.kind = kParamScalar;
+  p.value = {v};
+  param_pool().push_back(p);
+  return (int)param_pool().size() - 1;
+}
+
+// Tensor * Tensor for scene set-up (Tensor.h expression operators): two
+// constants multiply elementwise (a single value broadcasts); a constant
+// Vec3 times a scalar parameter stays linked to the parameter, so that
+// set_var / configure move it and gradients and derivative images reach it.
+Tensorf tensor_mul(const Tensorf& a, const Tensorf& b) {
+  const bool pa = scalar_param(a.param) != nullptr, pb = scalar_param(b.param) != nullptr;
+  if (a.lin_param >= 0 || b.lin_param >= 0 || (pa && pb))
+    throw std::runtime_error("Tensorf product: only constant * constant and constant Vec3 * scalar parameter");
+  if (pa || pb) {
+    const Tensorf& k = pa ? b : a;
+    const int prm = pa ? a.param : b.param;
+    std::array<float, 3> dir;
+    if (k.host.size() == 3) dir = {k.host[0], k.host[1], k.host[2]};
+    else throw std::runtime_error("Tensorf product with a parameter: the constant must be a Vec3");
+    Tensorf t = Tensorf::vec3(0.0f, 0.0f, 0.0f, false);
+    t.lin_param = prm;
+    t.lin_dir = dir;
+    t.host = t.resolved();
+    return t;
+  }
+  const std::vector<float> x = a.resolved(), y = b.resolved();
+  if (x.empty() || y.empty()) throw std::runtime_error("Tensorf product of an empty tensor");
+  if (x.size() != y.size() && x.size() != 1 && y.size() != 1)
+    throw std::runtime_error("Tensorf product: shapes do not broadcast");
+  Tensorf t = x.size() >= y.size() ? a : b;
+  t.param = t.lin_param = -1;
+  t.requires_grad = false;
+  const size_t n = std::max(x.size(), y.size());
+  t.host.resize(n);
+  for (size_t i = 0; i < n; ++i) t.host[i] = x[x.size() == 1 ? 0 : i] * y[y.size() == 1 ? 0 : i];
   return t;
 }
 
@@ -324,8 +396,8 @@ M4 Primitive::world_matrix() const {
     }
     cur = m4_mul(to_world, m4_mul(ci, m4_mul(rm, c)));
   }
-  if (translation.host.size() == 3)
-    return m4_mul(m4_translate(translation.host[0], translation.host[1], translation.host[2]), cur);
+  const std::vector<float> tr = translation.resolved();
+  if (tr.size() == 3) return m4_mul(m4_translate(tr[0], tr[1], tr[2]), cur);
   if (rotate_axis.host.size() == 3 && rotate_angle.host.size() == 1)
     return m4_mul(m4_rotate(rotate_angle.host[0], rotate_axis.host[0], rotate_axis.host[1], rotate_axis.host[2]), cur);
   return cur;
@@ -342,8 +414,9 @@ void Primitive::world_positions(std::vector<float>& out, std::array<double, 9>& 
     for (int i = 0; i < 3; ++i)
       out[3 * v + i] = (float)(m[4 * i] * x + m[4 * i + 1] * y + m[4 * i + 2] * z + m[4 * i + 3]);
   }
-  if (vertex_id >= 0 && (size_t)vertex_id < V && vertex_translation.host.size() == 3)
-    for (int i = 0; i < 3; ++i) out[3 * vertex_id + i] += vertex_translation.host[i];
+  const std::vector<float> vt = vertex_translation.resolved();
+  if (vertex_id >= 0 && (size_t)vertex_id < V && vt.size() == 3)
+    for (int i = 0; i < 3; ++i) out[3 * vertex_id + i] += vt[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -375,6 +448,7 @@ trc_options RenderOptions::to_trc() const {
 // scene
 // ---------------------------------------------------------------------------
 Scene::~Scene() {
+  live_scenes().erase(this);
   if (sc) trc_scene_destroy(sc);
   if (active_scene() == this) active_scene() = nullptr;
 }
@@ -440,6 +514,7 @@ void Scene::configure() {
     sc = nullptr;
   }
   check(trc_scene_create(ctx(), &sc), "trc_scene_create");
+  live_scenes().insert(this);
   tex_index.assign(bsdfs.size(), -1);
   tex_off.assign(bsdfs.size(), -1);
   tex_floats = 0;
@@ -459,8 +534,11 @@ void Scene::configure() {
   for (size_t i = 0; i < shapes.size(); ++i) {
     const Primitive& p = *shapes[i];
     if (p.bsdf < 0 || p.bsdf >= (int)bsdfs.size()) throw std::runtime_error("shape refers to an unknown BSDF");
-    const int flags = (p.smooth ? TRC_MESH_SMOOTH : 0) | (p.diff_pos || !p.translation.host.empty() ||
-                                                          !p.rotate_axis.host.empty() ? TRC_MESH_DIFF : 0);
+    const int flags = (p.smooth ? TRC_MESH_SMOOTH : 0) |
+                      (p.diff_pos || !p.translation.host.empty() || !p.rotate_axis.host.empty() ||
+                               p.vertex_translation.lin_param >= 0
+                           ? TRC_MESH_DIFF
+                           : 0);
     const float* uv = p.uv.empty() ? nullptr : p.uv.data();
     const uint32_t* uvi = p.uv_idx.empty() ? nullptr : p.uv_idx.data();
     if (trc_scene_add_mesh(sc, world.data() + 3 * (size_t)vert_off[i], p.vertex_count(), p.idx.data(), p.face_count(),
@@ -512,6 +590,7 @@ static int64_t texture_param_size(const BSDF& b, const std::string& slot) {
 }
 
 int64_t Param::size() const {
+  if (kind == kParamScalar) return 1;
   if (kind == kParamTexture) {
     auto b = bsdf.lock();
     return b ? texture_param_size(*b, slot) : 0;
@@ -585,6 +664,10 @@ static void set_texture_value(BSDF& b, const std::string& slot, const std::vecto
 void get_var(int index, float* dst, int offset) {
   Param& p = param_at(index);
   dst += offset;
+  if (p.kind == kParamScalar) {
+    copy_out(dst, p.value);
+    return;
+  }
   if (p.kind == kParamTexture) {
     auto b = p.bsdf.lock();
     if (!b) throw std::runtime_error("parameter's BSDF no longer exists");
@@ -603,6 +686,10 @@ void set_var(int index, const float* src, int offset) {
   src += offset;
   const size_t n = (size_t)p.size();
   std::vector<float> v = copy_in(src, n);
+  if (p.kind == kParamScalar) {
+    p.value = v;
+    return;
+  }
   if (p.kind == kParamTexture) {
     auto b = p.bsdf.lock();
     if (!b) throw std::runtime_error("parameter's BSDF no longer exists");
@@ -632,6 +719,7 @@ void Scene::fold_gradients() {
   if (dtex) dtex->zero();
   for (Param& p : param_pool()) {
     if (p.grad.size() != (size_t)p.size()) p.grad.assign((size_t)p.size(), 0.0);
+    if (p.kind == kParamScalar) continue;  // below
     if (p.kind == kParamTexture) {
       auto b = p.bsdf.lock();
       if (!b) continue;
@@ -684,6 +772,22 @@ void Scene::fold_gradients() {
         for (int i = 0; i < 3; ++i)
           acc += g[3 * v + i] * (d[3 * i] * q[3 * v] + d[3 * i + 1] * q[3 * v + 1] + d[3 * i + 2] * q[3 * v + 2]);
       p.grad[0] += acc;
+    }
+  }
+  // scalar parameters: the shapes' translations linked to them
+  for (size_t k = 0; k < param_pool().size(); ++k) {
+    Param& p = param_pool()[k];
+    if (p.kind != kParamScalar) continue;
+    for (size_t si = 0; si < shapes.size(); ++si) {
+      const Primitive& sh = *shapes[si];
+      const float* g = &gp[3 * (size_t)vert_off[si]];
+      const size_t V = (size_t)sh.vertex_count();
+      if (sh.translation.lin_param == (int)k)
+        for (size_t v = 0; v < V; ++v)
+          for (int j = 0; j < 3; ++j) p.grad[0] += (double)g[3 * v + j] * sh.translation.lin_dir[j];
+      if (sh.vertex_translation.lin_param == (int)k && sh.vertex_id >= 0 && (size_t)sh.vertex_id < V)
+        for (int j = 0; j < 3; ++j)
+          p.grad[0] += (double)g[3 * (size_t)sh.vertex_id + j] * sh.vertex_translation.lin_dir[j];
     }
   }
 }
@@ -759,6 +863,7 @@ Tensorf PathTracer::render_c(Scene& scene, const RenderOptions& o) {
 Tensorf PathTracer::render_d(Scene& scene, const RenderOptions& o, const Tensorf& dLdI) {
   const Camera& c = camera0(scene);
   set_param(o);
+  if (o.export_deriv) return derivative_images(scene, o, -1);
   scene.ensure_grad_buffers();
   std::shared_ptr<DevArray> hold;
   const float* g = dldi_device(scene, dLdI, hold);
@@ -782,6 +887,7 @@ Tensorf PathTracer::render_d(Scene& scene, const RenderOptions& o, const Tensorf
 Tensorf BoundaryIntegrator::render_d(Scene& scene, const RenderOptions& o, const Tensorf& dLdI) {
   const Camera& c = camera0(scene);
   set_param(o);
+  if (o.export_deriv) return derivative_images(scene, o, kind);
   scene.ensure_grad_buffers();
   std::shared_ptr<DevArray> hold;
   const float* g = dldi_device(scene, dLdI, hold);
@@ -792,6 +898,123 @@ Tensorf BoundaryIntegrator::render_d(Scene& scene, const RenderOptions& o, const
   scene.grads_pending = true;
   active_scene() = &scene;
   return Tensorf();
+}
+
+// RenderD with export_deriv (Integrator.h:106-121, GradiantImageHandler,
+// Integrator.cpp:305-379): one forward-mode derivative image per element of
+// every pool parameter, in pool order, concatenated along the pixel axis
+// of each colour plane ([3][nimg * W * H]).  Each element enters the
+// device as a tangent of the world vertex positions (and of the material
+// slots) and trc_render_deriv does the rest.
+static constexpr int64_t kMaxDerivImages = 64;
+
+Tensorf derivative_images(Scene& scene, const RenderOptions& o, int kind) {
+  const Camera& c = camera0(scene);
+  const int64_t WH = (int64_t)c.width * c.height;
+  auto& pool = param_pool();
+  int64_t nimg = 0;
+  for (const Param& p : pool) nimg += p.size();
+  if (nimg == 0) throw std::runtime_error("export_deriv: no differentiable parameter");
+  if (nimg > kMaxDerivImages)
+    throw std::runtime_error("export_deriv: " + std::to_string(nimg) + " derivative images asked for, at most " +
+                             std::to_string(kMaxDerivImages) + " (one per parameter element)");
+  const size_t nv3 = 3 * (size_t)scene.nv, nm8 = 8 * scene.bsdfs.size();
+  auto out = std::make_shared<DevArray>(3 * WH * nimg);
+  out->zero();
+  DevArray img(3 * WH), tp((int64_t)nv3), tm((int64_t)nm8);
+  const trc_camera tc = c.to_trc();
+  const trc_options to = o.to_trc();
+  int64_t k = 0;
+  auto emit = [&](const std::vector<float>& tpos, const std::vector<float>* tmat) {
+    img.zero();
+    hip_ok(hipMemcpy(tp.p, tpos.data(), nv3 * 4, hipMemcpyHostToDevice), "hipMemcpy");
+    if (tmat) hip_ok(hipMemcpy(tm.p, tmat->data(), nm8 * 4, hipMemcpyHostToDevice), "hipMemcpy");
+    check(trc_render_deriv(scene.sc, &tc, &to, kind, tp.p, tmat ? tm.p : nullptr, img.p, nullptr), "renderD");
+    for (int ch = 0; ch < 3; ++ch)
+      hip_ok(hipMemcpy(out->p + ch * nimg * WH + k * WH, img.p + ch * WH, (size_t)WH * 4, hipMemcpyDeviceToDevice),
+             "hipMemcpy");
+    ++k;
+  };
+  auto shape_index = [&](const std::shared_ptr<Primitive>& sp) {
+    for (size_t i = 0; i < scene.shapes.size(); ++i)
+      if (scene.shapes[i] == sp) return (int)i;
+    return -1;
+  };
+  for (size_t pi = 0; pi < pool.size(); ++pi) {
+    const Param& p = pool[pi];
+    const int64_t n = p.size();
+    if (p.kind == kParamScalar) {
+      std::vector<float> t(nv3, 0.0f);
+      for (size_t si = 0; si < scene.shapes.size(); ++si) {
+        const Primitive& sh = *scene.shapes[si];
+        const size_t vo = (size_t)scene.vert_off[si], V = (size_t)sh.vertex_count();
+        if (sh.translation.lin_param == (int)pi)
+          for (size_t v = 0; v < V; ++v)
+            for (int j = 0; j < 3; ++j) t[3 * (vo + v) + j] += sh.translation.lin_dir[j];
+        if (sh.vertex_translation.lin_param == (int)pi && sh.vertex_id >= 0 && (size_t)sh.vertex_id < V)
+          for (int j = 0; j < 3; ++j) t[3 * (vo + (size_t)sh.vertex_id) + j] += sh.vertex_translation.lin_dir[j];
+      }
+      emit(t, nullptr);
+    } else if (p.kind == kParamTexture) {
+      auto b = p.bsdf.lock();
+      int bi = -1;
+      for (size_t i = 0; i < scene.bsdfs.size(); ++i)
+        if (b && scene.bsdfs[i] == b) bi = (int)i;
+      if (bi < 0) throw std::runtime_error("export_deriv: parameter's BSDF is not in the scene");
+      int first;
+      if (p.slot == "roughness") first = 0;
+      else if (p.slot == "specularReflectance" || p.slot == "k") first = 4;
+      else if (!b->texels.empty()) throw std::runtime_error("export_deriv: texture texels carry no derivative image");
+      else first = 1;  // albedo / reflectance / diffuseReflectance / eta constants
+      const std::vector<float> zero(nv3, 0.0f);
+      for (int64_t e = 0; e < n; ++e) {
+        std::vector<float> m(nm8, 0.0f);
+        m[8 * (size_t)bi + (size_t)first + (size_t)e] = 1.0f;
+        emit(zero, &m);
+      }
+    } else {
+      auto sp = p.prim.lock();
+      const int si = sp ? shape_index(sp) : -1;
+      if (si < 0) throw std::runtime_error("export_deriv: parameter's shape is not in the scene");
+      const size_t vo = (size_t)scene.vert_off[si], V = (size_t)sp->vertex_count();
+      const auto& L = scene.lin[si];
+      if (p.kind == kParamVertices) {
+        for (int j = 0; j < 3; ++j)
+          for (size_t v = 0; v < V; ++v) {  // element j * V + v: raw coordinate j of vertex v
+            std::vector<float> t(nv3, 0.0f);
+            for (int i = 0; i < 3; ++i) t[3 * (vo + v) + i] = (float)L[3 * i + j];
+            emit(t, nullptr);
+          }
+      } else if (p.kind == kParamTranslation) {
+        for (int j = 0; j < 3; ++j) {
+          std::vector<float> t(nv3, 0.0f);
+          for (size_t v = 0; v < V; ++v) t[3 * (vo + v) + j] = 1.0f;
+          emit(t, nullptr);
+        }
+      } else {
+        // world = R(angle) * cur: d p / d angle = dR (cur p)
+        Primitive tmp = *sp;
+        tmp.rotate_axis = Tensorf();
+        tmp.rotate_angle = Tensorf();
+        std::vector<float> q;
+        std::array<double, 9> unused;
+        tmp.world_positions(q, unused);
+        const auto d = rot_deriv(sp->rotate_angle.host[0], sp->rotate_axis.host[0], sp->rotate_axis.host[1],
+                                 sp->rotate_axis.host[2]);
+        std::vector<float> t(nv3, 0.0f);
+        for (size_t v = 0; v < V; ++v)
+          for (int i = 0; i < 3; ++i)
+            t[3 * (vo + v) + i] =
+                (float)(d[3 * i] * q[3 * v] + d[3 * i + 1] * q[3 * v + 1] + d[3 * i + 2] * q[3 * v + 2]);
+        emit(t, nullptr);
+      }
+    }
+  }
+  hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  Tensorf r;
+  r.shape = TensorShape({(int)(nimg * WH)}, kVec3);
+  r.dev = out;
+  return r;
 }
 
 // ---------------------------------------------------------------------------

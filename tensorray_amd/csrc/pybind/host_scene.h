@@ -81,6 +81,14 @@ struct Tensorf {
   std::vector<float> host;
   std::shared_ptr<DevArray> dev;
   bool requires_grad = false;
+  // A standalone parameter (Tensorf(val, requiresGrad=True)) is pool entry
+  // `param`; a product of a constant Vec3 and such a parameter (the
+  // reference's `Tensorf(x, y, z) * param` expression) is linked to it:
+  // value = lin_dir * current value of pool entry lin_param.
+  int param = -1;
+  int lin_param = -1;
+  std::array<float, 3> lin_dir{0.0f, 0.0f, 0.0f};
+  std::vector<float> resolved() const;  // current host value (links followed)
   Tensorf() = default;
   int64_t size() const { return dev ? dev->n : (int64_t)host.size(); }
   void value_to(float* dst) const;
@@ -211,16 +219,22 @@ struct Scene {
 };
 
 // ---- parameter pool (Tensor.h:1408-1419, 1675-1685) ----
-enum ParamKind { kParamVertices, kParamTranslation, kParamRotation, kParamTexture };
+enum ParamKind { kParamVertices, kParamTranslation, kParamRotation, kParamTexture, kParamScalar };
 struct Param {
   ParamKind kind;
   std::weak_ptr<Primitive> prim;
   std::weak_ptr<BSDF> bsdf;
   std::string slot;  // texture slot name
+  std::vector<float> value;  // kParamScalar: its value
   std::vector<double> grad;
   int64_t size() const;
 };
 std::vector<Param>& param_pool();
+// Registers a standalone scalar parameter; returns its pool index.
+int register_scalar_param(float v);
+// The reference's Tensorf product for the shapes it takes in scene
+// set-up: constant * constant, and constant Vec3 * scalar parameter.
+Tensorf tensor_mul(const Tensorf& a, const Tensorf& b);
 int get_num_param();
 int get_size_param(std::vector<int>& sizes);
 void get_var(int index, float* dst, int offset);
@@ -241,6 +255,10 @@ struct PathTracer : Integrator {
   Tensorf render_c(Scene& scene, const RenderOptions& o);
   Tensorf render_d(Scene& scene, const RenderOptions& o, const Tensorf& dLdI);
 };
+// RenderD with RenderOptions.export_deriv (Integrator.h:106-121): one
+// derivative image [W*H] Vec3 per element of every pool parameter,
+// concatenated; `kind` -1 = interior, 0..3 = boundary terms.
+Tensorf derivative_images(Scene& scene, const RenderOptions& o, int kind);
 struct BoundaryIntegrator : Integrator {
   int kind;
   explicit BoundaryIntegrator(int k) : kind(k) {}

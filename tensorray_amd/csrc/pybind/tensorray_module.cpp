@@ -67,7 +67,14 @@ PYBIND11_MODULE(TensorRay, m) {
 
   py::class_<Tensorf>(m, "Tensorf")
       .def(py::init<>())
-      .def(py::init([](float v, bool rg) { return Tensorf::scalar(v, rg); }), "val"_a = 0.0f, "requiresGrad"_a = false)
+      // requiresGrad registers the scalar in the parameter pool, as the
+      // reference's SetRequiresGrad does (Tensor.h:1675-1685)
+      .def(py::init([](float v, bool rg) {
+             Tensorf t = Tensorf::scalar(v, rg);
+             if (rg) t.param = register_scalar_param(v);
+             return t;
+           }),
+           "val"_a = 0.0f, "requiresGrad"_a = false)
       .def(py::init([](float x, float y, float z, bool rg) { return Tensorf::vec3(x, y, z, rg); }), "x"_a = 0.0f,
            "y"_a = 0.0f, "z"_a = 0.0f, "requiresGrad"_a = false)
       // 4x4 matrix from its entries in row order (the reference's 16-value
@@ -79,6 +86,9 @@ PYBIND11_MODULE(TensorRay, m) {
         return t;
       }))
       .def(py::init([](FloatPtr p, const TensorShape& s) { return Tensorf::from_ptr(p.p, s); }))
+      .def("__mul__", [](const Tensorf& a, const Tensorf& b) { return tensor_mul(a, b); })
+      .def("__mul__", [](const Tensorf& a, float f) { return tensor_mul(a, Tensorf::scalar(f, false)); })
+      .def("__rmul__", [](const Tensorf& a, float f) { return tensor_mul(Tensorf::scalar(f, false), a); })
       .def("size", &Tensorf::size)
       .def("requires_grad", [](const Tensorf& t) { return t.requires_grad; })
       .def("set_requires_grad", [](Tensorf& t, bool v) { t.requires_grad = v; })

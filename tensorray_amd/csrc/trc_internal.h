@@ -216,9 +216,17 @@ hipError_t launch_guide_build(const SceneDev& S, const LaunchCfg& L, uint32_t se
 uint64_t boundary_queue_bytes(int kind, uint64_t n_samples);
 // kind: 0 primary edges, 1 pixel boundary, 2 secondary direct, 3 secondary
 // indirect; queue: boundary_queue_bytes(kind, spp * W * H) bytes of scratch.
+// Gradient mode (dimg null): dpos += d/dV of <dLdI, boundary term>.
+// Tangent mode: dimg [3][W*H] += derivative image along tpos [V][3].
 hipError_t launch_boundary(int kind, const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                            const LaunchCfg& L, const EdgeSet& prim, const GuideDev& guide, const float* dLdI,
-                           float* dpos, void* queue, hipStream_t st);
+                           float* dpos, const float* tpos, float* dimg, void* queue, hipStream_t st);
+// Interior derivative image: dimg [3][W*H] += d film / d theta along tpos
+// [V][3] and tmat [M][8] (or null); tvn [V][3] scratch for the vertex-normal
+// tangents when a differentiable mesh is smooth-shaded, else null.
+hipError_t launch_render_deriv(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
+                               const LaunchCfg& L, const float* tpos, const float* tmat, float* tvn, float* dimg,
+                               hipStream_t st);
 // vn_adj: [V][3] zeroed scratch for smooth-shaded differentiable meshes or
 // null; it is folded into dpos and cleared again by a second kernel.
 hipError_t launch_render_adj(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
