@@ -299,6 +299,26 @@ def multiview_comparison(ctx, a):
     return out
 
 
+def streaming_copy_gbs(mib=2048, reps=5):
+    """Achievable HBM bandwidth on this box: a device-to-device copy of
+    `mib` MiB (read + write bytes over the best of `reps` HIP-event timings)."""
+    import torch
+    src = torch.empty(mib * (1 << 20) // 4, device="cuda")
+    src.fill_(1.0)
+    dst = torch.empty_like(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    dst.copy_(src)
+    best = float("inf")
+    for _ in range(reps):
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    del src, dst
+    return 2.0 * mib * (1 << 20) / (best * 1e-3) / 1e9
+
+
 def boundary_terms(scene, cam, W, H, B, dldi, a):
     """Edge-sample throughput of the boundary integrators on the metric
     workload (512^2, depth 4, seed 1234): each class at `boundary_spp`
@@ -437,7 +457,9 @@ def main():
                 traffic = json.load(f).get(kname)
         except (OSError, ValueError):
             traffic = None
+    copy_gbs = streaming_copy_gbs()
     roofline = dict(bound="hbm", achieved=achieved / 1e9, peak=MI355X_HBM_PEAK / 1e9, unit="GB/s",
+                    achievable_copy=copy_gbs, frac_of_achievable=achieved / 1e9 / copy_gbs,
                     frac=achieved / MI355X_HBM_PEAK, traffic=traffic, kernel=kname, kernel_ms=kms,
                     algorithmic_bytes_per_launch=kbytes, fwd_kernel_ms=fwd_avg, adj_kernel_ms=adj_avg, **st)
     if a.copy_calibration:

@@ -131,29 +131,79 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAd
 }
 
 // Adjoint from a path checkpoint: the path is re-evaluated from its recorded
-// hits (no traversal), then swept backwards as in k_render_adj.
+// hits (no traversal), then swept backwards as in k_render_adj.  Blocks take
+// chunks of kAdjChunk consecutive paths from a counter (the next free chunk,
+// so no block idles while another still has several) and order each chunk
+// in LDS by path length (longest first, camera misses and zero dL/dI
+// dropped) with a counting sort, so the waves sweep paths of equal length:
+// on the metric scene 23 % of the camera rays miss and a wave of 64
+// consecutive paths runs 2.6 vertices where its paths average 1.1.
+// Measured there: 1.84 ms unsorted, 1.40 with 256-path chunks, 1.12 with
+// 1,024, 1.04 with 1,024 fetched from the counter (2,048: 1.19).
+constexpr int kAdjSortR = 8;                   // paths per thread per chunk
+constexpr int kAdjChunk = kBlock * kAdjSortR;
+constexpr int kLenBuckets = kMaxBounces + 2;   // K + 1 in 0 .. max_bounces + 1
+
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdjWaves))) k_render_adj_ckpt(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
                                                             uint64_t first, uint64_t count, CkptDev ck,
                                                             const float* dLdI, float* dpos, float* dmat, float* adj_vn,
-                                                            float* dtex, GradCopies gc) {
+                                                            float* dtex, GradCopies gc, uint32_t* chunk_ctr) {
   if (gc.base) dpos = gc.base + (size_t)(blockIdx.x % gc.copies) * gc.n3;
   __shared__ float mat_grad[kMaxMatGrad * kMatGradStride];
+  __shared__ uint32_t s_ch;
+  __shared__ uint16_t order[kAdjChunk];
+  __shared__ uint32_t bcount[kLenBuckets], bnext[kLenBuckets];
   for (int i = threadIdx.x; i < kMaxMatGrad * kMatGradStride; i += kBlock) mat_grad[i] = 0.0f;
-  __syncthreads();
   const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
   const float inv_spp = 1.0f / (float)P.spp;
-  const uint64_t stride = (uint64_t)L.grid * kBlock;
   const uint64_t end = first + count;
-  for (uint64_t p = first + (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < end; p += stride) {
-    const uint32_t pix = (uint32_t)(p % WH);
-    F3 g{dLdI[pix] * inv_spp, dLdI[WH + pix] * inv_spp, dLdI[2 * WH + pix] * inv_spp};
-    if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) continue;
-    CkptSrc src{&ck, p, ck.meta[p]};
-    if ((src.meta & 0xFFu) == 0u) continue;  // camera ray missed: no gradient
-    PathRec R;
-    path_forward<true>(S, C, P, p, src, &R);
-    GradSink sk{dpos, mat_grad, adj_vn, dtex};
-    path_backward(S, C, P, R, g, sk);
+  const uint64_t nchunks = (count + kAdjChunk - 1) / kAdjChunk;
+  for (;;) {
+    if (threadIdx.x == 0) s_ch = atomicAdd(chunk_ctr, 1u);
+    if (threadIdx.x < kLenBuckets) bcount[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint64_t ch = s_ch;
+    if (ch >= nchunks) break;  // s_ch is the block's: every thread leaves together
+    const uint64_t base = first + ch * kAdjChunk;
+    uint32_t kb[kAdjSortR];
+#pragma unroll
+    for (int r = 0; r < kAdjSortR; ++r) {
+      const uint64_t p = base + threadIdx.x + (uint64_t)r * kBlock;
+      kb[r] = 0u;
+      if (p < end) {
+        const uint32_t pix = (uint32_t)(p % WH);
+        const bool zero = dLdI[pix] == 0.0f && dLdI[WH + pix] == 0.0f && dLdI[2 * WH + pix] == 0.0f;
+        const uint32_t k1 = ck.meta[p] & 0xFFu;
+        kb[r] = zero ? 0u : (k1 < (uint32_t)kLenBuckets ? k1 : (uint32_t)kLenBuckets - 1u);
+        if (kb[r]) atomicAdd(&bcount[kb[r]], 1u);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t o = 0u;
+      for (int b = kLenBuckets - 1; b >= 1; --b) {
+        bnext[b] = o;
+        o += bcount[b];
+      }
+      bcount[0] = o;  // paths to sweep in this chunk
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kAdjSortR; ++r)
+      if (kb[r]) order[atomicAdd(&bnext[kb[r]], 1u)] = (uint16_t)(threadIdx.x + r * kBlock);
+    __syncthreads();
+    const uint32_t nsweep = bcount[0];
+    for (uint32_t j = threadIdx.x; j < nsweep; j += kBlock) {
+      const uint64_t p = base + order[j];
+      const uint32_t pix = (uint32_t)(p % WH);
+      F3 g{dLdI[pix] * inv_spp, dLdI[WH + pix] * inv_spp, dLdI[2 * WH + pix] * inv_spp};
+      CkptSrc src{&ck, p, ck.meta[p]};
+      PathRec R;
+      path_forward<true>(S, C, P, p, src, &R);
+      GradSink sk{dpos, mat_grad, adj_vn, dtex};
+      path_backward(S, C, P, R, g, sk);
+    }
+    __syncthreads();  // order / bcount are reused by the next chunk
   }
   __syncthreads();
   if (dmat) {
@@ -452,9 +502,14 @@ hipError_t launch_render_adj_ckpt(const SceneDev& S, const CameraDev& C, uint32_
   }
   const CkptDev ck = ckpt_view(const_cast<void*>(ckpt), np, max_bounces);
   const LaunchCfg Lk = fit_grid(k_render_adj_ckpt, L);
+  // this kernel traces no rays: word 0 of the stream's traversal spill
+  // buffer serves as its chunk counter, cleared by a kernel on the stream
+  if (!L.spill) return hipErrorInvalidValue;
+  hipError_t e = launch_fill_zero((float*)L.spill, 1, st);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_render_adj_ckpt, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, first, count, ck, dLdI,
-                     dpos, dmat, vn_adj, dtex, gc);
-  hipError_t e = hipGetLastError();
+                     dpos, dmat, vn_adj, dtex, gc, L.spill);
+  e = hipGetLastError();
   if (!fold) return e;  // the caller folds the gradient copies and vertex normals
   if (e == hipSuccess && gc.base) e = launch_grad_reduce(gc, dpos, st);
   if (e != hipSuccess || !vn_adj || S.n_verts == 0) return e;
