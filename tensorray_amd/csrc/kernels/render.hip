@@ -355,12 +355,26 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kFw
 }
 
 // Material gradients are summed per wave in LDS and flushed at the end of
-// each tile (one job per tile).
+// each tile (one job per tile).  Like k_render_adj_ckpt, each wave sorts
+// its tile's paths by length in LDS, kAdjChunk paths at a time, before the
+// sweep.
+__device__ inline uint32_t adj_bucket(const CkptDev& ck, const float* dLdI, uint32_t WH, uint64_t p) {
+  const uint32_t pix = (uint32_t)(p % WH);
+  if (dLdI[pix] == 0.0f && dLdI[WH + pix] == 0.0f && dLdI[2 * WH + pix] == 0.0f) return 0u;
+  const uint32_t k1 = ck.meta[p] & 0xFFu;
+  return k1 < (uint32_t)kLenBuckets ? k1 : (uint32_t)kLenBuckets - 1u;
+}
+
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdjWaves))) k_render_batch_adj(const JobDev* jobs, int njobs, uint64_t ntiles,
                                                              uint32_t tile, LaunchCfg L) {
   __shared__ float mat_lds[(kBlock / 64) * kMaxMatGrad * kMatGradStride];
+  __shared__ uint16_t ord_lds[(kBlock / 64) * kAdjChunk];
+  __shared__ uint32_t cnt_lds[(kBlock / 64) * 2 * kLenBuckets];
   const int lane = (int)(threadIdx.x & 63u);
   float* mat_grad = mat_lds + (threadIdx.x >> 6) * (kMaxMatGrad * kMatGradStride);
+  uint16_t* order = ord_lds + (threadIdx.x >> 6) * kAdjChunk;
+  uint32_t* bcount = cnt_lds + (threadIdx.x >> 6) * 2 * kLenBuckets;
+  uint32_t* bnext = bcount + kLenBuckets;
   const uint64_t nwaves = (uint64_t)L.grid * (kBlock / 64);
   const uint64_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
   for (uint64_t t = wave; t < ntiles; t += nwaves) {
@@ -384,16 +398,46 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAd
     __builtin_amdgcn_wave_barrier();
     const uint64_t begin = first + (t - t0) * tile;
     const uint64_t end = begin + tile < first + count ? begin + tile : first + count;
-    for (uint64_t p = begin + (uint64_t)lane; p < end; p += 64) {
-      const uint32_t pix = (uint32_t)(p % WH);
-      F3 g{dLdI[pix] * inv_spp, dLdI[WH + pix] * inv_spp, dLdI[2 * WH + pix] * inv_spp};
-      if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f) continue;
-      CkptSrc src{&ck, p, ck.meta[p]};
-      if ((src.meta & 0xFFu) == 0u) continue;
-      PathRec R;
-      path_forward<true>(S, C, P, p, src, &R);
-      GradSink sk{dpos, mat_grad, vn, dtex};
-      path_backward(S, C, P, R, g, sk);
+    for (uint64_t sub = begin; sub < end; sub += kAdjChunk) {
+      const uint64_t send = sub + kAdjChunk < end ? sub + kAdjChunk : end;
+      if (lane < kLenBuckets) bcount[lane] = 0u;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      for (uint64_t p = sub + (uint64_t)lane; p < send; p += 64) {
+        const uint32_t b = adj_bucket(ck, dLdI, WH, p);
+        if (b) atomicAdd(&bcount[b], 1u);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (lane == 0) {
+        uint32_t o = 0u;
+        for (int b = kLenBuckets - 1; b >= 1; --b) {
+          bnext[b] = o;
+          o += bcount[b];
+        }
+        bcount[0] = o;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      for (uint64_t p = sub + (uint64_t)lane; p < send; p += 64) {
+        const uint32_t b = adj_bucket(ck, dLdI, WH, p);
+        if (b) order[atomicAdd(&bnext[b], 1u)] = (uint16_t)(p - sub);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      const uint32_t nsweep = bcount[0];
+      for (uint32_t q = (uint32_t)lane; q < nsweep; q += 64) {
+        const uint64_t p = sub + order[q];
+        const uint32_t pix = (uint32_t)(p % WH);
+        F3 g{dLdI[pix] * inv_spp, dLdI[WH + pix] * inv_spp, dLdI[2 * WH + pix] * inv_spp};
+        CkptSrc src{&ck, p, ck.meta[p]};
+        PathRec R;
+        path_forward<true>(S, C, P, p, src, &R);
+        GradSink sk{dpos, mat_grad, vn, dtex};
+        path_backward(S, C, P, R, g, sk);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
