@@ -279,3 +279,39 @@ def test_pytensorray_helpers(scene_dir):
     assert o.spp_batch == 2 and o.g_direct_spp == 8 and o.g_direct is False and o.checkpoint is True
     s = fwd.Scene(str(scene_dir / "scene.xml"))
     assert len(s.shapes) == 2
+
+
+def test_scalar_parameter_and_linked_translation(scene_dir):
+    """Tensorf(v, True) registers a scalar parameter (Tensor.h:1675-1685) and
+    `Tensorf(x, y, z) * param` stays linked to it, the way the reference's
+    validation scripts move a shape (validate_gpu.py:69-80)."""
+    x = TR.Tensorf(0.5, True)
+    assert TR.get_num_param() == 1
+    sizes = []
+    assert TR.get_size_param(sizes) == 1 and sizes == [1]
+    t = TR.Tensorf(0.0, 0.0, 2.0) * x
+    out = torch.zeros(3)
+    t.value_to_torch(get_ptr(out))
+    assert out.tolist() == [0.0, 0.0, 1.0]
+    val = torch.tensor([-1.5])
+    TR.set_var(0, get_ptr(val), 0)
+    t.value_to_torch(get_ptr(out))
+    assert out.tolist() == [0.0, 0.0, -3.0]   # the link follows the parameter
+    got = torch.zeros(1)
+    TR.get_var(0, get_ptr(got), 0)
+    assert got.item() == -1.5
+    # constants multiply elementwise, a single value broadcasts
+    c = TR.Tensorf(1.0, 2.0, 3.0) * TR.Tensorf(2.0, 3.0, 4.0)
+    c.value_to_torch(get_ptr(out))
+    assert out.tolist() == [2.0, 6.0, 12.0]
+    c = 0.5 * TR.Tensorf(2.0, 4.0, 6.0)
+    c.value_to_torch(get_ptr(out))
+    assert out.tolist() == [1.0, 2.0, 3.0]
+    with pytest.raises(RuntimeError, match="Tensorf product"):
+        TR.Tensorf(1.0, True) * x
+    # a shape's translation may hold the linked product
+    s = TR.Scene()
+    s.load_file(str(scene_dir / "scene.xml"))
+    s.shapes[0].translation = TR.Tensorf(1.0, 0.0, 0.0) * x
+    s.shapes[0].translation.value_to_torch(get_ptr(out))
+    assert out.tolist() == [-1.5, 0.0, 0.0]
