@@ -1108,99 +1108,97 @@ trc_status trc_render_stats(trc_scene* s, const trc_camera* cam, const trc_optio
 // derivative image along tpos.  Runs inside guarded().
 static void boundary_call(trc_scene* s, const trc_camera* cam, const trc_options* opt, int32_t kind,
                           const float* dLdI, float* dpos, const float* tpos, float* dimg, trc_stream stream) {
-  {
-    check_render_common(s, opt);
-    CameraDev C = to_cam(cam);
-    if (kind < 0 || kind > 3) throw Fail(TRC_ERR_INVALID, "unknown boundary kind");
-    const int spp = kind == 0 ? opt->sppe : kind == 1 ? opt->sppe0 : kind == 2 ? opt->sppse0 : opt->sppse1;
-    if (spp < 0) throw Fail(TRC_ERR_INVALID, "negative boundary spp");
-    if (spp == 0) return;
-    const long long N = (long long)spp * C.width * C.height;
-    if (kind == 1 && N % 4 != 0) throw Fail(TRC_ERR_INVALID, "pixel-boundary samples (sppe0*W*H) must be a multiple of 4");
-    hipStream_t st = (hipStream_t)stream;
-    EdgeSet prim{};
-    if (kind == 0) {
-      select_prim_edges(s->host, C.eye, s->prim_edges);
-      if (s->prim_edges.empty()) return;
-      prim.integral = edge_distribution(s->host, s->prim_edges, s->prim_len, s->prim_cdf);
-      prim.n = (int32_t)s->prim_edges.size();
-      const size_t eb = s->prim_edges.size() * sizeof(EdgeDev), fb = s->prim_edges.size() * sizeof(float);
-      const size_t need = eb + 2 * fb;
-      if (s->prim_buf.bytes < need) {
-        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-        if (s->prim_buf.ptr) hipFree(s->prim_buf.ptr);
-        s->prim_buf.ptr = nullptr;
-        s->prim_buf.bytes = 0;
-        hip_check(hipMalloc(&s->prim_buf.ptr, need), "hipMalloc(primary edges)");
-        s->prim_buf.bytes = need;
-      }
-      char* base = (char*)s->prim_buf.ptr;
-      hip_check(hipMemcpyAsync(base, s->prim_edges.data(), eb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-      hip_check(hipMemcpyAsync(base + eb, s->prim_len.data(), fb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-      hip_check(hipMemcpyAsync(base + eb + fb, s->prim_cdf.data(), fb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-      prim.e = (const EdgeDev*)base;
-      prim.len = (const float*)(base + eb);
-      prim.cdf = (const float*)(base + eb + fb);
-    }
-    if (kind >= 2 && s->dev.sec.n == 0) return;
-    GuideDev guide{};
-    if (kind == 2 && opt->g_direct) {
-      const int n = s->dev.sec.n;
-      if (opt->g_direct_spp < 1 || opt->g_direct_spp > 4096)
-        throw Fail(TRC_ERR_INVALID, "g_direct_spp must be in [1, 4096]");
-      if (!(opt->g_eps >= 0.0f && opt->g_eps <= 1.0f)) throw Fail(TRC_ERR_INVALID, "g_eps must be in [0, 1]");
-      if (opt->g_direct_max_size > 0 && n > opt->g_direct_max_size)
-        throw Fail(TRC_ERR_INVALID, "guiding: " + std::to_string(n) + " secondary edges exceed g_direct_max_size " +
-                                        std::to_string(opt->g_direct_max_size));
-      const size_t nv = (size_t)n * (size_t)opt->g_direct_spp;
-      const size_t nparts = ((size_t)n + kSumChunk - 1) / kSumChunk + 1;
-      const size_t need = (nv + 2 * (size_t)n + 1 + nparts) * sizeof(float);
-      const uint32_t gseed = opt->g_keep ? opt->g_seed : opt->seed;
-      const trc_scene::GuideKey key{true, s->version, gseed, opt->g_direct_spp, n, opt->g_eps};
-      const trc_scene::GuideKey& k = s->guide_key;
-      const bool reuse = opt->g_keep && k.valid && k.version == key.version && k.seed == key.seed &&
-                         k.spp == key.spp && k.n == key.n && k.eps == key.eps && s->guide_buf.bytes >= need;
-      if (s->guide_buf.bytes < need) {
-        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-        if (s->guide_buf.ptr) hipFree(s->guide_buf.ptr);
-        s->guide_buf.ptr = nullptr;
-        s->guide_buf.bytes = 0;
-        s->guide_key.valid = false;
-        hip_check(hipMalloc(&s->guide_buf.ptr, need), "hipMalloc(guide)");
-        s->guide_buf.bytes = need;
-      }
-      float* base = (float*)s->guide_buf.ptr;
-      guide.vals = base;
-      guide.poly = base + nv;
-      guide.cdf = base + nv + n;
-      guide.fsum = base + nv + 2 * (size_t)n;
-      guide.part = base + nv + 2 * (size_t)n + 1;
-      guide.n = n;
-      guide.spp = opt->g_direct_spp;
-      guide.eps = opt->g_eps;
-      guide.enabled = 1;
-      if (!reuse) {
-        s->guide_key.valid = false;
-        LaunchCfg Lg = make_cfg(s->ctx, s, st, (long long)nv);
-        hip_check(launch_guide_build(s->dev, Lg, gseed, guide, st), "k_guide");
-        if (opt->g_keep) s->guide_key = key;
-      }
-    }
-    LaunchCfg L = make_cfg(s->ctx, s, st, N);
-    const size_t qb = (size_t)boundary_queue_bytes(kind, (uint64_t)N);
-    if (s->bq_buf.bytes < qb) {
+  check_render_common(s, opt);
+  CameraDev C = to_cam(cam);
+  if (kind < 0 || kind > 3) throw Fail(TRC_ERR_INVALID, "unknown boundary kind");
+  const int spp = kind == 0 ? opt->sppe : kind == 1 ? opt->sppe0 : kind == 2 ? opt->sppse0 : opt->sppse1;
+  if (spp < 0) throw Fail(TRC_ERR_INVALID, "negative boundary spp");
+  if (spp == 0) return;
+  const long long N = (long long)spp * C.width * C.height;
+  if (kind == 1 && N % 4 != 0) throw Fail(TRC_ERR_INVALID, "pixel-boundary samples (sppe0*W*H) must be a multiple of 4");
+  hipStream_t st = (hipStream_t)stream;
+  EdgeSet prim{};
+  if (kind == 0) {
+    select_prim_edges(s->host, C.eye, s->prim_edges);
+    if (s->prim_edges.empty()) return;
+    prim.integral = edge_distribution(s->host, s->prim_edges, s->prim_len, s->prim_cdf);
+    prim.n = (int32_t)s->prim_edges.size();
+    const size_t eb = s->prim_edges.size() * sizeof(EdgeDev), fb = s->prim_edges.size() * sizeof(float);
+    const size_t need = eb + 2 * fb;
+    if (s->prim_buf.bytes < need) {
       hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-      if (s->bq_buf.ptr) hipFree(s->bq_buf.ptr);
-      s->bq_buf.ptr = nullptr;
-      s->bq_buf.bytes = 0;
-      hip_check(hipMalloc(&s->bq_buf.ptr, qb), "hipMalloc(boundary queue)");
-      s->bq_buf.bytes = qb;
+      if (s->prim_buf.ptr) hipFree(s->prim_buf.ptr);
+      s->prim_buf.ptr = nullptr;
+      s->prim_buf.bytes = 0;
+      hip_check(hipMalloc(&s->prim_buf.ptr, need), "hipMalloc(primary edges)");
+      s->prim_buf.bytes = need;
     }
-    hip_check(launch_boundary(kind, s->dev, C, opt->seed, opt->max_bounces, spp, L, prim, guide, dLdI, dpos, tpos,
-                              dimg, s->bq_buf.ptr, st),
-              "k_boundary");
-    if (kind == 0) hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    char* base = (char*)s->prim_buf.ptr;
+    hip_check(hipMemcpyAsync(base, s->prim_edges.data(), eb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    hip_check(hipMemcpyAsync(base + eb, s->prim_len.data(), fb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    hip_check(hipMemcpyAsync(base + eb + fb, s->prim_cdf.data(), fb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    prim.e = (const EdgeDev*)base;
+    prim.len = (const float*)(base + eb);
+    prim.cdf = (const float*)(base + eb + fb);
   }
+  if (kind >= 2 && s->dev.sec.n == 0) return;
+  GuideDev guide{};
+  if (kind == 2 && opt->g_direct) {
+    const int n = s->dev.sec.n;
+    if (opt->g_direct_spp < 1 || opt->g_direct_spp > 4096)
+      throw Fail(TRC_ERR_INVALID, "g_direct_spp must be in [1, 4096]");
+    if (!(opt->g_eps >= 0.0f && opt->g_eps <= 1.0f)) throw Fail(TRC_ERR_INVALID, "g_eps must be in [0, 1]");
+    if (opt->g_direct_max_size > 0 && n > opt->g_direct_max_size)
+      throw Fail(TRC_ERR_INVALID, "guiding: " + std::to_string(n) + " secondary edges exceed g_direct_max_size " +
+                                      std::to_string(opt->g_direct_max_size));
+    const size_t nv = (size_t)n * (size_t)opt->g_direct_spp;
+    const size_t nparts = ((size_t)n + kSumChunk - 1) / kSumChunk + 1;
+    const size_t need = (nv + 2 * (size_t)n + 1 + nparts) * sizeof(float);
+    const uint32_t gseed = opt->g_keep ? opt->g_seed : opt->seed;
+    const trc_scene::GuideKey key{true, s->version, gseed, opt->g_direct_spp, n, opt->g_eps};
+    const trc_scene::GuideKey& k = s->guide_key;
+    const bool reuse = opt->g_keep && k.valid && k.version == key.version && k.seed == key.seed &&
+                       k.spp == key.spp && k.n == key.n && k.eps == key.eps && s->guide_buf.bytes >= need;
+    if (s->guide_buf.bytes < need) {
+      hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+      if (s->guide_buf.ptr) hipFree(s->guide_buf.ptr);
+      s->guide_buf.ptr = nullptr;
+      s->guide_buf.bytes = 0;
+      s->guide_key.valid = false;
+      hip_check(hipMalloc(&s->guide_buf.ptr, need), "hipMalloc(guide)");
+      s->guide_buf.bytes = need;
+    }
+    float* base = (float*)s->guide_buf.ptr;
+    guide.vals = base;
+    guide.poly = base + nv;
+    guide.cdf = base + nv + n;
+    guide.fsum = base + nv + 2 * (size_t)n;
+    guide.part = base + nv + 2 * (size_t)n + 1;
+    guide.n = n;
+    guide.spp = opt->g_direct_spp;
+    guide.eps = opt->g_eps;
+    guide.enabled = 1;
+    if (!reuse) {
+      s->guide_key.valid = false;
+      LaunchCfg Lg = make_cfg(s->ctx, s, st, (long long)nv);
+      hip_check(launch_guide_build(s->dev, Lg, gseed, guide, st), "k_guide");
+      if (opt->g_keep) s->guide_key = key;
+    }
+  }
+  LaunchCfg L = make_cfg(s->ctx, s, st, N);
+  const size_t qb = (size_t)boundary_queue_bytes(kind, (uint64_t)N);
+  if (s->bq_buf.bytes < qb) {
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (s->bq_buf.ptr) hipFree(s->bq_buf.ptr);
+    s->bq_buf.ptr = nullptr;
+    s->bq_buf.bytes = 0;
+    hip_check(hipMalloc(&s->bq_buf.ptr, qb), "hipMalloc(boundary queue)");
+    s->bq_buf.bytes = qb;
+  }
+  hip_check(launch_boundary(kind, s->dev, C, opt->seed, opt->max_bounces, spp, L, prim, guide, dLdI, dpos, tpos,
+                            dimg, s->bq_buf.ptr, st),
+            "k_boundary");
+  if (kind == 0) hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
 }
 
 trc_status trc_render_boundary(trc_scene* s, const trc_camera* cam, const trc_options* opt, int32_t kind,
