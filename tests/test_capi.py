@@ -155,4 +155,12 @@ def test_host_context_rejects_device_work(host_ctx):
         host_ctx.adam_uniform_step(1, 1, 1, 1, 4, 1)
     with pytest.raises(_capi.TRCError, match="TRC_ERR_STATE"):
         host_ctx.lsolver(np.array([[0, 1, 2]]), 3, 1.0)
+    for kind in ("interior", "primary"):
+        dopt = _capi.make_options(spp=1, sppe=1)
+        with pytest.raises(_capi.TRCError, match="TRC_ERR_STATE"):
+            ds.render_deriv(cam, dopt, kind, 1, None, 1)
+        with pytest.raises(_capi.TRCError, match="TRC_ERR_INVALID"):
+            ds.render_deriv(cam, dopt, kind, None, None, 1)   # no tangent buffer
+    with pytest.raises(_capi.TRCError, match="TRC_ERR_INVALID"):
+        ds.render_deriv(cam, opt, 7, 1, None, 1)              # no such integrator
     ds.close()
