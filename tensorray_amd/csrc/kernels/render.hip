@@ -327,7 +327,7 @@ __device__ inline int job_of_tile(const JobDev* jobs, int njobs, uint64_t t) {
   return __builtin_amdgcn_readfirstlane(lo);
 }
 
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kFwdWaves))) k_render_batch_fwd(const JobDev* jobs, int njobs, uint64_t ntiles,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kFwdWaves))) k_render_batch_fwd(const JobDev* __restrict__ jobs, int njobs, uint64_t ntiles,
                                                              uint32_t tile, LaunchCfg L) {
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx(lds_stack, L);
@@ -335,8 +335,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kFw
   const uint64_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
   for (uint64_t t = wave; t < ntiles;) {
     const int j = job_of_tile(jobs, njobs, t);
-    const SceneDev S = wave_uniform_load(&jobs[j].S);
-    const CameraDev C = wave_uniform_load(&jobs[j].C);
+    // read in place from the (read-only) job table: copying the records
+    // costs registers, and spill, in the path loop
+    const SceneDev& S = jobs[j].S;
+    const CameraDev& C = jobs[j].C;
     const PathParams P{jobs[j].seed, jobs[j].max_bounces, jobs[j].spp, 0u};
     const uint64_t first = jobs[j].first, count = jobs[j].count, t0 = jobs[j].tile0;
     const uint64_t npaths = (uint64_t)P.spp * (uint64_t)C.width * (uint64_t)C.height;
@@ -365,7 +367,7 @@ __device__ inline uint32_t adj_bucket(const CkptDev& ck, const float* dLdI, uint
   return k1 < (uint32_t)kLenBuckets ? k1 : (uint32_t)kLenBuckets - 1u;
 }
 
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdjWaves))) k_render_batch_adj(const JobDev* jobs, int njobs, uint64_t ntiles,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdjWaves))) k_render_batch_adj(const JobDev* __restrict__ jobs, int njobs, uint64_t ntiles,
                                                              uint32_t tile, LaunchCfg L) {
   __shared__ float mat_lds[(kBlock / 64) * kMaxMatGrad * kMatGradStride];
   __shared__ uint16_t ord_lds[(kBlock / 64) * kAdjChunk];
@@ -379,8 +381,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAd
   const uint64_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
   for (uint64_t t = wave; t < ntiles; t += nwaves) {
     const int j = job_of_tile(jobs, njobs, t);
-    const SceneDev S = wave_uniform_load(&jobs[j].S);
-    const CameraDev C = wave_uniform_load(&jobs[j].C);
+    // read in place from the (read-only) job table: copying the records
+    // costs registers, and spill, in the path loop
+    const SceneDev& S = jobs[j].S;
+    const CameraDev& C = jobs[j].C;
     const PathParams P{jobs[j].seed, jobs[j].max_bounces, jobs[j].spp, 0u};
     const uint64_t first = jobs[j].first, count = jobs[j].count, t0 = jobs[j].tile0;
     const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
