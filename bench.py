@@ -446,7 +446,8 @@ def main():
         kbytes = (bb["adj_ckpt"] if ck else bb["adj"]) * P
     else:
         # the forward kernel with (true) or without (false) checkpoint writes
-        kname, kms = ("k_render_paths<true>" if ck else "k_render_paths<false>"), fwd_avg
+        merged = "true" if B >= 5 else "false"  # merged-bounce kernel from 5 bounces on
+        kname, kms = f"k_render_paths<{'true' if ck else 'false'}, {merged}>", fwd_avg
         kbytes = (bb["fwd_ckpt"] if ck else bb["fwd"]) * P
     achieved = kbytes / (kms * 1e-3)
     traffic = None

@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(kBlock) k_intersect(SceneDev S, LaunchCfg L, c
   }
 }
 
-template <bool kCkpt>
+template <bool kCkpt, bool kMerged>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kFwdWaves))) k_render_paths(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
                                                          uint64_t first, uint64_t count, uint32_t tile, float* paths,
                                                          CkptDev ck) {
@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kFw
   const uint64_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
   const uint64_t ntiles = (count + tile - 1) / tile;
   TileSupply sup = TileSupply::make(wave, ntiles, nwaves, 0, first, first + count, tile);
-  render_paths_regen<kCkpt>(S, C, P, tc, sup, ck, [&](uint64_t p, F3 c) {
+  render_paths_regen<kCkpt, kMerged>(S, C, P, tc, sup, ck, [&](uint64_t p, F3 c) {
     paths[3 * p] = c.x;
     paths[3 * p + 1] = c.y;
     paths[3 * p + 2] = c.z;
@@ -513,6 +513,20 @@ static uint32_t forward_tile(uint64_t npaths, int grid) {
   return (uint32_t)(t < 64 ? 64 : (t > 4096 ? 4096 : t));
 }
 
+// Paths of at least this many bounces trace the shadow ray and the BSDF ray
+// of a bounce in one loop iteration (kMerged): one traversal loop fewer per
+// bounce, more registers live across it.  Measured on gfx950: 3-10% faster
+// at 8 bounces, 2% slower at 4; the results are the same either way.
+constexpr int kMergedMinBounces = 5;
+
+template <bool kCkpt, bool kMerged>
+static void launch_paths(const SceneDev& S, const CameraDev& C, const PathParams& P, const LaunchCfg& L,
+                         uint64_t first, uint64_t count, float* paths, const CkptDev& ck, hipStream_t st) {
+  const LaunchCfg Lk = fit_grid(k_render_paths<kCkpt, kMerged>, L);
+  hipLaunchKernelGGL((k_render_paths<kCkpt, kMerged>), dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, first,
+                     count, forward_tile(count, Lk.grid), paths, ck);
+}
+
 hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                                const LaunchCfg& L, float* paths, void* ckpt, hipStream_t st, uint64_t first,
                                uint64_t count) {
@@ -522,16 +536,13 @@ hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t s
     first = 0;
     count = np;
   }
+  const CkptDev ck = ckpt ? ckpt_view(ckpt, np, max_bounces) : CkptDev{};
   if (ckpt) {
-    const CkptDev ck = ckpt_view(ckpt, np, max_bounces);
-    const LaunchCfg Lk = fit_grid(k_render_paths<true>, L);
-    hipLaunchKernelGGL(k_render_paths<true>, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, first, count,
-                       forward_tile(count, Lk.grid), paths, ck);
+    if (max_bounces >= kMergedMinBounces) launch_paths<true, true>(S, C, P, L, first, count, paths, ck, st);
+    else launch_paths<true, false>(S, C, P, L, first, count, paths, ck, st);
   } else {
-    const CkptDev ck{};
-    const LaunchCfg Lk = fit_grid(k_render_paths<false>, L);
-    hipLaunchKernelGGL(k_render_paths<false>, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, first, count,
-                       forward_tile(count, Lk.grid), paths, ck);
+    if (max_bounces >= kMergedMinBounces) launch_paths<false, true>(S, C, P, L, first, count, paths, ck, st);
+    else launch_paths<false, false>(S, C, P, L, first, count, paths, ck, st);
   }
   return hipGetLastError();
 }
