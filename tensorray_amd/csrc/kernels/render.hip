@@ -513,11 +513,14 @@ static uint32_t forward_tile(uint64_t npaths, int grid) {
   return (uint32_t)(t < 64 ? 64 : (t > 4096 ? 4096 : t));
 }
 
-// Paths of at least this many bounces trace the shadow ray and the BSDF ray
-// of a bounce in one loop iteration (kMerged): one traversal loop fewer per
-// bounce, more registers live across it.  Measured on gfx950: 3-10% faster
-// at 8 bounces, 2% slower at 4; the results are the same either way.
+// Launches of paths with at least this many bounces, and at most
+// kMergedMaxPaths paths, trace the shadow ray and the BSDF ray of a bounce in
+// one loop iteration (kMerged): one traversal loop fewer per bounce, more
+// registers live across it.  Measured on gfx950 at 8 bounces: 3-10% faster
+// for 4-17M paths per launch, 3% slower for 67M (the 4,096-path tiles); 2%
+// slower at 4 bounces.  The results are the same either way.
 constexpr int kMergedMinBounces = 5;
+constexpr uint64_t kMergedMaxPaths = 1ull << 25;
 
 template <bool kCkpt, bool kMerged>
 static void launch_paths(const SceneDev& S, const CameraDev& C, const PathParams& P, const LaunchCfg& L,
@@ -537,11 +540,12 @@ hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t s
     count = np;
   }
   const CkptDev ck = ckpt ? ckpt_view(ckpt, np, max_bounces) : CkptDev{};
+  const bool merged = max_bounces >= kMergedMinBounces && count <= kMergedMaxPaths;
   if (ckpt) {
-    if (max_bounces >= kMergedMinBounces) launch_paths<true, true>(S, C, P, L, first, count, paths, ck, st);
+    if (merged) launch_paths<true, true>(S, C, P, L, first, count, paths, ck, st);
     else launch_paths<true, false>(S, C, P, L, first, count, paths, ck, st);
   } else {
-    if (max_bounces >= kMergedMinBounces) launch_paths<false, true>(S, C, P, L, first, count, paths, ck, st);
+    if (merged) launch_paths<false, true>(S, C, P, L, first, count, paths, ck, st);
     else launch_paths<false, false>(S, C, P, L, first, count, paths, ck, st);
   }
   return hipGetLastError();
