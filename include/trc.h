@@ -34,7 +34,7 @@ typedef int32_t trc_status;
 #define TRC_ERR_HIP 2       /* HIP runtime failure */
 #define TRC_ERR_NOMEM 3     /* device allocation failed or budget exceeded */
 #define TRC_ERR_STATE 4     /* call order (e.g. render before configure) */
-#define TRC_ERR_OVERFLOW 5  /* a traversal stack overflowed (never expected) */
+#define TRC_ERR_OVERFLOW 5  /* the scene's BVH is deeper than the traversal stack (16 LDS + 64 spill entries) */
 
 typedef struct trc_ctx trc_ctx;
 typedef struct trc_scene trc_scene;
@@ -273,6 +273,9 @@ trc_status trc_plan_create(trc_ctx* ctx, const trc_job* jobs, int32_t n, int32_t
 /* Paths per tile of the persistent launch (multiple of 64; 0 = built-in heuristic). */
 trc_status trc_plan_set_tile(trc_plan* plan, uint32_t tile_paths);
 uint32_t trc_plan_tile(const trc_plan* plan);
+/* A plan owns one spill buffer and one set of gradient copies: launches of
+ * the same plan must not overlap (launch it on one stream, or order the
+ * streams); different plans may run concurrently. */
 trc_status trc_plan_launch(trc_plan* plan, trc_stream stream);
 /* Autotuner: launches the plan `reps` times per candidate tile size, keeps
  * the fastest (median of HIP-event times) and returns it in *best.  The

@@ -26,7 +26,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 LIB = os.path.join(HERE, "libtrc.so")
 
 HIP_SOURCES = ["kernels/render.hip", "kernels/boundary.hip", "kernels/optim.hip"]
-HOST_SOURCES = ["host/scene_host.cpp", "host/bvh.cpp", "capi.cpp", "capi_boundary.cpp"]
+HOST_SOURCES = ["host/scene_host.cpp", "host/bvh.cpp", "capi.cpp"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
@@ -59,7 +59,7 @@ def _compile(src):
 
 def build_lib(jobs=8):
     os.makedirs(BUILD, exist_ok=True)
-    srcs = [s for s in HIP_SOURCES + HOST_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    srcs = HIP_SOURCES + HOST_SOURCES
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"

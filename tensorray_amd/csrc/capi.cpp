@@ -45,11 +45,6 @@ trc_status guarded(trc_ctx* ctx, F&& f) {
 constexpr int kSpillDepth = 64;
 constexpr int kBlocksPerCU = 2 * kFwdWaves;  // 128-lane blocks: 2 waves each, 4 SIMDs
 
-std::map<std::pair<trc_ctx*, hipStream_t>, uint32_t*>& spill_map() {
-  static std::map<std::pair<trc_ctx*, hipStream_t>, uint32_t*> m;
-  return m;
-}
-
 }  // namespace
 
 namespace trc {
@@ -92,20 +87,20 @@ LaunchCfg make_cfg(trc_ctx* ctx, const trc_scene* s, hipStream_t st, long long w
   L.grid = (int)(need < max_grid ? (need < 1 ? 1 : need) : max_grid);
   int stack_need = s->dev.stack_need;
   if (stack_need > kStackLds + kSpillDepth)
-    throw Fail(TRC_ERR_INVALID, "BVH too deep for the traversal stack (" + std::to_string(stack_need) + ")");
+    throw Fail(TRC_ERR_OVERFLOW, "BVH too deep for the traversal stack (" + std::to_string(stack_need) + ")");
   L.cap = kStackLds + kSpillDepth;
-  L.err = ctx->err_flag;
   L.cus = ctx->num_cus;
   L.spill = nullptr;
   if (!with_spill) return L;
+  // one spill buffer per (context, stream): launches on different streams
+  // never share one (word 0 doubles as the chunk counter of the adjoint)
   std::lock_guard<std::mutex> lk(ctx->mu);
-  auto key = std::make_pair(ctx, st);
-  auto it = spill_map().find(key);
-  if (it == spill_map().end()) {
+  auto it = ctx->spill.find(st);
+  if (it == ctx->spill.end()) {
     uint32_t* p = nullptr;
     size_t words = (size_t)kSpillDepth * (size_t)max_grid * kBlock;
     hip_check(hipMalloc(&p, words * 4), "hipMalloc(spill)");
-    it = spill_map().emplace(key, p).first;
+    it = ctx->spill.emplace(st, p).first;
   }
   L.spill = it->second;
   return L;
@@ -254,8 +249,6 @@ trc_status trc_context_create(int32_t device, trc_ctx** out) {
     c->device = device;
     hip_check(hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device),
               "hipDeviceGetAttribute");
-    hip_check(hipMalloc(&c->err_flag, 4), "hipMalloc(err)");
-    hip_check(hipMemset(c->err_flag, 0, 4), "hipMemset");
   });
   if (st != TRC_OK) {
     // keep the context so the caller can read the message
@@ -270,14 +263,8 @@ trc_status trc_context_destroy(trc_ctx* ctx) {
   if (!ctx) return TRC_ERR_INVALID;
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
-    for (auto it = spill_map().begin(); it != spill_map().end();) {
-      if (it->first.first == ctx) {
-        hipFree(it->second);
-        it = spill_map().erase(it);
-      } else {
-        ++it;
-      }
-    }
+    for (auto& kv : ctx->spill) hipFree(kv.second);
+    ctx->spill.clear();
   }
   for (auto& b : ctx->batch) {
     if (b.done) {
@@ -287,7 +274,6 @@ trc_status trc_context_destroy(trc_ctx* ctx) {
     if (b.host) hipHostFree(b.host);
     if (b.dev) hipFree(b.dev);
   }
-  if (ctx->err_flag) hipFree(ctx->err_flag);
   delete ctx;
   return TRC_OK;
 }

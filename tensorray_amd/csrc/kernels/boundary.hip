@@ -31,7 +31,6 @@ __device__ inline TraceCtx make_trace_ctx_b(uint32_t* lds, const LaunchCfg& L) {
   t.spill = L.spill + gtid;
   t.spill_stride = L.grid * kBlock;
   t.cap = L.cap;
-  t.err = L.err;
   t.n_nodes = t.n_tris = t.n_queries = 0u;
   return t;
 }

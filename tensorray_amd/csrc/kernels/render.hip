@@ -26,7 +26,6 @@ __device__ inline TraceCtx make_trace_ctx(uint32_t* lds, const LaunchCfg& L) {
   t.spill = L.spill + gtid;
   t.spill_stride = L.grid * kBlock;
   t.cap = L.cap;
-  t.err = L.err;
   t.n_nodes = t.n_tris = t.n_queries = 0u;
   return t;
 }
@@ -50,7 +49,7 @@ __global__ void __launch_bounds__(kBlock) k_intersect(SceneDev S, LaunchCfg L, c
 template <bool kCkpt, bool kMerged>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kFwdWaves))) k_render_paths(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
                                                          uint64_t first, uint64_t count, uint32_t tile, float* paths,
-                                                         CkptDev ck) {
+                                                         uint64_t paths_first, CkptDev ck) {
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx(lds_stack, L);
   // paths [first, first + count) in tiles of `tile` paths, dealt to the waves round-robin
@@ -59,9 +58,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kFw
   const uint64_t ntiles = (count + tile - 1) / tile;
   TileSupply sup = TileSupply::make(wave, ntiles, nwaves, 0, first, first + count, tile);
   render_paths_regen<kCkpt, kMerged>(S, C, P, tc, sup, ck, [&](uint64_t p, F3 c) {
-    paths[3 * p] = c.x;
-    paths[3 * p + 1] = c.y;
-    paths[3 * p + 2] = c.z;
+    const uint64_t q = p - paths_first;
+    paths[3 * q] = c.x;
+    paths[3 * q + 1] = c.y;
+    paths[3 * q + 2] = c.z;
   });
 }
 
@@ -524,15 +524,16 @@ constexpr uint64_t kMergedMaxPaths = 1ull << 25;
 
 template <bool kCkpt, bool kMerged>
 static void launch_paths(const SceneDev& S, const CameraDev& C, const PathParams& P, const LaunchCfg& L,
-                         uint64_t first, uint64_t count, float* paths, const CkptDev& ck, hipStream_t st) {
+                         uint64_t first, uint64_t count, float* paths, uint64_t paths_first, const CkptDev& ck,
+                         hipStream_t st) {
   const LaunchCfg Lk = fit_grid(k_render_paths<kCkpt, kMerged>, L);
   hipLaunchKernelGGL((k_render_paths<kCkpt, kMerged>), dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, first,
-                     count, forward_tile(count, Lk.grid), paths, ck);
+                     count, forward_tile(count, Lk.grid), paths, paths_first, ck);
 }
 
 hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                                const LaunchCfg& L, float* paths, void* ckpt, hipStream_t st, uint64_t first,
-                               uint64_t count) {
+                               uint64_t count, uint64_t paths_first) {
   PathParams P{seed, max_bounces, spp, 0u};
   const uint64_t np = (uint64_t)spp * (uint64_t)C.width * (uint64_t)C.height;
   if (count == 0) {
@@ -542,11 +543,11 @@ hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t s
   const CkptDev ck = ckpt ? ckpt_view(ckpt, np, max_bounces) : CkptDev{};
   const bool merged = max_bounces >= kMergedMinBounces && count <= kMergedMaxPaths;
   if (ckpt) {
-    if (merged) launch_paths<true, true>(S, C, P, L, first, count, paths, ck, st);
-    else launch_paths<true, false>(S, C, P, L, first, count, paths, ck, st);
+    if (merged) launch_paths<true, true>(S, C, P, L, first, count, paths, paths_first, ck, st);
+    else launch_paths<true, false>(S, C, P, L, first, count, paths, paths_first, ck, st);
   } else {
-    if (merged) launch_paths<false, true>(S, C, P, L, first, count, paths, ck, st);
-    else launch_paths<false, false>(S, C, P, L, first, count, paths, ck, st);
+    if (merged) launch_paths<false, true>(S, C, P, L, first, count, paths, paths_first, ck, st);
+    else launch_paths<false, false>(S, C, P, L, first, count, paths, paths_first, ck, st);
   }
   return hipGetLastError();
 }

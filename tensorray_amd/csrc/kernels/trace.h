@@ -98,7 +98,6 @@ struct TraceCtx {
   uint32_t* spill;      // &spill[gtid]
   int spill_stride;     // total threads of the grid
   int cap;              // total stack capacity (lds_depth + spill depth)
-  uint32_t* err;        // set to 1 if a traversal ever overflows (never expected)
   // work counters (node visits, triangle tests, queries); read only by the
   // statistics kernel, otherwise dead code
   uint32_t n_nodes, n_tris, n_queries;
@@ -187,7 +186,9 @@ __device__ inline Hit trace_ray_rt(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d,
       if (kc[0] != kEmpty) {
         int npush = (kc[1] != kEmpty) + (kc[2] != kEmpty) + (kc[3] != kEmpty);
         if (sp + npush > ctx.cap) {
-          *ctx.err = 1u;
+          // unreachable: the host refuses a launch whose BVH needs more than
+          // `cap` entries (stack_need, computed at build time); this branch
+          // only keeps the stack in bounds
           inner_done = true;
         } else {
           if (kc[3] != kEmpty) stack_push(ctx, sp, kc[3]);

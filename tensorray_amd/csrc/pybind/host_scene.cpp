@@ -827,11 +827,19 @@ static const float* dldi_device(const Scene& s, const Tensorf& dLdI, std::shared
   return hold->p;
 }
 
+// Key of a checkpoint: the scene's process-unique id and version, and the
+// exact bits of every camera float (two views that differ in the last bit
+// get different keys), the seed, depth and spp.
 static std::string ckpt_key_of(const Scene& s, const trc_camera& c, const trc_options& o) {
   std::ostringstream k;
-  k << &s << ':' << s.version << ':' << c.width << 'x' << c.height << ':' << c.fov;
-  for (int i = 0; i < 3; ++i) k << ':' << c.pos[i] << ',' << c.target[i] << ',' << c.up[i];
-  k << ':' << o.seed << ':' << o.max_bounces << ':' << o.spp;
+  auto bits = [](float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+  };
+  k << s.uid << ':' << s.version << ':' << c.width << 'x' << c.height << ':' << std::hex << bits(c.fov);
+  for (int i = 0; i < 3; ++i) k << ':' << bits(c.pos[i]) << ',' << bits(c.target[i]) << ',' << bits(c.up[i]);
+  k << std::dec << ':' << o.seed << ':' << o.max_bounces << ':' << o.spp;
   return k.str();
 }
 

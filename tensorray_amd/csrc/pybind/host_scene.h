@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <atomic>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
@@ -206,6 +207,7 @@ struct Scene {
   std::unique_ptr<DevArray> dpos, dmat, dtex;
   bool grads_pending = false;
   uint64_t version = 0;  // bumped by every configure()
+  const uint64_t uid = next_uid();  // unique per Scene object of the process (checkpoint keys)
 
   Scene() = default;
   ~Scene();
@@ -216,6 +218,12 @@ struct Scene {
   int num_bsdf() const { return (int)bsdfs.size(); }
   void ensure_grad_buffers();
   void fold_gradients();  // moves device gradients into the parameter pool
+
+ private:
+  static uint64_t next_uid() {
+    static std::atomic<uint64_t> n{0};
+    return ++n;
+  }
 };
 
 // ---- parameter pool (Tensor.h:1408-1419, 1675-1685) ----

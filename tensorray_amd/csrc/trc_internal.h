@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -14,10 +15,9 @@ struct trc_ctx {
   int device = 0;
   int num_cus = 0;
   std::string last_error;
-  uint32_t* err_flag = nullptr;   // device: traversal overflow flag
-  // traversal stack spill, grown on demand: [depth][threads]
-  uint32_t* spill = nullptr;
-  size_t spill_words = 0;
+  // traversal stack spill per stream ([depth][threads] of the largest grid),
+  // allocated on a stream's first launch; guarded by mu
+  std::map<hipStream_t, uint32_t*> spill;
   std::mutex mu;
   // job tables of batched launches: a ring of pinned staging + device
   // buffers, each reused once the batch that last used it has finished
@@ -94,7 +94,6 @@ struct LaunchCfg {
   int grid;             // persistent grid size (blocks)
   uint32_t* spill;      // spill buffer [depth][grid*kBlock]
   int cap;              // stack capacity
-  uint32_t* err;
   int cus;              // compute units of the device
 };
 
@@ -178,9 +177,11 @@ hipError_t launch_render_batch(const JobDev* jobs_dev, const JobDev* jobs_host, 
 
 // ckpt: null or a path checkpoint of checkpoint_bytes(spp*W*H, max_bounces)
 // bytes.  first / count: the range of path indices rendered (count 0: all).
+// paths_first: the path index whose radiance goes to paths[0] (0: paths holds
+// the request's whole [spp*W*H][3]).
 hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                                const LaunchCfg& L, float* paths, void* ckpt, hipStream_t st, uint64_t first = 0,
-                               uint64_t count = 0);
+                               uint64_t count = 0, uint64_t paths_first = 0);
 uint64_t checkpoint_bytes(uint64_t npaths, int max_bounces);
 hipError_t launch_render_adj_ckpt(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                                   const LaunchCfg& L, const void* ckpt, const float* dLdI, float* dpos, float* dmat,
