@@ -340,3 +340,93 @@ def laplacian_system(faces, nv, lam):
     deg = np.asarray(A.sum(axis=1)).ravel()
     L = sp.diags(deg) - A
     return (sp.identity(nv) + lam * L).tocsr()
+
+
+# ---- BSDFs in a bare local frame (normal +z), for closed-form checks ----
+def _decl_bsdf_local(L):
+    if getattr(L, "_bsdf_local_decl", False):
+        return
+    P = ctypes.POINTER
+    f32 = ctypes.c_float
+    L.oracle_bsdf_local.argtypes = [ctypes.c_int, f32, P(f32), P(f32), P(f32), P(f32), ctypes.c_int, P(f32), P(f32)]
+    L.oracle_bsdf_sample_local.argtypes = [ctypes.c_int, f32, P(f32), P(f32), ctypes.c_int, P(f32), P(f32)]
+    L.oracle_vndf.argtypes = [P(f32), f32, P(f32), ctypes.c_int, P(f32), P(f32)]
+    L.oracle_smith_g.argtypes = [P(f32), P(f32), f32, ctypes.c_int, P(f32)]
+    L.oracle_mf_parts.argtypes = [P(f32), ctypes.c_int, f32, P(f32)]
+    L._bsdf_local_decl = True
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, np.float32)
+    return a.reshape(shape) if shape is not None else a
+
+
+def bsdf_local(mat_type, rough, albedo, spec, wo, wi):
+    """BSDF::Eval (no shading correction) and BSDF::Pdf of one material for
+    local directions wo, wi [n, 3].  Returns (f [n, 3], pdf [n])."""
+    L = lib()
+    _decl_bsdf_local(L)
+    wo = _f32(wo, (-1, 3))
+    wi = _f32(wi, (-1, 3))
+    n = len(wo)
+    f = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    a = _f32(albedo)
+    s = _f32(spec)
+    fp = lambda x: _p(x, ctypes.c_float)
+    L.oracle_bsdf_local(int(mat_type), float(rough), fp(a), fp(s), fp(wo), fp(wi), n, fp(f), fp(pdf))
+    return f, pdf
+
+
+def bsdf_sample_local(mat_type, rough, wo, u):
+    """BSDF::SampleOnly for local wo [n, 3] and samples u [n, 2]: (wi, pdf)."""
+    L = lib()
+    _decl_bsdf_local(L)
+    wo = _f32(wo, (-1, 3))
+    u = _f32(u, (-1, 2))
+    n = len(wo)
+    wi = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    fp = lambda x: _p(x, ctypes.c_float)
+    L.oracle_bsdf_sample_local(int(mat_type), float(rough), fp(wo), fp(u), n, fp(wi), fp(pdf))
+    return wi, pdf
+
+
+def vndf(v, alpha, u):
+    """BSDF::GGX_VNDF_Sample and GGX_VNDF_Pdf for local v [n, 3] and samples
+    u [n, 2]: (wh [n, 3], pdf [n])."""
+    L = lib()
+    _decl_bsdf_local(L)
+    v = _f32(v, (-1, 3))
+    u = _f32(u, (-1, 2))
+    n = len(v)
+    wh = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    fp = lambda x: _p(x, ctypes.c_float)
+    L.oracle_vndf(fp(v), float(alpha), fp(u), n, fp(wh), fp(pdf))
+    return wh, pdf
+
+
+def smith_g(v, wh, alpha):
+    """BSDF::Smith_G(v, wh, alpha) for local v, wh [n, 3]."""
+    L = lib()
+    _decl_bsdf_local(L)
+    v = _f32(v, (-1, 3))
+    wh = _f32(wh, (-1, 3))
+    g = np.zeros(len(v), np.float32)
+    fp = lambda x: _p(x, ctypes.c_float)
+    L.oracle_smith_g(fp(v), fp(wh), float(alpha), len(v), fp(g))
+    return g
+
+
+def microfacet_parts(cos_h, vh, cos_n, rough):
+    """Microfacet::EvalInner's factors: GGX D (alpha = r^2) at cos_h, the
+    Fresnel weight 2^{vh (-5.55473 vh - 6.8316)} at vh, Smith-Schlick G1
+    (k = (r + 1)^2 / 8) at cos_n.  Returns three arrays."""
+    L = lib()
+    _decl_bsdf_local(L)
+    a = np.stack(np.broadcast_arrays(*(np.asarray(x, np.float32) for x in (cos_h, vh, cos_n))), -1)
+    a = _f32(a, (-1, 3))
+    out = np.zeros_like(a)
+    L.oracle_mf_parts(_p(a, ctypes.c_float), len(a), float(rough), _p(out, ctypes.c_float))
+    return out[:, 0], out[:, 1], out[:, 2]

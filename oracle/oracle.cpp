@@ -864,6 +864,21 @@ template <class S> static S fresnel_conductor(S ci, S eta, S k) {
   return S(0.5f) * (Rp2 + Rs2);
 }
 
+// Microfacet::EvalInner's three factors (Microfacet.cpp:49-63), as the
+// reference writes them: GGX NDF with alpha = r^2 in its
+// alpha / (cos^2 (alpha^2 - 1) + 1) form, Schlick's Fresnel with the
+// spherical-Gaussian exponent 2^{vh (-5.55473 vh - 6.8316)}, and the
+// Smith-Schlick G1 with k = (r + 1)^2 / 8.
+template <class S> static S mf_ggx(S nh, S alpha) {
+  S tmp = alpha / (nh * nh * (alpha * alpha - S(1.0f)) + S(1.0f));
+  return tmp * tmp * S(INV_PI);
+}
+template <class S> static S mf_fresnel_weight(S vh) {
+  S coeff = vh * (vh * S(-5.55473f) - S(6.8316f));
+  return exp2_(coeff);
+}
+template <class S> static S mf_smith_g1(S ndot, S k) { return ndot / (ndot * (S(1.0f) - k) + k); }
+
 // BSDF value (BSDF::Eval without correction), templated for AD.
 template <class S>
 static V3<S> bsdf_eval(Access<S>& A, const Its<S>& it, V3<S> wo_w, V3<S> wi_w) {
@@ -901,15 +916,13 @@ static V3<S> bsdf_eval(Access<S>& A, const Its<S>& it, V3<S> wo_w, V3<S> wi_w) {
   S r = A.rough(it.mat);
   S alpha = r * r;
   S k = (r + S(1.0f)) * (r + S(1.0f)) * S(0.125f);
-  S tmp = alpha / (nh * nh * (alpha * alpha - S(1.0f)) + S(1.0f));
-  S ggx = tmp * tmp * S(INV_PI);
-  S coeff = vh * (vh * S(-5.55473f) - S(6.8316f));
-  S e = exp2_(coeff);
+  S ggx = mf_ggx(nh, alpha);
+  S e = mf_fresnel_weight(vh);
   V3<S> F0 = A.spec(it.mat);
   V3<S> one{S(1.0f), S(1.0f), S(1.0f)};
   V3<S> fres = F0 + (one - F0) * e;
-  S g1 = nv / (nv * (S(1.0f) - k) + k);
-  S g2 = nl / (nl * (S(1.0f) - k) + k);
+  S g1 = mf_smith_g1(nv, k);
+  S g2 = mf_smith_g1(nl, k);
   S G = g1 * g2;
   V3<S> num = (ggx * G) * fres;
   S den = S(4.0f) * abs_(nv) * abs_(nl);
@@ -2170,4 +2183,89 @@ void oracle_render_adj(void* sp, const float* cam, int w, int h, uint32_t seed, 
   }
 }
 
+}  // extern "C"
+
+// ---- BSDFs in a bare local frame (shading frame = world axes, normal +z):
+// checks of the BSDF restatement against closed forms ----
+static Scene one_material_scene(int type, float rough, const float* albedo, const float* spec) {
+  Scene s;
+  Mat m;
+  m.type = type;
+  m.tex = -1;
+  m.roughness = rough;
+  m.albedo = F3{albedo[0], albedo[1], albedo[2]};
+  m.spec = F3{spec[0], spec[1], spec[2]};
+  s.mats.push_back(m);
+  return s;
+}
+static Its<float> local_its() {
+  Its<float> it;
+  it.p = F3{0.0f, 0.0f, 0.0f};
+  it.ng = it.ns = F3{0.0f, 0.0f, 1.0f};
+  it.t = F3{1.0f, 0.0f, 0.0f};
+  it.b = F3{0.0f, 1.0f, 0.0f};
+  it.tu = it.tv = 0.0f;
+  it.tri = 0;
+  it.mat = 0;
+  it.emitter = -1;
+  return it;
+}
+
+extern "C" {
+// f [n][3] = BSDF::Eval (no correction), pdf [n] = BSDF::Pdf for local wo, wi.
+void oracle_bsdf_local(int type, float rough, const float* albedo, const float* spec, const float* wo,
+                       const float* wi, int n, float* f, float* pdf) {
+  Scene s = one_material_scene(type, rough, albedo, spec);
+  Access<float> A(s);
+  Its<float> it = local_its();
+  for (int i = 0; i < n; ++i) {
+    F3 o{wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]}, w{wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]};
+    F3 v = bsdf_eval(A, it, o, w);
+    f[3 * i] = v.x;
+    f[3 * i + 1] = v.y;
+    f[3 * i + 2] = v.z;
+    pdf[i] = bsdf_pdf(s, it, o, w);
+  }
+}
+// BSDF::SampleOnly for local wo and sample pairs u [n][2]: wi [n][3], pdf [n].
+void oracle_bsdf_sample_local(int type, float rough, const float* wo, const float* u, int n, float* wi,
+                              float* pdf) {
+  const float zero[3] = {0.0f, 0.0f, 0.0f};
+  Scene s = one_material_scene(type, rough, zero, zero);
+  Its<float> it = local_its();
+  for (int i = 0; i < n; ++i) {
+    F3 o{wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]};
+    F3 w = bsdf_sample(s, it, o, u[2 * i], u[2 * i + 1], &pdf[i]);
+    wi[3 * i] = w.x;
+    wi[3 * i + 1] = w.y;
+    wi[3 * i + 2] = w.z;
+  }
+}
+// GGX_VNDF_Sample (half vector) and GGX_VNDF_Pdf (BSDF.cpp:262-302) with
+// alpha = a, for local v and sample pairs u [n][2]: wh [n][3], pdf [n].
+void oracle_vndf(const float* v, float a, const float* u, int n, float* wh, float* pdf) {
+  for (int i = 0; i < n; ++i) {
+    F3 vv{v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+    F3 h = VNDF_sample(vv, a, u[2 * i], u[2 * i + 1]);
+    wh[3 * i] = h.x;
+    wh[3 * i + 1] = h.y;
+    wh[3 * i + 2] = h.z;
+    pdf[i] = VNDF_pdf(vv, h, a);
+  }
+}
+// BSDF::Smith_G(v, wh, a) (BSDF.cpp:243-256) for local v [n][3], wh [n][3].
+void oracle_smith_g(const float* v, const float* wh, float a, int n, float* g) {
+  for (int i = 0; i < n; ++i)
+    g[i] = Smith_G(F3{v[3 * i], v[3 * i + 1], v[3 * i + 2]}, F3{wh[3 * i], wh[3 * i + 1], wh[3 * i + 2]}, a);
+}
+// The Microfacet factors: in [n][3] = (cos_h, vh, cos) and roughness r;
+// out [n][3] = (GGX D with alpha = r^2, Fresnel weight 2^{...}, Smith-Schlick G1).
+void oracle_mf_parts(const float* in, int n, float r, float* out) {
+  const float alpha = r * r, k = (r + 1.0f) * (r + 1.0f) * 0.125f;
+  for (int i = 0; i < n; ++i) {
+    out[3 * i] = mf_ggx(in[3 * i], alpha);
+    out[3 * i + 1] = mf_fresnel_weight(in[3 * i + 1]);
+    out[3 * i + 2] = mf_smith_g1(in[3 * i + 2], k);
+  }
+}
 }  // extern "C"
