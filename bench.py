@@ -48,8 +48,11 @@ def parse():
     p.add_argument("--serve-requests", type=int, default=48,
                    help="requests of the served-vs-plain comparison (0 skips it)")
     p.add_argument("--serve-sweep", action="store_true",
-                   help="also run the serving sweep: 4 resident scenes (1k / 20k / 200k / 1M triangles), "
-                        "res U(64..1024), spp U(4..64), depth U(2..6), Poisson rates 0.25..1.25 x plain capacity")
+                   help="serving workload over all load points: 4 resident scenes (1k / 20k / 200k / 1M "
+                        "triangles), res U(64..1024), spp U(4..64), depth U(2..6), Poisson rates 0.25..1.25 x "
+                        "plain capacity (default: saturation and the 0.75 load point only)")
+    p.add_argument("--no-serve-mix", action="store_true",
+                   help="skip the serving workload (4 scenes, mixed sizes) entirely")
     p.add_argument("--serve-large-scenes", action="store_true",
                    help="serve over 1k / 20k / 200k / 1M-triangle scenes (slower set-up) instead of 1k / 20k")
     p.add_argument("--multiview-views", type=int, default=32,
@@ -89,13 +92,45 @@ def algorithmic_bytes(stats, spp_paths, B, n_light_tris):
              bytes_per_query=b_query)
 
 
+def host_cpu_info():
+    """What the host offers: logical CPUs, the CPUs this process may run on,
+    the CPU model, and the thread share granted to one GPU's job (the
+    OMP_NUM_THREADS the job runs under; all usable CPUs when unset)."""
+    info = dict(os_cpu_count=os.cpu_count())
+    try:
+        info["usable_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        info["usable_cpus"] = os.cpu_count()
+    try:
+        import subprocess
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=10).stdout.strip())
+    except (OSError, ValueError, subprocess.SubprocessError):
+        info["nproc"] = None
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    info["cpu_model"] = model
+    share = os.environ.get("OMP_NUM_THREADS")
+    usable = info["usable_cpus"] or 1
+    info["thread_share"] = min(usable, int(share)) if share and share.isdigit() and int(share) > 0 else usable
+    return info
+
+
 def cpu_baseline(spec, cam, W, H, B, spp, cpu_spp, runs):
-    """The oracle (scalar C++ restatement, one thread per host core up to
-    16) on the same workload: the first cpu_spp samples of every pixel,
-    forward + adjoint; median of `runs` timed runs after one warm-up."""
+    """The oracle (scalar C++ restatement, one std::thread per host CPU of
+    this job's share) on the same workload: the first cpu_spp samples of
+    every pixel, forward + adjoint; median of `runs` timed runs after one
+    warm-up."""
     import numpy as np
     import oracle
-    threads = max(1, min(16, os.cpu_count() or 1))
+    host = host_cpu_info()
+    threads = host["thread_share"]
     S = oracle.Scene(spec)
     n = cpu_spp * W * H
     dl = np.full((3, W * H), 0.01, np.float32)
@@ -112,7 +147,10 @@ def cpu_baseline(spec, cam, W, H, B, spp, cpu_spp, runs):
     tot, tf, ta = times[len(times) // 2]
     return dict(value=n / tot / 1e6, unit="Mpaths/s", cores=threads, kind="port",
                 sample=f"{cpu_spp} spp x {W}x{H} = {n} paths of the same scene, depth {B + 1}, forward + adjoint "
-                       f"(replay); median of {runs} runs after a warm-up: fwd {tf:.2f} s, adj {ta:.2f} s")
+                       f"(replay); median of {runs} runs after a warm-up: fwd {tf:.2f} s, adj {ta:.2f} s; "
+                       f"{threads} threads = this job's CPU share (OMP_NUM_THREADS) of {host['usable_cpus']} "
+                       f"usable CPUs",
+                host=host)
 
 
 def served_comparison(ctx, a):
@@ -170,7 +208,7 @@ def served_comparison(ctx, a):
     return out
 
 
-def served_sweep(ctx, a):
+def served_sweep(ctx, a, loads=(0.25, 0.5, 0.75, 1.0, 1.25)):
     """Serving over four resident scenes of 1,280 / 20,480 / 200,000 /
     1,003,520 triangles (displaced spheres on a ground plane): requests with
     resolution uniform over 64..1024, spp over 4..64, depth over 2..6
@@ -223,7 +261,7 @@ def served_sweep(ctx, a):
     for mode in modes[1:]:
         out[mode + "_over_plain"] = out[mode]["mpaths_per_s"] / out["plain"]["mpaths_per_s"]
     sweep = []
-    for load in (0.25, 0.5, 0.75, 1.0, 1.25):
+    for load in loads:
         row = dict(load=load, rate_per_s=load * cap)
         for mode in modes:
             rs = reqs(load * cap, 2025)
@@ -459,9 +497,19 @@ def main():
         except (OSError, ValueError):
             traffic = None
     copy_gbs = streaming_copy_gbs()
+    # `achieved` / `frac` are the modeled (algorithmic) bytes over the kernel
+    # time; `dram_gbs` / `dram_frac` are the DRAM bytes the counters measured
+    # for the same kernel (FETCH_SIZE + WRITE_SIZE per launch, profiles/)
+    # over the same time.  Most modeled bytes are BVH node reads that hit
+    # in L2 / MALL, so the kernel is latency and issue bound, not DRAM bound.
+    dram_gbs = traffic / (kms * 1e-3) / 1e9 if traffic else None
     roofline = dict(bound="hbm", achieved=achieved / 1e9, peak=MI355X_HBM_PEAK / 1e9, unit="GB/s",
                     achievable_copy=copy_gbs, frac_of_achievable=achieved / 1e9 / copy_gbs,
-                    frac=achieved / MI355X_HBM_PEAK, traffic=traffic, kernel=kname, kernel_ms=kms,
+                    frac=achieved / MI355X_HBM_PEAK, traffic=traffic, dram_gbs=dram_gbs,
+                    dram_frac=dram_gbs * 1e9 / MI355X_HBM_PEAK if dram_gbs else None,
+                    achieved_is="modeled algorithmic bytes / kernel time; dram_gbs = measured DRAM bytes / "
+                                "kernel time",
+                    kernel=kname, kernel_ms=kms,
                     algorithmic_bytes_per_launch=kbytes, fwd_kernel_ms=fwd_avg, adj_kernel_ms=adj_avg, **st)
     if a.copy_calibration:
         src = torch.ones(a.copy_calibration * (1 << 20) // 4, device="cuda")
@@ -484,7 +532,12 @@ def main():
                 if a.boundary_spp > 0 else None)
     served = optional("served", served_comparison) if a.serve_requests > 0 else None
     multi = optional("multiview", multiview_comparison) if a.multiview_views > 0 else None
-    sweep = optional("served_sweep", served_sweep) if a.serve_sweep else None
+    if a.no_serve_mix:
+        sweep = None
+    elif a.serve_sweep:
+        sweep = optional("served_sweep", served_sweep)
+    else:
+        sweep = optional("served_sweep", lambda c, aa: served_sweep(c, aa, loads=(0.75,)))
     cpu = None
     if not a.no_cpu_baseline:
         try:

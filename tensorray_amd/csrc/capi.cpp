@@ -294,6 +294,11 @@ trc_status trc_scene_destroy(trc_scene* s) {
   if (!s) return TRC_ERR_INVALID;
   if (s->ctx->device >= 0) {
     hipSetDevice(s->ctx->device);
+    if (s->prim_done) {
+      hipEventSynchronize(s->prim_done);
+      hipEventDestroy(s->prim_done);
+      s->prim_done = nullptr;
+    }
     free_scene_bufs(s);
   }
   delete s;
@@ -1105,6 +1110,8 @@ static void boundary_call(trc_scene* s, const trc_camera* cam, const trc_options
   hipStream_t st = (hipStream_t)stream;
   EdgeSet prim{};
   if (kind == 0) {
+    // the previous call's uploads may still read the host staging
+    if (s->prim_done) hip_check(hipEventSynchronize(s->prim_done), "hipEventSynchronize");
     select_prim_edges(s->host, C.eye, s->prim_edges);
     if (s->prim_edges.empty()) return;
     prim.integral = edge_distribution(s->host, s->prim_edges, s->prim_len, s->prim_cdf);
@@ -1123,6 +1130,8 @@ static void boundary_call(trc_scene* s, const trc_camera* cam, const trc_options
     hip_check(hipMemcpyAsync(base, s->prim_edges.data(), eb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
     hip_check(hipMemcpyAsync(base + eb, s->prim_len.data(), fb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
     hip_check(hipMemcpyAsync(base + eb + fb, s->prim_cdf.data(), fb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    if (!s->prim_done) hip_check(hipEventCreateWithFlags(&s->prim_done, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventRecord(s->prim_done, st), "hipEventRecord");
     prim.e = (const EdgeDev*)base;
     prim.len = (const float*)(base + eb);
     prim.cdf = (const float*)(base + eb + fb);
@@ -1184,7 +1193,6 @@ static void boundary_call(trc_scene* s, const trc_camera* cam, const trc_options
   hip_check(launch_boundary(kind, s->dev, C, opt->seed, opt->max_bounces, spp, L, prim, guide, dLdI, dpos, tpos,
                             dimg, s->bq_buf.ptr, st),
             "k_boundary");
-  if (kind == 0) hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
 }
 
 trc_status trc_render_boundary(trc_scene* s, const trc_camera* cam, const trc_options* opt, int32_t kind,

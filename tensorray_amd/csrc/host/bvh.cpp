@@ -56,12 +56,8 @@ struct Builder {
 
   Builder(const std::vector<float>& p, const std::vector<uint32_t>& i) : pos(p), idx(i) {}
 
-  void prepare() {
-    const size_t T = idx.size() / 3;
-    tbox.resize(T);
-    cent.resize(3 * T);
-    order.resize(T);
-    for (size_t t = 0; t < T; ++t) {
+  void prepare_range(size_t lo, size_t hi) {
+    for (size_t t = lo; t < hi; ++t) {
       Box b;
       for (int j = 0; j < 3; ++j) b.grow(&pos[3 * (size_t)idx[3 * t + j]]);
       tbox[t] = b;
@@ -70,22 +66,42 @@ struct Builder {
     }
   }
 
-  int32_t make_leaf(uint32_t first, uint32_t count, const Box& box) {
+  // prepare() on `threads` threads over contiguous ranges (same result)
+  void prepare_par(unsigned threads) {
+    const size_t T = idx.size() / 3;
+    tbox.resize(T);
+    cent.resize(3 * T);
+    order.resize(T);
+    if (threads <= 1 || T < ((size_t)1 << 16)) {
+      prepare_range(0, T);
+      return;
+    }
+    std::vector<std::thread> th;
+    for (unsigned i = 0; i < threads; ++i) th.emplace_back([&, i] { prepare_range(T * i / threads, T * (i + 1) / threads); });
+    for (auto& x : th) x.join();
+  }
+
+  static int32_t make_leaf(std::vector<BinNode>& out, uint32_t first, uint32_t count, const Box& box) {
     BinNode n;
     n.box = box;
     n.first = first;
     n.count = count;
-    nodes.push_back(n);
-    return (int32_t)nodes.size() - 1;
+    out.push_back(n);
+    return (int32_t)out.size() - 1;
   }
 
-  int32_t build(uint32_t first, uint32_t count) {
+  // The subtree over order[first, first + count) appended to `out` in
+  // preorder (node, left subtree, right subtree); its root is the first node
+  // appended.  With par_depth > 0 the two subtrees of a large node are built
+  // on two threads into their own arrays and appended left then right, which
+  // gives the same array as the sequential build.
+  int32_t build_into(std::vector<BinNode>& out, uint32_t first, uint32_t count, int par_depth) {
     Box box, cbox;
     for (uint32_t i = first; i < first + count; ++i) {
       box.grow(tbox[order[i]]);
       cbox.grow(&cent[3 * (size_t)order[i]]);
     }
-    if (count <= 1) return make_leaf(first, count, box);
+    if (count <= 1) return make_leaf(out, first, count, box);
     double best_cost = DBL_MAX;
     int best_axis = -1, best_split = -1;
     for (int axis = 0; axis < 3; ++axis) {
@@ -124,7 +140,7 @@ struct Builder {
     double parent_area = std::max(box.area(), 1e-30);
     double leaf_cost = kCostIntersect * count;
     double split_cost = best_axis >= 0 ? kCostTraverse + kCostIntersect * best_cost / parent_area : DBL_MAX;
-    if (count <= (uint32_t)kMaxLeaf && leaf_cost <= split_cost) return make_leaf(first, count, box);
+    if (count <= (uint32_t)kMaxLeaf && leaf_cost <= split_cost) return make_leaf(out, first, count, box);
     uint32_t mid;
     if (best_axis < 0) {
       // all centroids coincide: split the range in half
@@ -140,15 +156,37 @@ struct Builder {
       mid = (uint32_t)(it - order.begin());
       if (mid == first || mid == first + count) mid = first + count / 2;
     }
-    int32_t self = (int32_t)nodes.size();
-    nodes.push_back(BinNode{});
-    nodes[self].box = box;
-    int32_t l = build(first, mid - first);
-    int32_t r = build(mid, first + count - mid);
-    nodes[self].left = l;
-    nodes[self].right = r;
+    int32_t self = (int32_t)out.size();
+    out.push_back(BinNode{});
+    out[self].box = box;
+    if (par_depth > 0 && count >= kParallelMin) {
+      std::vector<BinNode> lv, rv;
+      std::thread th([&] { build_into(lv, first, mid - first, par_depth - 1); });
+      build_into(rv, mid, first + count - mid, par_depth - 1);
+      th.join();
+      const int32_t l = (int32_t)out.size();
+      for (BinNode n : lv) {
+        if (!n.leaf()) { n.left += l; n.right += l; }
+        out.push_back(n);
+      }
+      const int32_t r = (int32_t)out.size();
+      for (BinNode n : rv) {
+        if (!n.leaf()) { n.left += r; n.right += r; }
+        out.push_back(n);
+      }
+      out[self].left = l;
+      out[self].right = r;
+      return self;
+    }
+    int32_t l = build_into(out, first, mid - first, 0);
+    int32_t r = build_into(out, mid, first + count - mid, 0);
+    out[self].left = l;
+    out[self].right = r;
     return self;
   }
+
+  // Subtrees smaller than this are built on the calling thread.
+  static constexpr uint32_t kParallelMin = 1u << 15;
 };
 
 // Outward rounding of a box so that the slab test stays conservative even
@@ -176,6 +214,7 @@ struct Collapser {
   const std::vector<float>& pos;
   const std::vector<uint32_t>& idx;
   BvhBuildStats& st;
+  std::vector<int> depth_of;  // depth of every 4-wide node (root 1)
 
   // Emits one leaf's triangles, splitting ranges larger than kMaxLeafTris.
   uint32_t emit_leaf(const BinNode& n) {
@@ -201,6 +240,8 @@ struct Collapser {
   // `slot_index`; returns the worst-case stack need below it.
   int collapse(int32_t b, uint32_t out_index, int depth) {
     st.max_depth = std::max(st.max_depth, depth);
+    if (depth_of.size() <= out_index) depth_of.resize(out_index + 1, 0);
+    depth_of[out_index] = depth;
     std::vector<int32_t> kids = {bn[b].left, bn[b].right};
     while (kids.size() < 4) {
       int best = -1;
@@ -247,14 +288,18 @@ struct Collapser {
 
 void bvh_build(const std::vector<float>& pos, const std::vector<uint32_t>& idx,
                std::vector<Bvh4Node>& nodes, std::vector<TriRecord>& tris,
-               std::vector<uint32_t>& order, BvhBuildStats& stats) {
+               std::vector<uint32_t>& order, BvhBuildStats& stats, BvhRefitPlan* plan) {
   const size_t T = idx.size() / 3;
   if (T == 0) throw std::runtime_error("bvh_build: no triangles");
   if (T >= (size_t)kLeafFirstMask) throw std::runtime_error("bvh_build: too many triangles");
   Builder b(pos, idx);
-  b.prepare();
+  // threads: up to 16, as 2^par_depth subtree builders
+  const unsigned hc = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  int par_depth = 0;
+  while ((2u << par_depth) <= hc) ++par_depth;
+  b.prepare_par(hc);
   b.nodes.reserve(2 * T);
-  int32_t root = b.build(0, (uint32_t)T);
+  int32_t root = b.build_into(b.nodes, 0, (uint32_t)T, par_depth);
   stats = BvhBuildStats{};
   nodes.clear();
   tris.clear();
@@ -272,6 +317,19 @@ void bvh_build(const std::vector<float>& pos, const std::vector<uint32_t>& idx,
     stats.stack_need = 1;
   } else {
     stats.stack_need = c.collapse(root, 0, 1) + 1;
+  }
+  // refit plan: nodes grouped by depth, deepest level first
+  if (plan) {
+    c.depth_of.resize(nodes.size(), 1);
+    const int maxd = *std::max_element(c.depth_of.begin(), c.depth_of.end());
+    plan->level_off.assign(1, 0u);
+    plan->nodes.clear();
+    plan->nodes.reserve(nodes.size());
+    for (int d = maxd; d >= 1; --d) {
+      for (size_t i = 0; i < nodes.size(); ++i)
+        if (c.depth_of[i] == d) plan->nodes.push_back((uint32_t)i);
+      plan->level_off.push_back((uint32_t)plan->nodes.size());
+    }
   }
   // the triangle order is the leaf order of the records
   order.resize(T);

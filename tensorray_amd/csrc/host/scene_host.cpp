@@ -258,7 +258,7 @@ float edge_distribution(const SceneHost& s, const std::vector<EdgeDev>& e, std::
   return integral;
 }
 
-void SceneHost::build_bvh() { bvh_build(pos, idx, nodes, tris, tri_order, bvh_stats); }
+void SceneHost::build_bvh() { bvh_build(pos, idx, nodes, tris, tri_order, bvh_stats, &refit_plan); }
 
 void SceneHost::refit_bvh() { bvh_refit(pos, idx, tri_order, nodes, tris); }
 

@@ -45,6 +45,14 @@ struct BvhBuildStats {
   double sah_cost = 0.0;
 };
 
+// Refit order of a 4-wide BVH: node ids grouped by depth, deepest level
+// first; level k holds nodes[level_off[k] .. level_off[k+1]).  Every child
+// of a node sits one level deeper, so the levels can be refit in turn.
+struct BvhRefitPlan {
+  std::vector<uint32_t> nodes;
+  std::vector<uint32_t> level_off;
+};
+
 struct SceneHost {
   std::vector<MeshHost> meshes;
   std::vector<Material> materials;
@@ -76,6 +84,7 @@ struct SceneHost {
   std::vector<TriRecord> tris;
   std::vector<uint32_t> tri_order;
   BvhBuildStats bvh_stats;
+  BvhRefitPlan refit_plan;
 
   int32_t num_vertices() const { return (int32_t)(pos.size() / 3); }
   int32_t num_triangles() const { return (int32_t)(idx.size() / 3); }
@@ -109,9 +118,11 @@ float edge_distribution(const SceneHost& s, const std::vector<EdgeDev>& e, std::
 std::vector<EdgeHost> build_edge_list(const std::vector<uint32_t>& idx);
 
 // BVH build (binned SAH, collapsed to 4-wide).
+// Subtrees are built on up to 16 threads; the result does not depend on the
+// thread count.  `plan` (optional) receives the refit order.
 void bvh_build(const std::vector<float>& pos, const std::vector<uint32_t>& idx,
                std::vector<Bvh4Node>& nodes, std::vector<TriRecord>& tris,
-               std::vector<uint32_t>& order, BvhBuildStats& stats);
+               std::vector<uint32_t>& order, BvhBuildStats& stats, BvhRefitPlan* plan = nullptr);
 void bvh_refit(const std::vector<float>& pos, const std::vector<uint32_t>& idx,
                const std::vector<uint32_t>& order, std::vector<Bvh4Node>& nodes,
                std::vector<TriRecord>& tris);

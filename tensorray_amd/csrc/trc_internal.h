@@ -52,6 +52,9 @@ struct trc_scene {
   std::vector<trc::EdgeDev> prim_edges;
   std::vector<float> prim_len, prim_cdf;
   DevBuf prim_buf;
+  // recorded after the last upload from that staging: the next call waits
+  // for it before overwriting the staging (instead of a stream sync per call)
+  hipEvent_t prim_done = nullptr;
   // guiding of the direct boundary term: per-sample values, leaf weights,
   // leaf CDF and the weight integral
   DevBuf guide_buf;

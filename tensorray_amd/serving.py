@@ -51,6 +51,14 @@ class Request:
     target: object = None           # torch [3][H*W] on the device; dL/dI = I - target (zeros if None)
     arrival: float = 0.0            # seconds after the start of the run
     rid: int = -1
+    # edge samples per pixel of the four boundary integrators (RenderOptions
+    # sppe / sppe0 / sppse0 / sppse1); their gradients add into dpos next to
+    # the interior adjoint, as the reference's backward sums every
+    # integrator's renderD (renderer_single_view.py:74-76)
+    sppe: int = 0
+    sppe0: int = 0
+    sppse0: int = 0
+    sppse1: int = 0
     # results
     film: object = None             # torch [3][H*W]
     dpos: object = None             # torch [V][3]
@@ -66,6 +74,22 @@ class Request:
     def latency(self):
         return None if self.t_done is None else self.t_done - self.arrival
 
+    def options(self):
+        return _capi.make_options(seed=self.seed, max_bounces=self.max_bounces, spp=self.spp, sppe=self.sppe,
+                                  sppse0=self.sppse0, sppse1=self.sppse1, sppe0=self.sppe0)
+
+    def edge_terms(self):
+        """(integrator, samples per pixel) of the boundary terms asked for."""
+        return [(k, n) for k, n in (("primary", self.sppe), ("pixel", self.sppe0), ("direct", self.sppse0),
+                                    ("indirect", self.sppse1)) if n > 0]
+
+
+def run_edge_terms(ds, req, cam, opt, dldi_ptr, dpos_ptr, stream):
+    """Adds the request's boundary-term gradients (each integrator's
+    renderD) into dpos, stream-ordered after dL/dI."""
+    for kind, _ in req.edge_terms():
+        ds.render_boundary(cam, opt, kind, dldi_ptr, dpos_ptr, stream)
+
 
 class _Active:
     """Device state of an admitted request."""
@@ -75,7 +99,7 @@ class _Active:
         self.ds = ds
         info = ds.info()
         self.cam = _capi.make_camera(req.width, req.height, **req.cam)
-        self.opt = _capi.make_options(seed=req.seed, max_bounces=req.max_bounces, spp=req.spp)
+        self.opt = req.options()
         P = req.paths
         self.nb = ds.checkpoint_bytes(self.cam, self.opt)
         with torch.cuda.stream(stream):
@@ -143,6 +167,8 @@ class Server:
     def submit(self, req):
         if req.scene not in self.scenes:
             raise KeyError(f"unknown scene {req.scene!r}")
+        if req.sppe0 and (req.sppe0 * req.width * req.height) % 4:
+            raise ValueError(f"request {req.rid}: sppe0 * W * H must be a multiple of 4")
         need = _Active.bytes_needed(req, self.scenes[req.scene])
         if need > self.budget:
             raise ValueError(f"request {req.rid} needs {need} bytes, more than the whole budget {self.budget}")
@@ -234,6 +260,9 @@ class Server:
                 else:
                     a.dldi.copy_(a.film)
                 a.req.film = a.film
+                # boundary terms need only dL/dI: they run in this step,
+                # beside the first chunk of the interior adjoint
+                run_edge_terms(a.ds, a.req, a.cam, a.opt, a.dldi.data_ptr(), a.req.dpos.data_ptr(), self.stream)
         if ajobs:
             self.ctx.render_batch(ajobs, adjoint=True, stream=self.stream)
         for a in adone:
@@ -286,7 +315,7 @@ def run_plain(ctx, scenes, requests, realtime=True):
         ds = scenes[r.scene]
         info = ds.info()
         cam = _capi.make_camera(r.width, r.height, **r.cam)
-        opt = _capi.make_options(seed=r.seed, max_bounces=r.max_bounces, spp=r.spp)
+        opt = r.options()
         nb = ds.checkpoint_bytes(cam, opt)
         ck = torch.empty(nb, dtype=torch.uint8, device="cuda")
         r.film = torch.empty(3, r.width * r.height, device="cuda")
@@ -296,6 +325,7 @@ def run_plain(ctx, scenes, requests, realtime=True):
         r.dmat = torch.zeros(info.num_materials, 8, device="cuda")
         ds.render_adj_ckpt(cam, opt, ck.data_ptr(), nb, dldi.data_ptr(), r.dpos.data_ptr(), r.dmat.data_ptr(),
                            stream=stream)
+        run_edge_terms(ds, r, cam, opt, dldi.data_ptr(), r.dpos.data_ptr(), stream)
         torch.cuda.synchronize()
         r.t_done = time.perf_counter() - t0
     return reqs
@@ -307,10 +337,13 @@ def percentiles(values, qs=(50, 99)):
     return {f"p{q}": float(np.percentile(v, q)) for q in qs}
 
 
-def make_requests(scene_names, n, rate, seed=2024, res=(64, 1024), spp=(4, 64), depth=(2, 6), cams=None):
+def make_requests(scene_names, n, rate, seed=2024, res=(64, 1024), spp=(4, 64), depth=(2, 6), cams=None,
+                  edge_spp=0):
     """Poisson arrivals (mean `rate` requests / s) of requests with
     resolution, spp and depth drawn uniformly (numpy PCG64(seed)); depth d
-    means max_bounces d - 1."""
+    means max_bounces d - 1.  edge_spp > 0 asks every request for the four
+    boundary terms at that many edge samples per pixel (the pixel term
+    rounded up so that sppe0 * W * H is a multiple of 4)."""
     import numpy as np
     rng = np.random.Generator(np.random.PCG64(seed))
     t = 0.0
@@ -324,4 +357,10 @@ def make_requests(scene_names, n, rate, seed=2024, res=(64, 1024), spp=(4, 64), 
                            cam=dict(cams[0]) if cams else dict(pos=(0.0, 1.0, 3.0), target=(0.0, 0.0, 0.0),
                                                                up=(0.0, 1.0, 0.0), fov=45.0),
                            arrival=t, rid=i))
+        if edge_spp > 0:
+            r = out[-1]
+            r.sppe = r.sppse0 = r.sppse1 = edge_spp
+            r.sppe0 = edge_spp
+            while (r.sppe0 * side * side) % 4:
+                r.sppe0 += 1
     return out

@@ -94,3 +94,58 @@ def test_largesteps_step_is_composition(gpu_ctx, oracle_mod):
 def inverse_faces(s):
     from tensorray_amd import scenes
     return scenes.geodesic_sphere(s.freq)[1]
+
+
+def test_config5_full_step(gpu_ctx, oracle_mod):
+    """One step of the inverse-rendering workload at full size: the
+    1,997,120-triangle geodesic sphere, 16 views at 512 x 512, 256 spp,
+    depth 8, primary-edge and guided direct-edge terms at 16 samples per
+    pixel.  The step's gradient equals the sum of its parts run by hand (up
+    to float-atomic order), the positions move by the oracle's Adam step bit
+    for bit, and 20,000 paths of one view (four ranges of 5,000) equal the
+    oracle's per-path radiance bit for bit."""
+    import torch
+    from tensorray_amd import scenes
+    s = inverse.Setup()
+    s.steps = 1
+    assert (s.freq, s.views, s.res, s.spp, s.depth) == (316, 16, 512, 256, 8)
+    prob = inverse.make_problem(gpu_ctx, s, log=lambda m: print("config5:", m, flush=True))
+    assert prob.ds.info().num_triangles == 1997124
+    x0 = prob.x.cpu().numpy().copy()
+    W = H = s.res
+    g = torch.zeros_like(prob.x)
+    film = torch.empty(3, W * H, device="cuda")
+    for k, cam in enumerate(prob.cams):
+        opt = prob._options(1, k)
+        prob.ds.render_fwd_ckpt(cam, opt, film.data_ptr(), prob.ckpt.data_ptr(), prob.ckpt.numel())
+        dl = (film - prob.targets[k]) * (2.0 / (3 * W * H))
+        prob.ds.render_adj_ckpt(cam, opt, prob.ckpt.data_ptr(), prob.ckpt.numel(), dl.data_ptr(), g.data_ptr())
+        prob.ds.render_boundary(cam, opt, "primary", dl.data_ptr(), g.data_ptr())
+        prob.ds.render_boundary(cam, opt, "direct", dl.data_ptr(), g.data_ptr())
+    torch.cuda.synchronize()
+    print("config5: step 1 by hand done", flush=True)
+    # per-path radiance of view 3 at step 1 (before the step moves the mesh)
+    k = 3
+    opt = prob._options(1, k)
+    paths = torch.empty(W * H * s.spp, 3, device="cuda")
+    prob.ds.render_paths(prob.cams[k], opt, paths.data_ptr())
+    firsts = [(W * H * s.spp - 5000) * q // 3 for q in range(4)]
+    got_paths = [paths[f:f + 5000].cpu().numpy() for f in firsts]
+    del paths
+    rec = prob.step()
+    print("config5: step", rec, flush=True)
+    gref = g.cpu().numpy()
+    scale = np.abs(gref).max()
+    assert scale > 0
+    assert np.abs(prob.grad.cpu().numpy() - gref).max() <= 1e-4 * scale
+    z = np.zeros_like(x0)
+    x1, _, _ = oracle_mod.adam_step(x0, prob.grad.cpu().numpy(), z, z, 1, lr=s.lr)
+    assert np.array_equal(prob.x.cpu().numpy(), x1)
+    assert np.array_equal(prob.ds.vertices(), x1)
+    prob.close()
+    pos, idx = scenes.geodesic_sphere(s.freq)
+    S = oracle_mod.Scene(inverse.shape_scene(pos, idx))
+    cam_d = scenes.fibonacci_cameras(s.views, radius=3.0, seed=0)[k]
+    for f, got in zip(firsts, got_paths):
+        ref = S.render_paths(cam_d, W, H, opt.seed, s.depth - 1, f, 5000)
+        assert np.array_equal(got, ref), (f, float(np.abs(got - ref).max()))

@@ -27,7 +27,7 @@ def _setup(ctx, smooth=False):
     views = multiview.random_views(cams, seed=1, res=(16, 40), spp_choices=(2, 4), depth=3)
     g = torch.Generator(device="cuda").manual_seed(5)
     targets = [torch.rand(3, v.width * v.height, device="cuda", generator=g) for v in views]
-    return ds, views, targets
+    return ds, views, targets, spec
 
 
 def _single_view_reference(ds, views, targets):
@@ -53,9 +53,10 @@ def _single_view_reference(ds, views, targets):
 
 
 @pytest.mark.parametrize("smooth", [False, True])
-def test_multiview_matches_single_views(gpu_ctx, smooth):
+def test_multiview_matches_single_views(gpu_ctx, oracle_mod, smooth):
+    import numpy as np
     import torch
-    ds, views, targets = _setup(gpu_ctx, smooth)
+    ds, views, targets, spec = _setup(gpu_ctx, smooth)
     films, dpos, dmat, dtex = _single_view_reference(ds, views, targets)
     assert dpos.abs().max().item() > 0 and dtex.abs().max().item() > 0
     mv = multiview.MultiView(gpu_ctx, ds, views, targets)
@@ -66,13 +67,27 @@ def test_multiview_matches_single_views(gpu_ctx, smooth):
         assert torch.equal(mv.film[k], films[k]), k
     assert _close(mv.dpos, dpos) and _close(mv.dmat, dmat) and _close(mv.dtex, dtex)
     assert (mv.vn == 0).all()
+    # every view's film equals the oracle's, and the summed gradients the
+    # oracle's sum over views
+    S = oracle_mod.Scene(spec)
+    ref = np.zeros((ds.info().num_vertices, 3))
+    for k, v in enumerate(views):
+        want = S.render_film(v.cam, v.width, v.height, v.seed, v.max_bounces, v.spp)
+        assert np.array_equal(mv.film[k].cpu().numpy(), want), k
+        dl = want - targets[k].cpu().numpy()
+        ref += S.render_adj(v.cam, v.width, v.height, v.seed, v.max_bounces, v.spp, dl)[0]
+    m = spec.diff_vertex_mask()
+    got = mv.dpos.cpu().numpy().astype(np.float64)[m]
+    scale = np.abs(ref[m]).max()
+    assert scale > 0
+    assert (np.abs(got - ref[m]) <= 1e-3 * np.abs(ref[m]) + 1e-5 * scale).all()
     mv.close()
     ds.close()
 
 
 def test_multiview_graph_replays(gpu_ctx):
     import torch
-    ds, views, targets = _setup(gpu_ctx)
+    ds, views, targets, _ = _setup(gpu_ctx)
     films, dpos, dmat, dtex = _single_view_reference(ds, views, targets)
     mv = multiview.MultiView(gpu_ctx, ds, views, targets)
     mv.capture()
@@ -88,5 +103,45 @@ def test_multiview_graph_replays(gpu_ctx):
         for k in range(len(views)):
             assert torch.equal(mv.film[k], films[k]), (rep, k)
         assert _close(mv.dpos, dpos) and _close(mv.dmat, dmat) and _close(mv.dtex, dtex), rep
+    mv.close()
+    ds.close()
+
+
+def test_config3_workload_matches_single_views_and_oracle(gpu_ctx, oracle_mod):
+    """The multi-view workload at full size: 32 views on a Fibonacci sphere
+    of radius 3 (seed 0) around the 200,000-triangle displaced geodesic
+    sphere with a 1024^2 Perlin albedo texture, square resolutions uniform
+    over 128..512, spp from {8, 16, 32}, depth 4, tuned tiles and a captured
+    graph.  Every view's film equals its single-view render bit for bit, the
+    summed gradients equal the sum of the single-view adjoints up to
+    float-atomic order, and 20,000 paths of two views (four ranges of 5,000
+    spread over each view) equal the oracle's per-path radiance bit for bit."""
+    import numpy as np
+    import torch
+    spec = scenes.multiview_scene(freq=100, amplitude=0.05, noise_seed=1, texture_res=1024, texture_seed=7)
+    ds = _capi.DeviceScene(gpu_ctx, spec)
+    assert ds.info().num_triangles == 200002
+    cams = scenes.fibonacci_cameras(32, radius=3.0, seed=0)
+    views = multiview.random_views(cams, seed=0, res=(128, 512), spp_choices=(8, 16, 32), depth=4)
+    g = torch.Generator(device="cuda").manual_seed(11)
+    targets = [torch.rand(3, v.width * v.height, device="cuda", generator=g) for v in views]
+    print(f"config3: {sum(v.paths for v in views)} paths over {len(views)} views", flush=True)
+    mv = multiview.MultiView(gpu_ctx, ds, views, targets)
+    mv.tune((128, 256, 1024), reps=1)
+    mv.capture()
+    mv.replay()
+    torch.cuda.synchronize()
+    films, dpos, dmat, dtex = _single_view_reference(ds, views, targets)
+    for k in range(len(views)):
+        assert torch.equal(mv.film[k], films[k]), k
+    assert _close(mv.dpos, dpos) and _close(mv.dmat, dmat) and _close(mv.dtex, dtex)
+    S = oracle_mod.Scene(spec)
+    for k in (0, len(views) // 2):
+        v = views[k]
+        for q in range(4):
+            first = (v.paths - 5000) * q // 3
+            ref = S.render_paths(v.cam, v.width, v.height, v.seed, v.max_bounces, first, 5000)
+            got = mv.paths[k][first:first + 5000].cpu().numpy()
+            assert np.array_equal(got, ref), (k, first, float(np.abs(got - ref).max()))
     mv.close()
     ds.close()
