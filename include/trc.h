@@ -151,6 +151,18 @@ trc_status trc_intersect(trc_scene* scene, const float* rays, int32_t n, int32_t
  * film: [3][H*W] device, overwritten.  */
 trc_status trc_render_fwd(trc_scene* scene, const trc_camera* cam, const trc_options* opt, float* film,
                           trc_stream stream);
+/* trc_render_fwd without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The film
+ * is bit-identical to trc_render_fwd's.  Its per-path radiance and traversal
+ * spill live in one fixed 64 MiB device array per GPU, held until the
+ * process ends; the samples are rendered in chunks that fit it.  Fails with
+ * TRC_ERR_INVALID when one sample of every pixel (12 B x W x H) plus the
+ * scene's spill slice does not fit (about 5.5M pixels).  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other contexts or threads) race and give wrong films. */
+trc_status trc_render_fwd_noalloc(trc_scene* scene, const trc_camera* cam, const trc_options* opt, float* film,
+                                  trc_stream stream);
 /* Per-path radiance (before the 1/spp factor), [spp*H*W][3], path index =
  * sample * H*W + pixel.  Used by tests and the serving loop. */
 trc_status trc_render_paths(trc_scene* scene, const trc_camera* cam, const trc_options* opt, float* paths,

@@ -78,6 +78,8 @@ SIGNATURES = [
     ("trc_scene_get_edges", ctypes.c_int32, [vp, c_i32_p]),
     ("trc_intersect", ctypes.c_int32, [vp, vp, ctypes.c_int32, vp, vp, ctypes.c_int32, vp]),
     ("trc_render_fwd", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options), vp, vp]),
+    ("trc_render_fwd_noalloc", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options), vp,
+                                                vp]),
     ("trc_render_paths", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options), vp, vp]),
     ("trc_film_from_paths", ctypes.c_int32, [vp, vp, ctypes.c_int32, ctypes.c_int32, vp, vp]),
     ("trc_render_stats", ctypes.c_int32, [vp, ctypes.POINTER(trc_camera), ctypes.POINTER(trc_options),
@@ -410,6 +412,10 @@ class DeviceScene:
     def render_fwd(self, cam, opt, film_ptr, stream=None):
         self.ctx.check(self.ctx.lib.trc_render_fwd(self.h, ctypes.byref(cam), ctypes.byref(opt), film_ptr,
                                                    _stream_ptr(stream)), "trc_render_fwd")
+
+    def render_fwd_noalloc(self, cam, opt, film_ptr, stream=None):
+        self.ctx.check(self.ctx.lib.trc_render_fwd_noalloc(self.h, ctypes.byref(cam), ctypes.byref(opt), film_ptr,
+                                                           _stream_ptr(stream)), "trc_render_fwd_noalloc")
 
     def render_paths(self, cam, opt, paths_ptr, stream=None):
         self.ctx.check(self.ctx.lib.trc_render_paths(self.h, ctypes.byref(cam), ctypes.byref(opt), paths_ptr,

@@ -45,6 +45,22 @@ trc_status guarded(trc_ctx* ctx, F&& f) {
 constexpr int kSpillDepth = 64;
 constexpr int kBlocksPerCU = 2 * kFwdWaves;  // 128-lane blocks: 2 waves each, 4 SIMDs
 
+// Address of the allocation-free forward's scratch on `device` (the current
+// device), looked up once per device: each GPU holds its own copy of the
+// array at its own address.
+void* device_scratch(int device) {
+  static std::mutex mu;
+  static std::map<int, void*> addr;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = addr.find(device);
+  if (it != addr.end()) return it->second;
+  void* p = nullptr;
+  hip_check(trc::noalloc_scratch_address(&p), "hipGetSymbolAddress(scratch)");
+  if (!p) throw Fail(TRC_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+  addr.emplace(device, p);
+  return p;
+}
+
 }  // namespace
 
 namespace trc {
@@ -249,6 +265,7 @@ trc_status trc_context_create(int32_t device, trc_ctx** out) {
     c->device = device;
     hip_check(hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device),
               "hipDeviceGetAttribute");
+    c->scratch = device_scratch(device);
   });
   if (st != TRC_OK) {
     // keep the context so the caller can read the message
@@ -548,6 +565,49 @@ trc_status trc_render_fwd(trc_scene* s, const trc_camera* cam, const trc_options
                           trc_stream stream) {
   if (!s) return TRC_ERR_INVALID;
   return guarded(s->ctx, [&] { render_fwd_impl(s, cam, opt, film, nullptr, 0, stream); });
+}
+
+trc_status trc_render_fwd_noalloc(trc_scene* s, const trc_camera* cam, const trc_options* opt, float* film,
+                                  trc_stream stream) {
+  if (!s) return TRC_ERR_INVALID;
+  return guarded(s->ctx, [&] {
+    check_render(s, opt);
+    CameraDev C = to_cam(cam);
+    if (!film) throw Fail(TRC_ERR_INVALID, "film is NULL");
+    trc_ctx* ctx = s->ctx;
+    if (!ctx->scratch) throw Fail(TRC_ERR_STATE, "context has no device scratch");
+    hipStream_t st = (hipStream_t)stream;
+    const uint64_t wh = (uint64_t)C.width * (uint64_t)C.height;
+    const uint64_t np = (uint64_t)opt->spp * wh;
+    // launch shape as trc_render_fwd's, but the traversal stack gets only the
+    // spill depth this scene's BVH needs, and the spill slice and the chunk
+    // of per-path radiance live in the fixed scratch
+    LaunchCfg L = make_cfg(ctx, s, st, (long long)np, /*with_spill=*/false);
+    const int cap = std::max(s->dev.stack_need, kStackLds);
+    L.cap = cap;
+    const uint64_t spill_bytes = ((uint64_t)(cap - kStackLds) * (uint64_t)L.grid * kBlock * 4 + 255) / 256 * 256;
+    const uint64_t total = trc::noalloc_scratch_bytes();
+    const uint64_t per_sample = 12 * wh;
+    if (spill_bytes + per_sample > total)
+      throw Fail(TRC_ERR_INVALID, "trc_render_fwd_noalloc: a " + std::to_string(C.width) + "x" +
+                                      std::to_string(C.height) + " image needs " +
+                                      std::to_string(spill_bytes + per_sample) + " bytes of scratch, more than its " +
+                                      std::to_string(total));
+    unsigned char* base = (unsigned char*)ctx->scratch;
+    L.spill = (uint32_t*)base;
+    float* chunk = (float*)(base + spill_bytes);
+    const int chunk_spp = (int)std::min<uint64_t>((total - spill_bytes) / per_sample, (uint64_t)opt->spp);
+    hipError_t e = hipSuccess;
+    for (int s0 = 0; s0 < opt->spp && e == hipSuccess; s0 += chunk_spp) {
+      const int ns = std::min(chunk_spp, opt->spp - s0);
+      const uint64_t first = (uint64_t)s0 * wh;
+      e = launch_render_paths(s->dev, C, opt->seed, opt->max_bounces, opt->spp, L, chunk, nullptr, st, first,
+                              (uint64_t)ns * wh, first);
+      if (e == hipSuccess)
+        e = launch_film_accum(chunk, ns, (int)wh, film, s0 == 0, s0 + ns == opt->spp, opt->spp, st);
+    }
+    hip_check(e, "k_render_paths/k_film_accum");
+  });
 }
 
 int64_t trc_checkpoint_bytes(const trc_camera* cam, const trc_options* opt) {

@@ -596,6 +596,55 @@ hipError_t launch_film_reduce(const float* paths, int spp, int wh, float* film, 
   return hipGetLastError();
 }
 
+// k_film_reduce over samples [s0, s0 + ns) only: paths[s][pix] holds sample
+// s0 + s.  The running per-pixel sums stay unscaled in film between chunks
+// and the last chunk applies 1/spp, so the additions are the same, in the
+// same order, as one k_film_reduce over all samples.
+__global__ void k_film_accum(const float* paths, int ns, int wh, float* film, int first_chunk, int last_chunk,
+                             int spp) {
+  int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= wh) return;
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  if (!first_chunk) {
+    sx = film[pix];
+    sy = film[wh + pix];
+    sz = film[2 * wh + pix];
+  }
+  for (int s = 0; s < ns; ++s) {
+    const float* p = paths + 3 * ((size_t)s * wh + pix);
+    sx += p[0];
+    sy += p[1];
+    sz += p[2];
+  }
+  if (last_chunk) {
+    const float inv = 1.0f / (float)spp;
+    sx *= inv;
+    sy *= inv;
+    sz *= inv;
+  }
+  film[pix] = sx;
+  film[wh + pix] = sy;
+  film[2 * wh + pix] = sz;
+}
+
+hipError_t launch_film_accum(const float* paths, int ns, int wh, float* film, bool first_chunk, bool last_chunk,
+                             int spp, hipStream_t st) {
+  hipLaunchKernelGGL(k_film_accum, dim3((wh + 255) / 256), dim3(256), 0, st, paths, ns, wh, film,
+                     first_chunk ? 1 : 0, last_chunk ? 1 : 0, spp);
+  return hipGetLastError();
+}
+
+// Scratch of trc_render_fwd_noalloc: the traversal spill slice and the
+// per-path radiance of one chunk of samples.  Host code never names this
+// array in a kernel argument or copy: it uses the address that
+// noalloc_scratch_address returns for the current device.
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+size_t noalloc_scratch_bytes() { return kNoAllocScratchBytes; }
+
+hipError_t noalloc_scratch_address(void** out) { return hipGetSymbolAddress(out, HIP_SYMBOL(scratch)); }
+
 hipError_t launch_render_adj(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                              const LaunchCfg& L, const float* dLdI, float* dpos, float* dmat, float* dtex,
                              float* vn_adj, const GradCopies& gc, hipStream_t st) {

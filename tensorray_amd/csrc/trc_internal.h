@@ -19,6 +19,9 @@ struct trc_ctx {
   // allocated on a stream's first launch; guarded by mu
   std::map<hipStream_t, uint32_t*> spill;
   std::mutex mu;
+  // this device's copy of the fixed scratch of trc_render_fwd_noalloc
+  // (hipGetSymbolAddress, looked up once per device at context creation)
+  void* scratch = nullptr;
   // job tables of batched launches: a ring of pinned staging + device
   // buffers, each reused once the batch that last used it has finished
   struct BatchSlot {
@@ -193,6 +196,16 @@ hipError_t launch_render_adj_ckpt(const SceneDev& S, const CameraDev& C, uint32_
 hipError_t launch_render_stats(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
                                const LaunchCfg& L, unsigned long long* counters, hipStream_t st);
 hipError_t launch_film_reduce(const float* paths, int spp, int wh, float* film, hipStream_t st);
+// One chunk of the same reduction: paths holds samples [s0, s0 + ns) of every
+// pixel; film carries the running (unscaled) sums between chunks, so chunking
+// the samples gives the same bits as launch_film_reduce.
+hipError_t launch_film_accum(const float* paths, int ns, int wh, float* film, bool first_chunk, bool last_chunk,
+                             int spp, hipStream_t st);
+// Fixed device scratch of the allocation-free forward (one array per device,
+// held until the process ends): its size, and its address on the current
+// device (hipGetSymbolAddress).
+size_t noalloc_scratch_bytes();
+hipError_t noalloc_scratch_address(void** out);
 // Guide of the direct boundary term: one leaf per secondary edge i over
 // x in [sec.cdf[i-1], sec.cdf[i]], y, z in [0, 1].
 struct GuideDev {
