@@ -91,6 +91,8 @@ typedef struct trc_scene_info {
   double bvh_sah_cost;
   int64_t device_bytes;
   int64_t texture_floats; /* size of the texture-gradient buffer of trc_render_adj */
+  int32_t bvh_refits;     /* trc_scene_set_vertices refits (on the device) since the scene was created */
+  int32_t bvh_rebuilds;   /* full rebuilds: refit = 0, or a refit tree whose SAH cost grew past 1.25x its build's */
 } trc_scene_info;
 
 /* ---- context: replaces EnvCreate / EnvRelease (pybind_utils.h:38-61) ---- */
@@ -124,7 +126,11 @@ trc_status trc_scene_configure(trc_scene* scene, trc_stream stream);
  * without any device work. */
 trc_status trc_scene_prepare(trc_scene* scene);
 /* Scene::Update (Scene.cpp:535-546): replace all vertex positions ([V][3],
- * host or device pointer) and refit (refit = 1) or rebuild (0) the BVH. */
+ * host or device pointer) and refit (refit = 1) or rebuild (0) the BVH.
+ * A refit runs on the device (normals, areas, emitter table, secondary
+ * edges, BVH boxes; the positions never travel to the host) and rebuilds the
+ * tree on the host when its SAH cost has grown past 1.25x its cost at the
+ * last build.  Synchronous with respect to `stream`. */
 trc_status trc_scene_set_vertices(trc_scene* scene, const float* pos, int32_t nv, int32_t refit,
                                   trc_stream stream);
 /* Set a material's roughness / albedo / specular (BSDF parameters). */

@@ -50,7 +50,8 @@ class trc_scene_info(ctypes.Structure):
                 ("num_edges", ctypes.c_int32), ("bvh_nodes", ctypes.c_int32), ("bvh_leaves", ctypes.c_int32),
                 ("bvh_max_depth", ctypes.c_int32), ("bvh_stack_need", ctypes.c_int32),
                 ("bvh_sah_cost", ctypes.c_double), ("device_bytes", ctypes.c_int64),
-                ("texture_floats", ctypes.c_int64)]
+                ("texture_floats", ctypes.c_int64), ("bvh_refits", ctypes.c_int32),
+                ("bvh_rebuilds", ctypes.c_int32)]
 
 
 # (name, restype, argtypes) of every exported entry point

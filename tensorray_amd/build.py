@@ -25,7 +25,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
 LIB = os.path.join(HERE, "libtrc.so")
 
-HIP_SOURCES = ["kernels/render.hip", "kernels/boundary.hip", "kernels/optim.hip"]
+HIP_SOURCES = ["kernels/render.hip", "kernels/boundary.hip", "kernels/optim.hip", "kernels/update.hip"]
 HOST_SOURCES = ["host/scene_host.cpp", "host/bvh.cpp", "capi.cpp"]
 
 HIP_FLAGS = [

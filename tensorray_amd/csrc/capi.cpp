@@ -141,11 +141,30 @@ void reupload(const T* dst, const std::vector<T>& v, hipStream_t st) {
             "hipMemcpyAsync");
 }
 
+// An uninitialised device buffer owned by the scene.
+void* alloc_dev(trc_scene* s, size_t bytes) {
+  DevBuf b;
+  b.bytes = bytes == 0 ? 16 : bytes;
+  hip_check(hipMalloc(&b.ptr, b.bytes), "hipMalloc(scene)");
+  s->bufs.push_back(b);
+  s->device_bytes += (int64_t)b.bytes;
+  return b.ptr;
+}
+
+// `v` padded with zero records to `cap` entries, uploaded.
+template <class T>
+T* upload_cap(trc_scene* s, std::vector<T> v, size_t cap, hipStream_t st) {
+  v.resize(std::max<size_t>(std::max(cap, v.size()), 1));
+  return upload(s, v, st);
+}
+
 void free_scene_bufs(trc_scene* s) {
   for (auto& b : s->bufs) hipFree(b.ptr);
   s->bufs.clear();
-  if (s->prim_buf.ptr) hipFree(s->prim_buf.ptr);
-  s->prim_buf = DevBuf{};
+  s->upd = SceneUpdateDev{};
+  s->prim_e = nullptr;
+  s->prim_len = s->prim_cdf = nullptr;
+  s->prim_cnt = nullptr;
   if (s->guide_buf.ptr) hipFree(s->guide_buf.ptr);
   s->guide_buf = DevBuf{};
   s->guide_key.valid = false;
@@ -183,9 +202,12 @@ void upload_scene(trc_scene* s, hipStream_t st) {
   d.light_cdf = upload(s, h.light_cdf, st);
   d.light_f = upload(s, h.light_f, st);
   d.emit_rad = upload(s, h.emit_rad, st);
-  d.sec.e = upload(s, h.sec_edges, st);
-  d.sec.len = upload(s, h.sec_len, st);
-  d.sec.cdf = upload(s, h.sec_cdf, st);
+  // secondary-edge arrays sized for every edge: a device update may select
+  // any number of them
+  const size_t E = h.edges.size();
+  d.sec.e = upload_cap(s, h.sec_edges, E, st);
+  d.sec.len = upload_cap(s, h.sec_len, E, st);
+  d.sec.cdf = upload_cap(s, h.sec_cdf, E, st);
   d.sec.n = (int32_t)h.sec_edges.size();
   d.sec.integral = h.sec_integral;
   d.n_light = (int32_t)h.light_tri.size();
@@ -213,8 +235,68 @@ void upload_scene(trc_scene* s, hipStream_t st) {
     std::vector<float> zeros(3 * (size_t)h.num_vertices(), 0.0f);
     s->vn_adj = (float*)upload(s, zeros, st);
   }
+  // device-side update: every edge, smooth faces per vertex, the refit plan
+  SceneUpdateDev& U = s->upd;
+  {
+    std::vector<EdgeDev> all(E);
+    for (size_t i = 0; i < E; ++i) {
+      const EdgeHost& e = h.edges[i];
+      all[i] = EdgeDev{e.v0, e.v1, e.v2, e.f0, e.f1, {0, 0, 0}};
+    }
+    U.all_edges = upload(s, all, st);
+    U.n_edges = (uint32_t)E;
+  }
+  {
+    std::vector<uint32_t> off(h.num_vertices() + 1, 0u), list;
+    auto smooth = [&](uint32_t t) { return (h.shade[t].flags & kTriSmooth) != 0u; };
+    for (uint32_t t = 0; t < T; ++t)
+      if (smooth(t))
+        for (int j = 0; j < 3; ++j) off[h.idx[3 * t + j] + 1]++;
+    for (size_t v = 0; v + 1 < off.size(); ++v) off[v + 1] += off[v];
+    if (off.back() > 0) {
+      list.resize(off.back());
+      std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+      for (uint32_t t = 0; t < T; ++t)
+        if (smooth(t))
+          for (int j = 0; j < 3; ++j) list[fill[h.idx[3 * t + j]]++] = t;
+      U.vs_off = upload(s, off, st);
+      U.vs_list = upload(s, list, st);
+    }
+  }
+  U.refit_nodes = upload(s, h.refit_plan.nodes, st);
+  U.level_off = h.refit_plan.level_off;
+  U.n_nodes = (uint32_t)h.nodes.size();
+  U.nbox = (float*)alloc_dev(s, 6 * sizeof(float) * (size_t)U.n_nodes);
+  U.scan = (uint32_t*)alloc_dev(s, 4 * (size_t)edge_scan_words((uint32_t)E));
+  {
+    std::vector<UpdateScalars> sc(1);
+    sc[0].sec = EdgeCount{(int32_t)h.sec_edges.size(), h.sec_integral};
+    sc[0].light_integral = h.light_integral;
+    sc[0].pad = 0.0f;
+    sc[0].sah = h.bvh_stats.sah_cost;
+    U.scal = upload(s, sc, st);
+  }
+  s->prim_e = (EdgeDev*)alloc_dev(s, sizeof(EdgeDev) * std::max<size_t>(E, 1));
+  s->prim_len = (float*)alloc_dev(s, sizeof(float) * std::max<size_t>(E, 1));
+  s->prim_cdf = (float*)alloc_dev(s, sizeof(float) * std::max<size_t>(E, 1));
+  s->prim_cnt = (EdgeCount*)alloc_dev(s, sizeof(EdgeCount));
+  s->sah_built = h.bvh_stats.sah_cost;
+  s->host_stale = false;
   s->dev = d;
   hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+// Brings the host copies of the positions and of everything derived from
+// them (normals, emitter table, secondary edges, BVH boxes) up to date after
+// device updates.  Only explicit queries and a rebuild need them.
+void sync_host(trc_scene* s) {
+  if (!s->host_stale) return;
+  hip_check(hipMemcpy(s->host.pos.data(), s->dev.pos, s->host.pos.size() * sizeof(float), hipMemcpyDeviceToHost),
+            "hipMemcpy(positions)");
+  s->host.update_positions(nullptr);
+  s->host.refit_bvh();
+  s->host.bvh_stats.sah_cost = s->upd_host ? s->upd_host->sah : s->host.bvh_stats.sah_cost;
+  s->host_stale = false;
 }
 
 CameraDev to_cam(const trc_camera* c) {
@@ -311,12 +393,9 @@ trc_status trc_scene_destroy(trc_scene* s) {
   if (!s) return TRC_ERR_INVALID;
   if (s->ctx->device >= 0) {
     hipSetDevice(s->ctx->device);
-    if (s->prim_done) {
-      hipEventSynchronize(s->prim_done);
-      hipEventDestroy(s->prim_done);
-      s->prim_done = nullptr;
-    }
     free_scene_bufs(s);
+    if (s->upd_host) hipHostFree(s->upd_host);
+    s->upd_host = nullptr;
   }
   delete s;
   return TRC_OK;
@@ -408,42 +487,53 @@ trc_status trc_scene_configure(trc_scene* s, trc_stream stream) {
   });
 }
 
+// A refit tree whose SAH cost grows past this factor of the cost it had
+// when it was built is rebuilt.
+constexpr double kRebuildFactor = 1.25;
+
 trc_status trc_scene_set_vertices(trc_scene* s, const float* pos, int32_t nv, int32_t refit, trc_stream stream) {
   if (!s) return TRC_ERR_INVALID;
   return guarded(s->ctx, [&] {
     if (!s->configured) throw Fail(TRC_ERR_STATE, "scene is not configured");
     if (!pos || nv != s->host.num_vertices()) throw Fail(TRC_ERR_INVALID, "vertex count mismatch");
     hipStream_t st = (hipStream_t)stream;
-    std::vector<float> tmp(3 * (size_t)nv);
-    hip_check(hipMemcpyAsync(tmp.data(), pos, tmp.size() * 4, hipMemcpyDefault, st), "hipMemcpyAsync");
-    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-    const size_t n_sec_before = s->host.sec_edges.size();
-    s->host.update_positions(tmp.data());
     SceneDev& d = s->dev;
-    if (refit && s->host.sec_edges.size() != n_sec_before) {
-      // the secondary edge list changed size: re-upload everything
-      s->host.refit_bvh();
-      upload_scene(s, st);
-    } else if (refit) {
-      reupload(d.sec.e, s->host.sec_edges, st);
-      reupload(d.sec.len, s->host.sec_len, st);
-      reupload(d.sec.cdf, s->host.sec_cdf, st);
-      d.sec.integral = s->host.sec_integral;
-      s->host.refit_bvh();
-      reupload(d.pos, s->host.pos, st);
-      reupload(d.vnrm, s->host.vnrm, st);
-      reupload(d.shade, s->host.shade, st);
-      reupload(d.tris, s->host.tris, st);
-      reupload(d.nodes, s->host.nodes, st);
-      reupload(d.light_cdf, s->host.light_cdf, st);
-      reupload(d.light_f, s->host.light_f, st);
-      d.light_integral = s->host.light_integral;
+    if (!refit) {
+      // rebuild: the host builds the tree over the new positions
+      hip_check(hipMemcpyAsync(s->host.pos.data(), pos, s->host.pos.size() * sizeof(float), hipMemcpyDefault, st),
+                "hipMemcpyAsync");
       hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-    } else {
+      s->host_stale = false;
+      s->host.update_positions(nullptr);
       s->host.build_bvh();
       upload_scene(s, st);
+      ++s->rebuilds;
+      ++s->version;
+      return;
     }
+    // refit on the device: positions in place, then every derived array and
+    // the BVH boxes; only the scalars the launches take by value (edge count,
+    // integrals) and the tree's SAH cost come back to the host
+    hip_check(hipMemcpyAsync((void*)d.pos, pos, s->host.pos.size() * sizeof(float), hipMemcpyDefault, st),
+              "hipMemcpyAsync(positions)");
+    hip_check(launch_scene_update(d, s->upd, st), "scene update kernels");
+    if (!s->upd_host) hip_check(hipHostMalloc((void**)&s->upd_host, sizeof(UpdateScalars)), "hipHostMalloc");
+    hip_check(hipMemcpyAsync(s->upd_host, s->upd.scal, sizeof(UpdateScalars), hipMemcpyDeviceToHost, st),
+              "hipMemcpyAsync(scalars)");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    d.sec.n = s->upd_host->sec.n;
+    d.sec.integral = s->upd_host->sec.integral;
+    d.light_integral = s->upd_host->light_integral;
+    s->host.bvh_stats.sah_cost = s->upd_host->sah;
+    s->host_stale = true;
+    ++s->refits;
     ++s->version;
+    if (s->upd_host->sah > kRebuildFactor * s->sah_built) {
+      sync_host(s);
+      s->host.build_bvh();
+      upload_scene(s, st);
+      ++s->rebuilds;
+    }
   });
 }
 
@@ -480,6 +570,8 @@ trc_status trc_scene_get_info(const trc_scene* s, trc_scene_info* out) {
   out->bvh_max_depth = h.bvh_stats.max_depth;
   out->bvh_stack_need = h.bvh_stats.stack_need;
   out->bvh_sah_cost = h.bvh_stats.sah_cost;
+  out->bvh_refits = s->refits;
+  out->bvh_rebuilds = s->rebuilds;
   out->device_bytes = s->device_bytes;
   int64_t tf = 0;
   for (size_t i = 0; i < h.textures.size(); ++i) tf += (int64_t)h.textures[i].size();
@@ -489,8 +581,11 @@ trc_status trc_scene_get_info(const trc_scene* s, trc_scene_info* out) {
 
 trc_status trc_scene_get_vertices(const trc_scene* s, float* out) {
   if (!s || !out) return TRC_ERR_INVALID;
-  std::memcpy(out, s->host.pos.data(), s->host.pos.size() * 4);
-  return TRC_OK;
+  trc_scene* m = const_cast<trc_scene*>(s);
+  return guarded(m->ctx, [&] {
+    sync_host(m);
+    std::memcpy(out, s->host.pos.data(), s->host.pos.size() * 4);
+  });
 }
 
 trc_status trc_scene_get_indices(const trc_scene* s, uint32_t* out) {
@@ -501,9 +596,12 @@ trc_status trc_scene_get_indices(const trc_scene* s, uint32_t* out) {
 
 trc_status trc_scene_get_bvh(const trc_scene* s, uint32_t* nodes, uint32_t* order) {
   if (!s) return TRC_ERR_INVALID;
-  if (nodes) std::memcpy(nodes, s->host.nodes.data(), s->host.nodes.size() * sizeof(Bvh4Node));
-  if (order) std::memcpy(order, s->host.tri_order.data(), s->host.tri_order.size() * 4);
-  return TRC_OK;
+  trc_scene* m = const_cast<trc_scene*>(s);
+  return guarded(m->ctx, [&] {
+    sync_host(m);
+    if (nodes) std::memcpy(nodes, s->host.nodes.data(), s->host.nodes.size() * sizeof(Bvh4Node));
+    if (order) std::memcpy(order, s->host.tri_order.data(), s->host.tri_order.size() * 4);
+  });
 }
 
 trc_status trc_scene_get_edges(const trc_scene* s, int32_t* out) {
@@ -1170,31 +1268,15 @@ static void boundary_call(trc_scene* s, const trc_camera* cam, const trc_options
   hipStream_t st = (hipStream_t)stream;
   EdgeSet prim{};
   if (kind == 0) {
-    // the previous call's uploads may still read the host staging
-    if (s->prim_done) hip_check(hipEventSynchronize(s->prim_done), "hipEventSynchronize");
-    select_prim_edges(s->host, C.eye, s->prim_edges);
-    if (s->prim_edges.empty()) return;
-    prim.integral = edge_distribution(s->host, s->prim_edges, s->prim_len, s->prim_cdf);
-    prim.n = (int32_t)s->prim_edges.size();
-    const size_t eb = s->prim_edges.size() * sizeof(EdgeDev), fb = s->prim_edges.size() * sizeof(float);
-    const size_t need = eb + 2 * fb;
-    if (s->prim_buf.bytes < need) {
-      hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-      if (s->prim_buf.ptr) hipFree(s->prim_buf.ptr);
-      s->prim_buf.ptr = nullptr;
-      s->prim_buf.bytes = 0;
-      hip_check(hipMalloc(&s->prim_buf.ptr, need), "hipMalloc(primary edges)");
-      s->prim_buf.bytes = need;
-    }
-    char* base = (char*)s->prim_buf.ptr;
-    hip_check(hipMemcpyAsync(base, s->prim_edges.data(), eb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-    hip_check(hipMemcpyAsync(base + eb, s->prim_len.data(), fb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-    hip_check(hipMemcpyAsync(base + eb + fb, s->prim_cdf.data(), fb, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-    if (!s->prim_done) hip_check(hipEventCreateWithFlags(&s->prim_done, hipEventDisableTiming), "hipEventCreate");
-    hip_check(hipEventRecord(s->prim_done, st), "hipEventRecord");
-    prim.e = (const EdgeDev*)base;
-    prim.len = (const float*)(base + eb);
-    prim.cdf = (const float*)(base + eb + fb);
+    // boundary and silhouette edges seen from the camera, selected on the
+    // device; the kernels read their count and length integral from there
+    if (s->upd.n_edges == 0) return;
+    hip_check(launch_prim_edges(s->dev, s->upd, C.eye, s->prim_e, s->prim_len, s->prim_cdf, s->prim_cnt, st),
+              "primary edge selection");
+    prim.e = s->prim_e;
+    prim.len = s->prim_len;
+    prim.cdf = s->prim_cdf;
+    prim.dyn = s->prim_cnt;
   }
   if (kind >= 2 && s->dev.sec.n == 0) return;
   GuideDev guide{};

@@ -101,14 +101,23 @@ struct alignas(16) EdgeDev {
   uint32_t pad[3];
 };
 
+// Size and length integral of an edge list built on the device.
+struct EdgeCount {
+  int32_t n;
+  float integral;
+};
+
 // A sampled edge list with its length distribution (Distribution1D over
-// |p1 - p0|, Edge.cpp:90-106 / 350-367).
+// |p1 - p0|, Edge.cpp:90-106 / 350-367).  When `dyn` is set, the list was
+// selected on the device and its size and integral are read from there
+// (n and integral are then unused).
 struct EdgeSet {
   const EdgeDev* e;
   const float* len;
   const float* cdf;
   int32_t n;
   float integral;
+  const EdgeCount* dyn;
 };
 
 // Texture descriptor: FP32 RGB texels, row-major [h][w][3], nearest lookup

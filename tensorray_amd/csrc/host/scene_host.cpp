@@ -229,16 +229,6 @@ void select_sec_edges(const SceneHost& s, std::vector<EdgeDev>& out) {
   });
 }
 
-void select_prim_edges(const SceneHost& s, const float eye[3], std::vector<EdgeDev>& out) {
-  const F3 c{eye[0], eye[1], eye[2]};
-  filter_edges(s.edges, out, [&](const EdgeHost& e) {
-    F3 d = c - vget(s.pos, e.v0);
-    bool ff0 = dot(d, face_n(s, e.f0)) > 0.0f;
-    bool ff1 = dot(d, face_n(s, e.f1)) > 0.0f;
-    return e.f1 < 0 || ff0 != ff1;
-  });
-}
-
 float edge_distribution(const SceneHost& s, const std::vector<EdgeDev>& e, std::vector<float>& len,
                         std::vector<float>& cdf) {
   const size_t n = e.size();

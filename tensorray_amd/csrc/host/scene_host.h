@@ -104,11 +104,10 @@ struct SceneHost {
 CameraDev make_camera(const float pos[3], const float target[3], const float up[3], int w, int h,
                       float fov_deg);
 
-// Edge filters of the boundary terms.
-// Secondary: boundary edges, or non-coplanar convex ones (Edge.cpp:42-88).
+// Edge filters of the boundary terms.  Secondary: boundary edges, or
+// non-coplanar convex ones (Edge.cpp:42-88).  The per-camera primary list
+// (Edge.cpp:313-348) is selected on the device (kernels/update.hip).
 void select_sec_edges(const SceneHost& s, std::vector<EdgeDev>& out);
-// Primary: boundary edges or silhouettes seen from `eye` (Edge.cpp:313-348).
-void select_prim_edges(const SceneHost& s, const float eye[3], std::vector<EdgeDev>& out);
 // Distribution1D over edge lengths: len, normalized cdf, integral.
 float edge_distribution(const SceneHost& s, const std::vector<EdgeDev>& e, std::vector<float>& len,
                         std::vector<float>& cdf);

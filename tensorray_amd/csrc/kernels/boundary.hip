@@ -40,8 +40,10 @@ __device__ inline F3 cam_eye(const CameraDev& c) { return F3{c.eye[0], c.eye[1],
 // Edge sampling with the length distribution; returns the edge index and
 // the pdf per unit length.
 __device__ inline int32_t sample_edge(const EdgeSet& E, float u, float* pdf) {
-  int32_t i = cdf_search(E.cdf, E.n, u);
-  float pmf = E.len[i] / (E.integral * (float)E.n);
+  const int32_t n = E.dyn ? E.dyn->n : E.n;
+  const float integral = E.dyn ? E.dyn->integral : E.integral;
+  int32_t i = cdf_search(E.cdf, n, u);
+  float pmf = E.len[i] / (integral * (float)n);
   *pdf = pmf / E.len[i];
   return i;
 }
@@ -344,6 +346,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
   const F3 eye = cam_eye(C);
   const uint64_t stride = (uint64_t)L.grid * kBlock;
+  // no silhouette or boundary edge seen from this camera: no samples
+  if ((E.dyn ? E.dyn->n : E.n) <= 0) return;
   for (uint64_t i = first + (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < end; i += stride) {
     bool keep = false;
     Hit h;

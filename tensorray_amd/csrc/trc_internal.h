@@ -41,6 +41,32 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+namespace trc {
+// Device-side scene update (kernels/update.hip).
+// Edges per block of the order-preserving edge selection.
+constexpr int kEdgeTile = 1024;
+// Scalars the update writes on the device; the host reads them back once.
+struct UpdateScalars {
+  EdgeCount sec;          // secondary edges: count and length integral
+  float light_integral;   // Distribution1D integral of the emitter table
+  float pad;
+  double sah;             // SAH cost of the refit tree (rebuild decision)
+};
+// Device arrays of the update, owned by the scene.
+struct SceneUpdateDev {
+  const EdgeDev* all_edges = nullptr;  // every mesh edge (global ids), edge order
+  uint32_t n_edges = 0;
+  const uint32_t* vs_off = nullptr;    // smooth faces per vertex (CSR, triangle order) or null
+  const uint32_t* vs_list = nullptr;
+  const uint32_t* refit_nodes = nullptr;  // BVH nodes by level, deepest first
+  std::vector<uint32_t> level_off;        // host copy of the level offsets
+  uint32_t n_nodes = 0;
+  float* nbox = nullptr;                  // [nodes][6] unpadded node boxes
+  uint32_t* scan = nullptr;               // [edge_scan_words(n_edges)] block counts / offsets
+  UpdateScalars* scal = nullptr;
+};
+}  // namespace trc
+
 struct trc_scene {
   trc_ctx* ctx = nullptr;
   trc::SceneHost host;
@@ -50,14 +76,20 @@ struct trc_scene {
   trc::SceneDev dev{};
   trc::TexDesc* dev_texs = nullptr;
   int64_t device_bytes = 0;
-  // primary-edge list of the last camera (host staging kept alive until the
-  // next call, device copy in prim_buf)
-  std::vector<trc::EdgeDev> prim_edges;
-  std::vector<float> prim_len, prim_cdf;
-  DevBuf prim_buf;
-  // recorded after the last upload from that staging: the next call waits
-  // for it before overwriting the staging (instead of a stream sync per call)
-  hipEvent_t prim_done = nullptr;
+  // device-side update of the derived data and the BVH (trc_scene_set_vertices)
+  trc::SceneUpdateDev upd;
+  trc::UpdateScalars* upd_host = nullptr;  // pinned readback of upd.scal
+  double sah_built = 0.0;                  // SAH cost of the tree when it was last built
+  int32_t refits = 0, rebuilds = 0;
+  // the host copies of positions, derived data and BVH boxes lag behind the
+  // device after a device update (sync_host brings them up to date)
+  bool host_stale = false;
+  // primary-edge list of the last camera, selected on the device (capacity:
+  // every edge)
+  trc::EdgeDev* prim_e = nullptr;
+  float* prim_len = nullptr;
+  float* prim_cdf = nullptr;
+  trc::EdgeCount* prim_cnt = nullptr;
   // guiding of the direct boundary term: per-sample values, leaf weights,
   // leaf CDF and the weight integral
   DevBuf guide_buf;
@@ -206,6 +238,16 @@ hipError_t launch_film_accum(const float* paths, int ns, int wh, float* film, bo
 // device (hipGetSymbolAddress).
 size_t noalloc_scratch_bytes();
 hipError_t noalloc_scratch_address(void** out);
+// ---- device-side scene update (kernels/update.hip) ----
+// Words of the block-count scratch of an edge selection over n_edges edges.
+uint32_t edge_scan_words(uint32_t n_edges);
+// Recomputes face normals and areas, vertex normals, the emitter table, the
+// secondary-edge list, the triangle records and the BVH boxes from S.pos.
+hipError_t launch_scene_update(const SceneDev& S, const SceneUpdateDev& U, hipStream_t st);
+// Primary edges seen from `eye` (boundary or silhouette) into out_* (capacity
+// U.n_edges), their count and length integral into *count.
+hipError_t launch_prim_edges(const SceneDev& S, const SceneUpdateDev& U, const float eye[3], EdgeDev* out_e,
+                             float* out_len, float* out_cdf, EdgeCount* count, hipStream_t st);
 // Guide of the direct boundary term: one leaf per secondary edge i over
 // x in [sec.cdf[i-1], sec.cdf[i]], y, z in [0, 1].
 struct GuideDev {

@@ -13,12 +13,15 @@ examples use Adam (Optimizer.h:66-123).  Here every step
   reference's edge integrators in its examples), all accumulating into one
   [V][3] gradient,
 * takes one device Adam step on the vertex positions (trc_adam_step), and
-* refits the BVH to the moved vertices (trc_scene_set_vertices with refit:
-  same topology, node bounds recomputed) instead of rebuilding it.
+* updates the scene on the device (trc_scene_set_vertices with refit: face
+  and vertex normals, emitter table, secondary edges and the BVH's node
+  bounds recomputed on the GPU, same topology); the library rebuilds the
+  tree once its SAH cost has grown past 1.25x its cost at the last build.
 
 Each phase is timed (HIP events on the stream for device work, the host
-clock for the refit, which runs on the host), and the vertex error against
-the target mesh is reported.
+clock around the scene update, which ends with a synchronisation), the
+refit and rebuild counts are reported, and so is the vertex error against
+the target mesh.
 """
 import argparse
 import dataclasses
@@ -201,9 +204,11 @@ class InverseRendering:
         self.ds.set_vertices(self.x.data_ptr(), self.V, refit=True, stream=st)
         torch.cuda.synchronize()
         acc["refit"] = (time.perf_counter() - r0) * 1e3
+        info = self.ds.info()
         acc["total"] = (time.perf_counter() - t0) * 1e3
         self.steps_done = t
         rec = dict(step=t, loss=loss.item(), vertex_error=self.vertex_error(), radial_error=self.radial_error(),
+                   bvh_sah=round(info.bvh_sah_cost, 4), bvh_rebuilds=info.bvh_rebuilds,
                    ms={k: round(v, 3) for k, v in acc.items()})
         self.log.append(rec)
         return rec
