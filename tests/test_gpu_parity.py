@@ -106,11 +106,14 @@ def test_forward_bit_exact(gpu_ctx, oracle_mod, case):
     ds.close()
 
 
-def _grad_close(g, r, rel=1e-3):
+def _grad_close(g, r, rel=1e-3, floor=1e-5):
+    """Per entry: |g - r| <= rel |r| + floor max|r|.  The floor covers the
+    reassociation of the float atomics that accumulate each entry on the GPU
+    (the oracle sums in double); returns (ok, worst error / bound)."""
     scale = np.abs(r).max()
     err = np.abs(g - r)
-    ok = (err <= rel * np.abs(r)) | (err <= rel * scale)
-    return ok.all(), err.max() / scale
+    bound = rel * np.abs(r) + floor * scale
+    return bool((err <= bound).all()), float((err / np.maximum(bound, 1e-300)).max())
 
 
 @pytest.mark.parametrize("case", ["config1", "metric_small", "textured", "conductor", "smooth", "deepest"])
