@@ -221,7 +221,8 @@ def served_sweep(ctx, a, loads=(0.25, 0.5, 0.75, 1.0, 1.25)):
              "s20k": scenes.sphere_scene(subdiv=5, amplitude=0.05, noise_seed=1),
              "s200k": scenes.sphere_scene(freq=100, amplitude=0.05, noise_seed=3),
              "s1m": scenes.sphere_scene(freq=224, amplitude=0.05, noise_seed=4)}
-    srvs = {"fifo": serving.Server(ctx, budget_bytes=64 << 30, batch_paths=16 << 20, chunk_paths=16 << 20),
+    srvs = {"fifo": serving.Server(ctx, budget_bytes=64 << 30, batch_paths=16 << 20, chunk_paths=16 << 20,
+                                   small_paths=4 << 20),
             "srpt": serving.Server(ctx, budget_bytes=64 << 30, batch_paths=8 << 20, chunk_paths=8 << 20,
                                    order="srpt")}
     srv = srvs["fifo"]
@@ -239,8 +240,10 @@ def served_sweep(ctx, a, loads=(0.25, 0.5, 0.75, 1.0, 1.25)):
     for sv in srvs.values():
         sv.run(reqs(0, 7)[:4], realtime=False)
     out = dict(scenes={k: v.num_triangles() for k, v in specs.items()}, scene_build_s=build_s,
-               policies={"served_fifo": "oldest first, 16M paths per step", "served_srpt":
-                         "shortest remaining work first, 8M paths per step"})
+               policies={"served_fifo": "oldest first; a request of more than 4M paths runs alone and whole, "
+                                        "smaller ones are batched (up to 16M paths) with the small ones behind "
+                                        "them", "served_srpt": "shortest remaining work first, 8M-path chunks "
+                                                               "and steps"})
     modes = ("plain", "served_fifo", "served_srpt")
 
     def run_mode(mode, rs, realtime):

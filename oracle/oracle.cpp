@@ -1226,8 +1226,9 @@ static std::vector<EdgeRec> prim_edges(const Scene& s, const Cam& c) {
 }
 
 // Distribution1D over edge lengths (Distribution.cpp:45-58, Distribution.cu:37-48)
-// Blocked summation order of the guide's reductions over leaves (the order
-// the device uses; the reference leaves it to its reduction library): chunks
+// and the guide's reductions over leaves use a blocked summation order (the
+// order the device uses; the reference leaves it to its scan and reduction
+// library, CUB): chunks
 // of kSumChunk consecutive values are summed sequentially, then the chunk
 // sums sequentially; an inclusive prefix is the sequential prefix of the
 // preceding chunk sums plus the sequential prefix within the chunk.  Up to
@@ -2063,7 +2064,7 @@ void oracle_boundary_adj_gs(void* sp, const float* cam, int w, int h, uint32_t s
   else if (kind >= 2) E = sec_edges(s);
   if (kind != 1 && E.empty()) return;
   std::unique_ptr<Dist1> D;
-  if (kind != 1) D.reset(new Dist1(edge_lengths(E)));
+  if (kind != 1) D.reset(new Dist1(edge_lengths(E), Blocked{}));
   std::unique_ptr<Guide> G;
   if (kind == 2 && g_direct) G = build_guide(s, E, *D, g_seed, g_spp, g_eps, threads);
   const uint64_t N = (uint64_t)spp * w * h;
@@ -2102,7 +2103,7 @@ void oracle_guide(void* sp, uint32_t seed, int g_spp, float g_eps, float* x0, fl
   std::vector<EdgeRec> E = sec_edges(s);
   *n_out = (int)E.size();
   if (E.empty()) return;
-  Dist1 D(edge_lengths(E));
+  Dist1 D(edge_lengths(E), Blocked{});
   std::unique_ptr<Guide> G = build_guide(s, E, D, seed, g_spp, g_eps, 8);
   for (size_t i = 0; i < E.size(); ++i) {
     if (x0) x0[i] = G->x0[i];

@@ -268,6 +268,7 @@ void upload_scene(trc_scene* s, hipStream_t st) {
   U.n_nodes = (uint32_t)h.nodes.size();
   U.nbox = (float*)alloc_dev(s, 6 * sizeof(float) * (size_t)U.n_nodes);
   U.scan = (uint32_t*)alloc_dev(s, 4 * (size_t)edge_scan_words((uint32_t)E));
+  U.part = (float*)alloc_dev(s, 4 * std::max<size_t>(1, (E + kCdfChunk - 1) / kCdfChunk));
   {
     std::vector<UpdateScalars> sc(1);
     sc[0].sec = EdgeCount{(int32_t)h.sec_edges.size(), h.sec_integral};
@@ -815,12 +816,14 @@ static uint64_t default_tile(const trc_ctx* ctx, uint64_t total) {
 // times the resident lanes) fill the GPU on their own: they run through the
 // single-request kernels over their path range, one scene at a time in the
 // caches, and only the rest is tiled.  (Serving at saturation: 667 -> 702
-// Mpaths/s.)  A batch of one scene -- the views of a multi-view plan -- is
-// tiled whole, which keeps the tail of each view hidden behind the others.
+// Mpaths/s.)  So does a batch of a single large job, which then launches
+// exactly what the plain path launches.  A batch of several jobs of one
+// scene -- the views of a multi-view plan -- is tiled whole, which keeps
+// the tail of each view hidden behind the others.
 constexpr uint64_t kSoloPaths = 4u << 20;
 
 static uint64_t mark_solo(std::vector<JobDev>& jd) {
-  bool mixed = false;
+  bool mixed = jd.size() == 1;
   for (const JobDev& d : jd) mixed = mixed || d.S.nodes != jd[0].S.nodes;
   uint64_t tiled = 0;
   for (JobDev& d : jd) {

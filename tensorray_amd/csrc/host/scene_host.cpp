@@ -236,11 +236,19 @@ float edge_distribution(const SceneHost& s, const std::vector<EdgeDev>& e, std::
   cdf.resize(n);
   if (n == 0) return 0.0f;
   for (size_t i = 0; i < n; ++i) len[i] = length(vget(s.pos, e[i].v1) - vget(s.pos, e[i].v0));
+  // inclusive prefix in the blocked order: sequential within chunks of
+  // kCdfChunk edges, chunk sums sequential, chunk offset + local prefix
   const float inv = 1.0f / (float)n;
-  float run = 0.0f;
-  for (size_t i = 0; i < n; ++i) {
-    run += len[i];
-    cdf[i] = run * inv;
+  const size_t nc = (n + kCdfChunk - 1) / kCdfChunk;
+  float off = 0.0f;
+  for (size_t c = 0; c < nc; ++c) {
+    float local = 0.0f;
+    const size_t end = std::min(n, (c + 1) * (size_t)kCdfChunk);
+    for (size_t i = c * kCdfChunk; i < end; ++i) {
+      local = local + len[i];
+      cdf[i] = (off + local) * inv;
+    }
+    off = off + local;
   }
   float integral = cdf[n - 1];
   if (integral > 0.0f)

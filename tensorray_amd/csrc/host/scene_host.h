@@ -108,7 +108,10 @@ CameraDev make_camera(const float pos[3], const float target[3], const float up[
 // non-coplanar convex ones (Edge.cpp:42-88).  The per-camera primary list
 // (Edge.cpp:313-348) is selected on the device (kernels/update.hip).
 void select_sec_edges(const SceneHost& s, std::vector<EdgeDev>& out);
-// Distribution1D over edge lengths: len, normalized cdf, integral.
+// Distribution1D over edge lengths: len, normalized cdf, integral.  The
+// prefix sum runs in a blocked order (sequential within chunks of kCdfChunk
+// edges, then over the chunk sums), which the device reproduces in parallel.
+constexpr int kCdfChunk = 1024;
 float edge_distribution(const SceneHost& s, const std::vector<EdgeDev>& e, std::vector<float>& len,
                         std::vector<float>& cdf);
 

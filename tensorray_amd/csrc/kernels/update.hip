@@ -16,14 +16,15 @@
 //   k_edge_count / k_block_scan / k_edge_scatter
 //                     order-preserving selection of edges (secondary edges:
 //                     Edge.cpp:42-88; primary edges for an eye: Edge.cpp:313-348)
-//   k_edge_len / k_edge_cdf / k_edge_cdf_norm
+//   k_edge_len / k_cdf_local / k_cdf_offsets / k_cdf_final / k_edge_cdf_norm
 //                     their length distribution (Edge.cpp:90-106, 350-367)
 //   k_tri_records / k_refit_level / k_sah
 //                     BVH refit level by level, deepest first, and the tree's
 //                     SAH cost (the reference refits its GAS, Scene.cpp:368-469)
 //
-// The prefix sums over floats that define a CDF run on one lane in index
-// order: they are part of what the results must reproduce bit for bit.
+// The prefix sums over floats that define a CDF follow the host code's
+// order (sequential for the emitter table, blocked for the edge lists): they
+// are part of what the results must reproduce bit for bit.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -233,21 +234,73 @@ __global__ void k_edge_len(const EdgeDev* e, const float* pos, float* len, uint3
   len[i] = length(pos3(pos, e[i].v1) - pos3(pos, e[i].v0));
 }
 
-// cdf[i] = (len[0] + ... + len[i]) / n summed in index order; integral = cdf[n-1].
-__global__ void k_edge_cdf(const float* len, float* cdf, EdgeCount* count) {
+// Inclusive prefix of the lengths in the blocked order of the host code:
+// k_cdf_local sums each chunk of kCdfChunk lengths sequentially (one lane
+// per chunk) into cdf and the chunk sum into part; k_cdf_offsets turns the
+// chunk sums into sequential offsets; k_cdf_final adds the offset and
+// scales by 1/n; k_cdf_integral reads the integral (cdf[n-1]).
+__global__ void k_cdf_local(const float* __restrict__ len, float* __restrict__ cdf, float* __restrict__ part,
+                            uint32_t cap, const EdgeCount* __restrict__ count) {
+  const uint32_t c = blockIdx.x * kUpdBlock + threadIdx.x;
+  const uint32_t n = (uint32_t)max(count->n, 0);
+  const uint32_t a = c * (uint32_t)kCdfChunk;
+  if (c * (uint64_t)kCdfChunk >= cap || a >= n) return;
+  const uint32_t b = min(n, a + (uint32_t)kCdfChunk);
+  float local = 0.0f;
+  uint32_t i = a;
+  for (; i + 8 <= b; i += 8) {  // loads ahead of the dependent additions
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = len[i + k];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      local = local + v[k];
+      cdf[i + k] = local;
+    }
+  }
+  for (; i < b; ++i) {
+    local = local + len[i];
+    cdf[i] = local;
+  }
+  part[c] = local;
+}
+
+// One lane: the chunk sums (n / kCdfChunk of them) become exclusive offsets.
+__global__ void k_cdf_offsets(float* __restrict__ part, const EdgeCount* __restrict__ count) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const uint32_t n = (uint32_t)max(count->n, 0);
+  const uint32_t nc = (n + kCdfChunk - 1) / kCdfChunk;
+  float off = 0.0f;
+  uint32_t c = 0;
+  for (; c + 16 <= nc; c += 16) {
+    float v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = part[c + k];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      part[c + k] = off;
+      off = off + v[k];
+    }
+  }
+  for (; c < nc; ++c) {
+    const float acc = part[c];
+    part[c] = off;
+    off = off + acc;
+  }
+}
+
+__global__ void k_cdf_final(float* cdf, const float* part, uint32_t cap, const EdgeCount* count) {
+  const uint32_t i = blockIdx.x * kUpdBlock + threadIdx.x;
+  const int32_t n = count->n;
+  if (i >= cap || (int32_t)i >= n) return;
+  const float inv = 1.0f / (float)n;
+  cdf[i] = (part[i / kCdfChunk] + cdf[i]) * inv;
+}
+
+__global__ void k_cdf_integral(const float* cdf, EdgeCount* count) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   const int32_t n = count->n;
-  if (n <= 0) {
-    count->integral = 0.0f;
-    return;
-  }
-  const float inv = 1.0f / (float)n;
-  float run = 0.0f;
-  for (int32_t i = 0; i < n; ++i) {
-    run += len[i];
-    cdf[i] = run * inv;
-  }
-  count->integral = cdf[n - 1];
+  count->integral = n > 0 ? cdf[n - 1] : 0.0f;
 }
 
 __global__ void k_edge_cdf_norm(float* cdf, uint32_t cap, const EdgeCount* count) {
@@ -354,8 +407,9 @@ __global__ void k_zero_sah(UpdateScalars* out) {
 inline unsigned blocks_for(uint64_t n) { return (unsigned)((n + kUpdBlock - 1) / kUpdBlock); }
 
 template <class Pred>
-hipError_t select_edges(const EdgeDev* all, uint32_t n_all, Pred pred, uint32_t* scratch, EdgeDev* out_e,
-                        float* out_len, float* out_cdf, EdgeCount* count, const float* pos, hipStream_t st) {
+hipError_t select_edges(const EdgeDev* all, uint32_t n_all, Pred pred, uint32_t* scratch, float* part,
+                        EdgeDev* out_e, float* out_len, float* out_cdf, EdgeCount* count, const float* pos,
+                        hipStream_t st) {
   const uint32_t nb = n_all == 0 ? 1 : (n_all + kEdgeTile - 1) / kEdgeTile;
   hipLaunchKernelGGL(k_edge_count<Pred>, dim3(nb), dim3(kUpdBlock), 0, st, all, n_all, pred, scratch);
   hipLaunchKernelGGL(k_block_scan, dim3(1), dim3(1024), 0, st, scratch, nb, count);
@@ -364,7 +418,13 @@ hipError_t select_edges(const EdgeDev* all, uint32_t n_all, Pred pred, uint32_t*
   const uint32_t cap = n_all > 0 ? n_all : 1;
   hipLaunchKernelGGL(k_edge_len, dim3(blocks_for(cap)), dim3(kUpdBlock), 0, st, out_e, pos, out_len, cap,
                      (const EdgeCount*)count);
-  hipLaunchKernelGGL(k_edge_cdf, dim3(1), dim3(1), 0, st, (const float*)out_len, out_cdf, count);
+  const uint32_t nchunks = (cap + kCdfChunk - 1) / kCdfChunk;
+  hipLaunchKernelGGL(k_cdf_local, dim3(blocks_for(nchunks)), dim3(kUpdBlock), 0, st, (const float*)out_len, out_cdf,
+                     part, cap, (const EdgeCount*)count);
+  hipLaunchKernelGGL(k_cdf_offsets, dim3(1), dim3(1), 0, st, part, (const EdgeCount*)count);
+  hipLaunchKernelGGL(k_cdf_final, dim3(blocks_for(cap)), dim3(kUpdBlock), 0, st, out_cdf, (const float*)part, cap,
+                     (const EdgeCount*)count);
+  hipLaunchKernelGGL(k_cdf_integral, dim3(1), dim3(1), 0, st, (const float*)out_cdf, count);
   hipLaunchKernelGGL(k_edge_cdf_norm, dim3(blocks_for(cap)), dim3(kUpdBlock), 0, st, out_cdf, cap,
                      (const EdgeCount*)count);
   return hipGetLastError();
@@ -388,7 +448,7 @@ hipError_t launch_scene_update(const SceneDev& S, const SceneUpdateDev& U, hipSt
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (U.n_edges > 0) {
-    e = select_edges(U.all_edges, U.n_edges, SecPred{pos, shade}, U.scan, const_cast<EdgeDev*>(S.sec.e),
+    e = select_edges(U.all_edges, U.n_edges, SecPred{pos, shade}, U.scan, U.part, const_cast<EdgeDev*>(S.sec.e),
                      const_cast<float*>(S.sec.len), const_cast<float*>(S.sec.cdf), &U.scal->sec, pos, st);
     if (e != hipSuccess) return e;
   }
@@ -409,8 +469,8 @@ hipError_t launch_scene_update(const SceneDev& S, const SceneUpdateDev& U, hipSt
 
 hipError_t launch_prim_edges(const SceneDev& S, const SceneUpdateDev& U, const float eye[3], EdgeDev* out_e,
                              float* out_len, float* out_cdf, EdgeCount* count, hipStream_t st) {
-  return select_edges(U.all_edges, U.n_edges, PrimPred{S.pos, S.shade, F3{eye[0], eye[1], eye[2]}}, U.scan, out_e,
-                      out_len, out_cdf, count, S.pos, st);
+  return select_edges(U.all_edges, U.n_edges, PrimPred{S.pos, S.shade, F3{eye[0], eye[1], eye[2]}}, U.scan, U.part,
+                      out_e, out_len, out_cdf, count, S.pos, st);
 }
 
 }  // namespace trc

@@ -63,6 +63,7 @@ struct SceneUpdateDev {
   uint32_t n_nodes = 0;
   float* nbox = nullptr;                  // [nodes][6] unpadded node boxes
   uint32_t* scan = nullptr;               // [edge_scan_words(n_edges)] block counts / offsets
+  float* part = nullptr;                  // [ceil(n_edges / kCdfChunk)] CDF chunk sums / offsets
   UpdateScalars* scal = nullptr;
 };
 }  // namespace trc

@@ -13,7 +13,9 @@ kernel launches (trc_render_batch):
   `batch_paths` paths as chunks (path ranges) of the admitted requests'
   current phase -- forward, or adjoint once the forward is complete -- and
   launches one forward batch and one adjoint batch; requests are taken
-  oldest first (order="fifo") or
+  oldest first (order="fifo"; a request of more than `small_paths` paths
+  at the head of the queue then runs alone and whole, small ones are
+  batched with the small ones queued behind them) or
   least remaining work first (order="srpt": shortest remaining processing
   time, which lets small requests pass large ones; with `aging` > 0 the
   remaining work is discounted by the time waited, which bounds how long a
@@ -128,7 +130,7 @@ class Server:
     """Continuous-batching server over resident scenes (one GPU, one stream)."""
 
     def __init__(self, ctx, budget_bytes=64 << 30, batch_paths=16 << 20, chunk_paths=16 << 20, max_inflight=2,
-                 order="fifo", aging=0.0):
+                 order="fifo", aging=0.0, small_paths=4 << 20):
         import torch
         self.torch = torch
         self.ctx = ctx
@@ -155,6 +157,16 @@ class Server:
         # `aging` paths per second it has waited since arrival, so that
         # large requests are not starved (0: pure shortest-remaining-first)
         self.aging = float(aging)
+        # small_paths > 0: a step never mixes sizes.  A request of more than
+        # small_paths paths at the head of the queue runs alone, whole (one
+        # forward and one adjoint launch, as the plain path would run it);
+        # a small one at the head is batched with the small requests queued
+        # right behind it, up to batch_paths, stopping at the first large
+        # one.  No request waits for a later one, and small requests share
+        # launches instead of each paying a launch sequence that cannot fill
+        # the GPU.  FIFO only (srpt preempts large requests by chunks); 0
+        # hands one path budget per step to requests of any size.
+        self.small_paths = int(small_paths) if order == "fifo" else 0
 
     def add_scene(self, name, spec):
         """Uploads a scene (BVH, triangles, textures) once; it stays resident."""
@@ -227,12 +239,18 @@ class Server:
         # this step can start its adjoint in the same step: the film and
         # dL/dI are computed on the stream between the two launches)
         fjobs, ajobs, fdone, adone, budget = [], [], [], [], self.batch_paths
-        for a in queue:
+        chunk = self.chunk_paths
+        head_large = self.small_paths > 0 and queue[0].req.paths > self.small_paths
+        if head_large:
+            budget = chunk = 1 << 62
+        for i, a in enumerate(queue):
             if budget <= 0:
+                break
+            if self.small_paths > 0 and i > 0 and (head_large or a.req.paths > self.small_paths):
                 break
             P = a.req.paths
             if a.fwd_next < P:
-                n = min(P - a.fwd_next, self.chunk_paths, budget)
+                n = min(P - a.fwd_next, chunk, budget)
                 fjobs.append(a.ds.job(a.cam, a.opt, a.fwd_next, n, paths=a.paths.data_ptr(),
                                       ckpt=a.ckpt.data_ptr()))
                 a.fwd_next += n
@@ -241,7 +259,7 @@ class Server:
                     continue
                 fdone.append(a)
             if a.adj_next < P and budget > 0:
-                n = min(P - a.adj_next, self.chunk_paths, budget)
+                n = min(P - a.adj_next, chunk, budget)
                 ajobs.append(a.ds.job(a.cam, a.opt, a.adj_next, n, ckpt=a.ckpt.data_ptr(), dldi=a.dldi.data_ptr(),
                                       dpos=a.req.dpos.data_ptr(), dmat=a.req.dmat.data_ptr(),
                                       vn_scratch=a.vn.data_ptr()))
