@@ -60,8 +60,9 @@ def parse():
     p.add_argument("--boundary-spp", type=int, default=16,
                    help="edge samples per pixel of each boundary class in the boundary section (0: skip it)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                   help="after the timed region, copy a MIB-sized device buffer (reference kernel of known "
-                        "traffic for scaling the FETCH_SIZE / WRITE_SIZE counters)")
+                   help="after the timed region, run an elementwise kernel over a MIB-sized device buffer "
+                        "(reads and writes MIB: the reference of known traffic for scaling the FETCH_SIZE / "
+                        "WRITE_SIZE counters)")
     return p.parse_args()
 
 
@@ -515,10 +516,13 @@ def main():
                     kernel=kname, kernel_ms=kms,
                     algorithmic_bytes_per_launch=kbytes, fwd_kernel_ms=fwd_avg, adj_kernel_ms=adj_avg, **st)
     if a.copy_calibration:
+        # a shader kernel of known traffic (reads MIB, writes MIB): torch's
+        # elementwise add; a plain copy_ may run on a DMA engine, which the
+        # cache counters do not see
         src = torch.ones(a.copy_calibration * (1 << 20) // 4, device="cuda")
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
-        dst.copy_(src)
+        torch.add(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
     def optional(section, fn):
