@@ -488,8 +488,7 @@ def main():
         kbytes = (bb["adj_ckpt"] if ck else bb["adj"]) * P
     else:
         # the forward kernel with (true) or without (false) checkpoint writes
-        merged = "true" if B >= 5 and P <= (1 << 25) else "false"  # merged-bounce selection of the launcher
-        kname, kms = f"k_render_paths<{'true' if ck else 'false'}, {merged}>", fwd_avg
+        kname, kms = f"k_render_paths<{'true' if ck else 'false'}>", fwd_avg
         kbytes = (bb["fwd_ckpt"] if ck else bb["fwd"]) * P
     achieved = kbytes / (kms * 1e-3)
     traffic = None

@@ -43,6 +43,7 @@ trc_status guarded(trc_ctx* ctx, F&& f) {
 }
 
 constexpr int kSpillDepth = 64;
+static_assert(kStackLds + kSpillDepth <= 255, "a query's stack depth is kept in 8 bits of TraceState");
 constexpr int kBlocksPerCU = 2 * kFwdWaves;  // 128-lane blocks: 2 waves each, 4 SIMDs
 
 // Address of the allocation-free forward's scratch on `device` (the current

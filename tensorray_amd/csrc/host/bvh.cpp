@@ -42,8 +42,8 @@ struct BinNode {
 };
 
 constexpr int kBins = 32;
-constexpr double kCostTraverse = 1.0;
-constexpr double kCostIntersect = 1.0;
+constexpr double kCostTraverse = kSahTraverse;
+constexpr double kCostIntersect = kSahIntersect;
 constexpr int kMaxLeaf = 4;
 
 struct Builder {

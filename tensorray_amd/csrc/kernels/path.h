@@ -264,14 +264,22 @@ __device__ F3 path_forward(const SceneDev& S, const CameraDev& C, const PathPara
 //
 // path_forward runs one path to completion per lane, so a wave waits for its
 // longest path and every traversal is entered by whichever lanes still live.
-// Here each loop iteration traces exactly one ray per lane -- the camera ray,
-// an NEE shadow ray or a BSDF ray, depending on the lane's stage -- then
-// shades it; a lane whose path ends takes the next path of its wave's chunk
-// (ballot + popcount hand-out, no global atomics).  Each path evaluates the
-// same expressions in the same order as path_forward<false>, so the
-// per-path radiance is identical.
+// Here each loop iteration advances one ray query per lane -- the camera ray,
+// an NEE shadow ray or a BSDF ray, depending on the lane's stage -- and
+// shades the queries that have finished; a lane whose path ends takes the
+// next path of its wave's chunk (ballot + popcount hand-out, no global
+// atomics).  A query that is still running when most of the wave's have
+// finished keeps its traversal state and continues in the next iteration.
+// Each path evaluates the same expressions in the same order as
+// path_forward<false>, so the per-path radiance is identical.
 // ---------------------------------------------------------------------------
 enum PathStage : uint32_t { kStageIdle = 0, kStageCamera = 1, kStageShadow = 2, kStageBsdf = 3 };
+
+// Fraction (of 64) of a wave's queries still unfinished at which the wave
+// stops traversing and shades the finished ones.  Forward on the metric
+// scene, depth 4, at 4 / 8 / 16 / 20 / 24: 4.77 / 4.69 / 4.60 / 4.63 /
+// 4.76 ms; running every query to its end (0) is 6 % slower than 16.
+constexpr int kTraceKeep64 = 16;
 
 struct PathLane {
   uint64_t path;
@@ -282,10 +290,11 @@ struct PathLane {
   float la, lb;          // light-sample uniforms of bounce b
   float pdfb;            // BSDF-sample pdf of bounce b
   F3 wib;                // BSDF-sampled direction of bounce b
-  // ray of this iteration
+  // ray of this iteration (from tmin = kShadowEpsilon to tmax; a shadow
+  // ray is an any-hit query) and its traversal state
   F3 o, d;
-  float tmin, tmax;
-  bool any;
+  float tmax;
+  TraceState ts;
 };
 
 // NEE geometry of bounce b at `its` (Integrator.cpp:196-213).
@@ -310,10 +319,7 @@ __device__ inline NeeGeo nee_geo(const SceneDev& S, const Its& its, float la, fl
   return g;
 }
 
-// Sets up the shadow ray of bounce L.b at vertex `its`.  kMerged: also draws
-// the bounce's BSDF direction now, so that both rays of the bounce are
-// traced in the lane's next iteration.
-template <bool kMerged>
+// Sets up the shadow ray of bounce L.b at vertex `its`.
 __device__ inline void lane_setup_nee(const SceneDev& S, const PathParams& P, const Its& its, PathLane& L) {
   Rnd4 rl = rng4(P.seed, kRngInterior, L.path, 1u + 2u * L.b);
   L.la = rl.a;
@@ -322,13 +328,8 @@ __device__ inline void lane_setup_nee(const SceneDev& S, const PathParams& P, co
   L.stage = kStageShadow;
   L.o = its.p;
   L.d = g.wi;
-  L.tmin = kShadowEpsilon;
   L.tmax = g.ray ? g.dist - kShadowEpsilon : -1.0f;
-  L.any = true;
-  if constexpr (kMerged) {
-    Rnd4 rb = rng4(P.seed, kRngInterior, L.path, 1u + 2u * L.b + 1u);
-    L.wib = bsdf_sample(S, its, -L.raydir, rb.a, rb.b, &L.pdfb);
-  }
+  trace_begin(L.ts, L.tmax);
 }
 
 // The path ranges a wave renders: tiles t, t + step, ... (below tile `tend`)
@@ -379,7 +380,7 @@ struct TileSupply {
 // The BSDF-sampled ray of bounce L.b from `its` came back with hit `h`:
 // MIS-weighted emission, throughput update, the next vertex's set-up, or the
 // end of the path (Integrator.cpp:232-293).
-template <bool kCkpt, bool kMerged, class Finish>
+template <bool kCkpt, class Finish>
 __device__ inline void bsdf_hit(const SceneDev& S, const PathParams& P, const CkptDev& ck, const Its& its, F3 wo,
                                 const Hit& h, PathLane& L, uint32_t& meta, Finish& finish) {
   const F3 zero{0.0f, 0.0f, 0.0f};
@@ -425,17 +426,13 @@ __device__ inline void bsdf_hit(const SceneDev& S, const PathParams& P, const Ck
     L.stage = kStageIdle;
     return;
   }
-  lane_setup_nee<kMerged>(S, P, its2, L);
+  lane_setup_nee(S, P, its2, L);
 }
 
 // The wave renders the paths of its tile supply.  A bounce is two ray
 // queries, the shadow ray of next-event estimation and the BSDF-sampled
-// ray.  By default each gets its own iteration of the state machine; with
-// kMerged a lane traces both in one iteration and shades the bounce in one
-// block (same random numbers and arithmetic, so the same paths bit for bit):
-// measured 2 % slower at depth 4 on the metric scene, where most paths end
-// after a bounce or two, and 3-10 % faster at depth 8.
-template <bool kCkpt, bool kMerged = false, class Finish>
+// ray, each traced in an iteration of its own.
+template <bool kCkpt, class Finish>
 __device__ void render_paths_regen(const SceneDev& S, const CameraDev& C, const PathParams& P, TraceCtx& tc,
                                    TileSupply& sup, const CkptDev& ck, Finish finish) {
   const F3 zero{0.0f, 0.0f, 0.0f};
@@ -445,6 +442,7 @@ __device__ void render_paths_regen(const SceneDev& S, const CameraDev& C, const 
   const int lane = (int)(threadIdx.x & 63u);
   PathLane L;
   L.stage = kStageIdle;
+  trace_none(L.ts);
   uint32_t meta = 0u;  // checkpoint meta word of the current path
   for (;;) {
     // ---- regenerate: idle lanes take the next paths of the wave's tiles ----
@@ -462,9 +460,8 @@ __device__ void render_paths_regen(const SceneDev& S, const CameraDev& C, const 
         L.stage = kStageCamera;
         L.o = eye;
         L.d = camera_dir(C, (float)px + rc.a, (float)py + rc.b);
-        L.tmin = kShadowEpsilon;
         L.tmax = 1e32f;
-        L.any = false;
+        trace_begin(L.ts, L.tmax);
       }
     }
     {
@@ -473,11 +470,12 @@ __device__ void render_paths_regen(const SceneDev& S, const CameraDev& C, const 
     }
     if (__ballot(L.stage != kStageIdle) == 0ull) break;
     // ---- one ray per lane ----
-    Hit h, h2;
-    if (L.stage != kStageIdle) h = trace_ray_rt(S, tc, L.o, L.d, L.tmin, L.tmax, L.any);
-    if constexpr (kMerged) {
-      if (L.stage == kStageShadow) h2 = trace_ray_rt(S, tc, L.o, L.wib, kShadowEpsilon, 1e32f, false);
-    }
+    // The wave leaves the traversal loop once at most kTraceKeep64 / 64 of
+    // its queries are unfinished; those lanes skip shading and resume their
+    // query in the next iteration, beside the new rays of the others.
+    trace_run(S, tc, L.o, L.d, kShadowEpsilon, L.tmax, L.stage == kStageShadow, L.ts, kTraceKeep64);
+    if (!trace_finished(L.ts)) continue;
+    const Hit h = trace_hit(L.ts);
     // ---- shade ----
     if (L.stage == kStageCamera) {
       if (h.tri < 0) {
@@ -518,7 +516,7 @@ __device__ void render_paths_regen(const SceneDev& S, const CameraDev& C, const 
         L.stage = kStageIdle;
         continue;
       }
-      lane_setup_nee<kMerged>(S, P, its, L);
+      lane_setup_nee(S, P, its, L);
     } else if (L.stage == kStageShadow) {
       Its its = post_intersect(S, L.tri, L.u, L.v);
       NeeGeo g = nee_geo(S, its, L.la, L.lb);
@@ -536,24 +534,17 @@ __device__ void render_paths_regen(const SceneDev& S, const CameraDev& C, const 
       val = val * mis;
       L.ret = zero;
       L.ret = L.ret + val;
-      if constexpr (!kMerged) {
-        Rnd4 rb = rng4(P.seed, kRngInterior, L.path, 1u + 2u * L.b + 1u);
-        L.wib = bsdf_sample(S, its, wo, rb.a, rb.b, &L.pdfb);
-        L.stage = kStageBsdf;
-        L.o = its.p;
-        L.d = L.wib;
-        L.tmin = kShadowEpsilon;
-        L.tmax = 1e32f;
-        L.any = false;
-        continue;
-      } else {
-        h = h2;
-        bsdf_hit<kCkpt, kMerged>(S, P, ck, its, wo, h, L, meta, finish);
-      }
+      Rnd4 rb = rng4(P.seed, kRngInterior, L.path, 1u + 2u * L.b + 1u);
+      L.wib = bsdf_sample(S, its, wo, rb.a, rb.b, &L.pdfb);
+      L.stage = kStageBsdf;
+      L.o = its.p;
+      L.d = L.wib;
+      L.tmax = 1e32f;
+      trace_begin(L.ts, L.tmax);
     } else if (L.stage == kStageBsdf) {
       Its its = post_intersect(S, L.tri, L.u, L.v);
       const F3 wo = -L.raydir;
-      bsdf_hit<kCkpt, kMerged>(S, P, ck, its, wo, h, L, meta, finish);
+      bsdf_hit<kCkpt>(S, P, ck, its, wo, h, L, meta, finish);
     }
   }
 }

@@ -46,7 +46,7 @@ __global__ void __launch_bounds__(kBlock) k_intersect(SceneDev S, LaunchCfg L, c
   }
 }
 
-template <bool kCkpt, bool kMerged>
+template <bool kCkpt>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kFwdWaves))) k_render_paths(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
                                                          uint64_t first, uint64_t count, uint32_t tile, float* paths,
                                                          uint64_t paths_first, CkptDev ck) {
@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kFw
   const uint64_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
   const uint64_t ntiles = (count + tile - 1) / tile;
   TileSupply sup = TileSupply::make(wave, ntiles, nwaves, 0, first, first + count, tile);
-  render_paths_regen<kCkpt, kMerged>(S, C, P, tc, sup, ck, [&](uint64_t p, F3 c) {
+  render_paths_regen<kCkpt>(S, C, P, tc, sup, ck, [&](uint64_t p, F3 c) {
     const uint64_t q = p - paths_first;
     paths[3 * q] = c.x;
     paths[3 * q + 1] = c.y;
@@ -513,21 +513,12 @@ static uint32_t forward_tile(uint64_t npaths, int grid) {
   return (uint32_t)(t < 64 ? 64 : (t > 4096 ? 4096 : t));
 }
 
-// Launches of paths with at least this many bounces, and at most
-// kMergedMaxPaths paths, trace the shadow ray and the BSDF ray of a bounce in
-// one loop iteration (kMerged): one traversal loop fewer per bounce, more
-// registers live across it.  Measured on gfx950 at 8 bounces: 3-10% faster
-// for 4-17M paths per launch, 3% slower for 67M (the 4,096-path tiles); 2%
-// slower at 4 bounces.  The results are the same either way.
-constexpr int kMergedMinBounces = 5;
-constexpr uint64_t kMergedMaxPaths = 1ull << 25;
-
-template <bool kCkpt, bool kMerged>
+template <bool kCkpt>
 static void launch_paths(const SceneDev& S, const CameraDev& C, const PathParams& P, const LaunchCfg& L,
                          uint64_t first, uint64_t count, float* paths, uint64_t paths_first, const CkptDev& ck,
                          hipStream_t st) {
-  const LaunchCfg Lk = fit_grid(k_render_paths<kCkpt, kMerged>, L);
-  hipLaunchKernelGGL((k_render_paths<kCkpt, kMerged>), dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, first,
+  const LaunchCfg Lk = fit_grid(k_render_paths<kCkpt>, L);
+  hipLaunchKernelGGL((k_render_paths<kCkpt>), dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, first,
                      count, forward_tile(count, Lk.grid), paths, paths_first, ck);
 }
 
@@ -541,14 +532,8 @@ hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t s
     count = np;
   }
   const CkptDev ck = ckpt ? ckpt_view(ckpt, np, max_bounces) : CkptDev{};
-  const bool merged = max_bounces >= kMergedMinBounces && count <= kMergedMaxPaths;
-  if (ckpt) {
-    if (merged) launch_paths<true, true>(S, C, P, L, first, count, paths, paths_first, ck, st);
-    else launch_paths<true, false>(S, C, P, L, first, count, paths, paths_first, ck, st);
-  } else {
-    if (merged) launch_paths<false, true>(S, C, P, L, first, count, paths, paths_first, ck, st);
-    else launch_paths<false, false>(S, C, P, L, first, count, paths, paths_first, ck, st);
-  }
+  if (ckpt) launch_paths<true>(S, C, P, L, first, count, paths, paths_first, ck, st);
+  else launch_paths<false>(S, C, P, L, first, count, paths, paths_first, ck, st);
   return hipGetLastError();
 }
 

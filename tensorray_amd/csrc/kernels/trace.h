@@ -115,6 +115,55 @@ __device__ inline uint32_t stack_pop(TraceCtx& c, int& sp) {
 
 __device__ inline F3 tri_vertex(const float* v) { return F3{v[0], v[1], v[2]}; }
 
+// State of one ray query between calls of trace_run.  The stack and the
+// postponed-leaf queue stay in LDS (and the spill slice) meanwhile, so a
+// wave can leave the traversal loop with some lanes unfinished, give its
+// finished lanes new rays, and continue all of them together.
+struct TraceState {
+  float best_t, u, v;  // closest accepted hit so far (t, barycentrics)
+  uint32_t best_id;    // its triangle, kNoHit if none
+  uint32_t node;       // next inner node to visit
+  uint32_t bits;       // stack depth (bits 0-7), queued leaves (8-11), inner_done (12), found (13)
+};
+constexpr uint32_t kNoHit = 0xFFFFFFFFu;
+constexpr uint32_t kTsInnerDone = 1u << 12;
+constexpr uint32_t kTsFound = 1u << 13;
+
+// A new query with far bound tmax.
+__device__ inline void trace_begin(TraceState& s, float tmax) {
+  s.best_t = tmax;
+  s.u = 0.0f;
+  s.v = 0.0f;
+  s.best_id = kNoHit;
+  s.node = 0u;
+  s.bits = 0u;
+}
+// No query (a lane without a ray): finished and missed.
+__device__ inline void trace_none(TraceState& s) {
+  s.best_id = kNoHit;
+  s.bits = kTsInnerDone;
+}
+// The query has finished: no inner node left and no leaf queued.
+__device__ inline bool trace_finished(const TraceState& s) {
+  return (s.bits & (kTsInnerDone | 0xF00u)) == kTsInnerDone;
+}
+// Hit record of a finished query (ptx.cu:92-113 semantics).
+__device__ inline Hit trace_hit(const TraceState& s) {
+  Hit h;
+  if (s.best_id != kNoHit) {
+    h.tri = (int32_t)s.best_id;
+    h.u = s.u;
+    h.v = s.v;
+    h.t = s.best_t;
+  } else {
+    h.tri = -1;
+    h.u = 0.0f;
+    h.v = 0.0f;
+    h.t = -1.0f;
+  }
+  return h;
+}
+
 // Closest hit (any = false) or first found hit (any = true) in (tmin, tmax).
 // Ties in t resolve to the smallest triangle id, so the result does not
 // depend on traversal order.  A ray with tmax < tmin visits the root only
@@ -130,19 +179,25 @@ __device__ inline F3 tri_vertex(const float* v) { return F3{v[0], v[1], v[2]}; }
 // changes the result, since the closest hit does not depend on test order.
 // Lanes run this loop in step with ballots, so the triangle test executes
 // with most lanes active instead of once per child slot that any lane hit.
-__device__ inline Hit trace_ray_rt(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, float tmin, float tmax, bool any) {
-  ctx.n_queries++;
+//
+// Runs the wave's queries until all have finished or until at most
+// keep64/64 of the queries live on entry are still unfinished (keep64 = 0:
+// to the end).  The state of unfinished queries is in `st`; calling again
+// with the same ray continues them exactly where they stopped, so the
+// result is the same however the traversal is split.
+__device__ inline void trace_run(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, float tmin, float tmax, bool any,
+                                 TraceState& st, int keep64) {
   RayPre rp = ray_pre(o, d);
   const float ix = safe_inv(d.x), iy = safe_inv(d.y), iz = safe_inv(d.z);
-  Hit h;
-  h.tri = -1; h.u = 0.0f; h.v = 0.0f; h.t = -1.0f;
-  float best_t = tmax;
-  uint32_t best_id = 0xFFFFFFFFu;
-  int sp = 0;
-  uint32_t node = 0;
-  bool inner_done = false;  // no inner node left to visit
-  int nq = 0;               // queued leaves
-  bool found = false;       // any-hit: a hit was found
+  float best_t = st.best_t, hu = st.u, hv = st.v;
+  uint32_t best_id = st.best_id;
+  uint32_t node = st.node;
+  int sp = (int)(st.bits & 0xFFu);
+  int nq = (int)((st.bits >> 8) & 0xFu);
+  bool inner_done = (st.bits & kTsInnerDone) != 0u;
+  bool found = (st.bits & kTsFound) != 0u;
+  int n_keep = 0;
+  if (keep64 > 0) n_keep = (__popcll(__ballot(!inner_done || nq > 0)) * keep64) >> 6;
   const int qs = ctx.lds_stride;
   for (;;) {
     // ---- visit one inner node (lanes with room in their leaf queue) ----
@@ -207,6 +262,7 @@ __device__ inline Hit trace_ray_rt(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d,
     const bool live = !inner_done || pend;
     const uint64_t m_live = __ballot(live);
     if (m_live == 0ull) break;
+    if (__popcll(m_live) <= n_keep) break;
     const uint64_t m_pend = __ballot(pend);
     const uint64_t m_urgent = __ballot(pend && (inner_done || nq > kLeafQueue - 4));
     const bool flush = m_urgent != 0ull || 4 * __popcll(m_pend) >= 3 * __popcll(m_live);
@@ -233,26 +289,33 @@ __device__ inline Hit trace_ray_rt(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d,
         uint32_t id = __float_as_uint(a.w);
         if (!(tt > tmin && tt < tmax)) continue;
         if (any) {
-          h.tri = (int32_t)id; h.u = uu; h.v = vv; h.t = tt;
+          best_id = id; best_t = tt; hu = uu; hv = vv;
           found = true;
           break;
         }
         if (tt < best_t || (tt == best_t && id < best_id)) {
-          best_t = tt; best_id = id; h.u = uu; h.v = vv;
+          best_t = tt; best_id = id; hu = uu; hv = vv;
         }
       }
       nq = 0;
       if (found) inner_done = true;
     }
   }
-  if (found) return h;
-  if (best_id != 0xFFFFFFFFu) {
-    h.tri = (int32_t)best_id;
-    h.t = best_t;
-  } else {
-    h.u = 0.0f; h.v = 0.0f;
-  }
-  return h;
+  st.best_t = best_t;
+  st.u = hu;
+  st.v = hv;
+  st.best_id = best_id;
+  st.node = node;
+  st.bits = (uint32_t)sp | ((uint32_t)nq << 8) | (inner_done ? kTsInnerDone : 0u) | (found ? kTsFound : 0u);
+}
+
+// One query to the end.
+__device__ inline Hit trace_ray_rt(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, float tmin, float tmax, bool any) {
+  ctx.n_queries++;
+  TraceState st;
+  trace_begin(st, tmax);
+  trace_run(S, ctx, o, d, tmin, tmax, any, st, 0);
+  return trace_hit(st);
 }
 
 template <bool kAnyHit>

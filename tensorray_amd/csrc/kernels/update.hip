@@ -370,7 +370,8 @@ __global__ void k_refit_level(Bvh4Node* nodes, float* nbox, const TriRecord* tri
 
 // SAH cost of the 4-wide tree relative to the root box (the build's
 // diagnostic): sum over child slots of area(child) / area(root) times the
-// slot's triangle count (leaves) or 1 (inner nodes).  Order of the double
+// slot's triangle count x kSahIntersect (leaves) or kSahTraverse (inner
+// nodes).  Order of the double
 // additions is not fixed; the value only drives the rebuild decision.
 __global__ void k_sah(const Bvh4Node* nodes, uint32_t N, const float* nbox, UpdateScalars* out) {
   const uint32_t i = blockIdx.x * kUpdBlock + threadIdx.x;
@@ -388,7 +389,7 @@ __global__ void k_sah(const Bvh4Node* nodes, uint32_t N, const float* nbox, Upda
       const double dx = (double)n.hix[s] - n.lox[s], dy = (double)n.hiy[s] - n.loy[s],
                    dz = (double)n.hiz[s] - n.loz[s];
       const double a = 2.0 * (dx * dy + dy * dz + dz * dx) / ra;
-      cost += a * (child_is_leaf(c) ? (double)leaf_count(c) : 1.0);
+      cost += a * (child_is_leaf(c) ? (double)leaf_count(c) * kSahIntersect : kSahTraverse);
     }
   }
   part[threadIdx.x] = cost;
