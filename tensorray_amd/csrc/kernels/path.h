@@ -283,17 +283,19 @@ constexpr int kTraceKeep64 = 16;
 
 struct PathLane {
   uint64_t path;
-  uint32_t stage, b;
+  uint32_t sb;           // stage (bits 0-1) and bounce index b (bits 2-31), one register
+  __device__ uint32_t stage() const { return sb & 3u; }
+  __device__ uint32_t b() const { return sb >> 2; }
+  __device__ void set_stage(uint32_t s) { sb = (sb & ~3u) | s; }
   int32_t tri;           // current vertex x_b
   float u, v;
   F3 beta, raydir, contrib, ret;
   float la, lb;          // light-sample uniforms of bounce b
   float pdfb;            // BSDF-sample pdf of bounce b
-  F3 wib;                // BSDF-sampled direction of bounce b
-  // ray of this iteration (from tmin = kShadowEpsilon to tmax; a shadow
-  // ray is an any-hit query) and its traversal state
+  // ray of this iteration (from tmin = kShadowEpsilon; a shadow ray is an
+  // any-hit query) and its traversal state, which holds the far bound; in
+  // the BSDF stage d is the sampled direction of bounce b
   F3 o, d;
-  float tmax;
   TraceState ts;
 };
 
@@ -317,19 +319,6 @@ __device__ inline NeeGeo nee_geo(const SceneDev& S, const Its& its, float la, fl
   bool noleak = (g.dsh * dgeo) > 0.0f;
   g.ray = stable && noleak;
   return g;
-}
-
-// Sets up the shadow ray of bounce L.b at vertex `its`.
-__device__ inline void lane_setup_nee(const SceneDev& S, const PathParams& P, const Its& its, PathLane& L) {
-  Rnd4 rl = rng4(P.seed, kRngInterior, L.path, 1u + 2u * L.b);
-  L.la = rl.a;
-  L.lb = rl.b;
-  NeeGeo g = nee_geo(S, its, L.la, L.lb);
-  L.stage = kStageShadow;
-  L.o = its.p;
-  L.d = g.wi;
-  L.tmax = g.ray ? g.dist - kShadowEpsilon : -1.0f;
-  trace_begin(L.ts, L.tmax);
 }
 
 // The path ranges a wave renders: tiles t, t + step, ... (below tile `tend`)
@@ -377,20 +366,21 @@ struct TileSupply {
   }
 };
 
-// The BSDF-sampled ray of bounce L.b from `its` came back with hit `h`:
-// MIS-weighted emission, throughput update, the next vertex's set-up, or the
-// end of the path (Integrator.cpp:232-293).
+// The BSDF-sampled ray of bounce b from `its` came back with hit `h`:
+// MIS-weighted emission, throughput update and the move to the next vertex,
+// or the end of the path (Integrator.cpp:232-293).  Returns true, with
+// `its` replaced by the new vertex x_{b+1}, when the path continues.
 template <bool kCkpt, class Finish>
-__device__ inline void bsdf_hit(const SceneDev& S, const PathParams& P, const CkptDev& ck, const Its& its, F3 wo,
+__device__ inline bool bsdf_hit(const SceneDev& S, const PathParams& P, const CkptDev& ck, Its& its, F3 wo,
                                 const Hit& h, PathLane& L, uint32_t& meta, Finish& finish) {
   const F3 zero{0.0f, 0.0f, 0.0f};
-  bool ok = h.tri >= 0 && L.pdfb > 0.0f && bsdf_dir_valid(S, its, L.wib);
+  bool ok = h.tri >= 0 && L.pdfb > 0.0f && bsdf_dir_valid(S, its, L.d);
   if (!ok) {
     if (finite3(L.ret)) L.contrib = L.contrib + L.ret;
     if (kCkpt) ck.meta[L.path] = meta;
     finish(L.path, L.contrib);
-    L.stage = kStageIdle;
-    return;
+    L.set_stage(kStageIdle);
+    return false;
   }
   Its its2 = post_intersect(S, h.tri, h.u, h.v);
   F3 dd = its2.p - its.p;
@@ -415,18 +405,19 @@ __device__ inline void bsdf_hit(const SceneDev& S, const PathParams& P, const Ck
   L.tri = h.tri; L.u = h.u; L.v = h.v;
   L.beta = beta2;
   L.raydir = nd;
-  L.b += 1u;
+  L.sb += 4u;  // b + 1
   if (kCkpt) {
-    ckpt_put_vertex(ck, L.path, (int)L.b, h.tri, h.u, h.v);
+    ckpt_put_vertex(ck, L.path, (int)L.b(), h.tri, h.u, h.v);
     meta += 1u;
   }
-  if ((int)L.b >= P.max_bounces) {
+  if ((int)L.b() >= P.max_bounces) {
     if (kCkpt) ck.meta[L.path] = meta;
     finish(L.path, L.contrib);
-    L.stage = kStageIdle;
-    return;
+    L.set_stage(kStageIdle);
+    return false;
   }
-  lane_setup_nee(S, P, its2, L);
+  its = its2;
+  return true;
 }
 
 // The wave renders the paths of its tile supply.  A bounce is two ray
@@ -441,14 +432,14 @@ __device__ void render_paths_regen(const SceneDev& S, const CameraDev& C, const 
   const F3 cdir{C.dir[0], C.dir[1], C.dir[2]};
   const int lane = (int)(threadIdx.x & 63u);
   PathLane L;
-  L.stage = kStageIdle;
+  L.sb = kStageIdle;
   trace_none(L.ts);
   uint32_t meta = 0u;  // checkpoint meta word of the current path
   for (;;) {
     // ---- regenerate: idle lanes take the next paths of the wave's tiles ----
     if (sup.next >= sup.end) sup.more();
     const uint64_t next = sup.next, end = sup.end;
-    const bool idle = L.stage == kStageIdle;
+    const bool idle = L.stage() == kStageIdle;
     const uint64_t idle_mask = __ballot(idle);
     if (idle) {
       const uint64_t rank = (uint64_t)__popcll(idle_mask & ((1ull << lane) - 1ull));
@@ -457,38 +448,46 @@ __device__ void render_paths_regen(const SceneDev& S, const CameraDev& C, const 
         const uint32_t pix = (uint32_t)(L.path % WH);
         const uint32_t px = pix % (uint32_t)C.width, py = pix / (uint32_t)C.width;
         Rnd4 rc = rng4(P.seed, kRngInterior, L.path, 0u);
-        L.stage = kStageCamera;
+        L.sb = kStageCamera;  // b = 0
         L.o = eye;
         L.d = camera_dir(C, (float)px + rc.a, (float)py + rc.b);
-        L.tmax = 1e32f;
-        trace_begin(L.ts, L.tmax);
+        trace_begin(L.ts, 1e32f);
       }
     }
     {
       const uint64_t n_idle = (uint64_t)__popcll(idle_mask);
       sup.next = next + n_idle < end ? next + n_idle : end;
     }
-    if (__ballot(L.stage != kStageIdle) == 0ull) break;
+    if (__ballot(L.stage() != kStageIdle) == 0ull) break;
     // ---- one ray per lane ----
     // The wave leaves the traversal loop once at most kTraceKeep64 / 64 of
     // its queries are unfinished; those lanes skip shading and resume their
     // query in the next iteration, beside the new rays of the others.
-    trace_run(S, tc, L.o, L.d, kShadowEpsilon, L.tmax, L.stage == kStageShadow, L.ts, kTraceKeep64);
+    trace_run(S, tc, L.o, L.d, kShadowEpsilon, L.stage() == kStageShadow, L.ts, kTraceKeep64);
     if (!trace_finished(L.ts)) continue;
     const Hit h = trace_hit(L.ts);
     // ---- shade ----
-    if (L.stage == kStageCamera) {
-      if (h.tri < 0) {
-        if (kCkpt) ck.meta[L.path] = 0u;
-        finish(L.path, zero);
-        L.stage = kStageIdle;
-        continue;
-      }
+    // Every stage starts from one surface interaction -- the first vertex
+    // for a camera ray, the current vertex x_b for a shadow or BSDF ray --
+    // and ends with one Philox draw and one light-sample geometry: the next
+    // shadow ray's for camera and BSDF lanes, the traced one's for shadow
+    // lanes.  Those are evaluated once for the whole wave, whatever the
+    // lanes' stages, instead of once in each stage's masked branch.
+    const uint32_t stage = L.stage();
+    if (stage == kStageIdle) continue;  // no path left for this lane
+    const bool cam = stage == kStageCamera;
+    if (cam && h.tri < 0) {
+      if (kCkpt) ck.meta[L.path] = 0u;
+      finish(L.path, zero);
+      L.set_stage(kStageIdle);
+      continue;
+    }
+    Its its = post_intersect(S, cam ? h.tri : L.tri, cam ? h.u : L.u, cam ? h.v : L.v);
+    if (cam) {
       if (kCkpt) {
         ckpt_put_vertex(ck, L.path, 0, h.tri, h.u, h.v);
         meta = 1u;
       }
-      Its its = post_intersect(S, h.tri, h.u, h.v);
       float beta0;
       {
         F3 cd = its.p - eye;
@@ -509,20 +508,35 @@ __device__ void render_paths_regen(const SceneDev& S, const CameraDev& C, const 
         if (finite3(e0)) L.contrib = L.contrib + e0;
       }
       L.tri = h.tri; L.u = h.u; L.v = h.v;
-      L.b = 0u;
       if (P.max_bounces <= 0) {
         if (kCkpt) ck.meta[L.path] = meta;
         finish(L.path, L.contrib);
-        L.stage = kStageIdle;
+        L.set_stage(kStageIdle);
         continue;
       }
-      lane_setup_nee(S, P, its, L);
-    } else if (L.stage == kStageShadow) {
-      Its its = post_intersect(S, L.tri, L.u, L.v);
-      NeeGeo g = nee_geo(S, its, L.la, L.lb);
+    } else if (stage == kStageBsdf) {
+      if (!bsdf_hit<kCkpt>(S, P, ck, its, -L.raydir, h, L, meta, finish)) continue;
+    }
+    // Next-event estimation at `its` (Integrator.cpp:196-230): camera and
+    // BSDF lanes draw the light sample of bounce b and set up its shadow
+    // ray; shadow lanes evaluate the sample they traced and draw the
+    // bounce's BSDF direction.
+    const bool shadow = stage == kStageShadow;
+    const Rnd4 rn = rng4(P.seed, kRngInterior, L.path, 1u + 2u * L.b() + (shadow ? 1u : 0u));
+    if (!shadow) {
+      L.la = rn.a;
+      L.lb = rn.b;
+    }
+    const NeeGeo g = nee_geo(S, its, L.la, L.lb);
+    if (!shadow) {
+      L.set_stage(kStageShadow);
+      L.o = its.p;
+      L.d = g.wi;
+      trace_begin(L.ts, g.ray ? g.dist - kShadowEpsilon : -1.0f);
+    } else {
       const F3 wo = -L.raydir;
       bool visible = g.ray && h.tri < 0;
-      if (kCkpt && visible) meta |= 1u << (8u + L.b);
+      if (kCkpt && visible) meta |= 1u << (8u + L.b());
       float valid = visible ? 1.0f : 0.0f;
       float Gl = fabsf(dot(g.ls.n, -g.wi)) / g.d2;
       F3 f = bsdf_eval(S, its, wo, g.wi) * fabsf(g.dsh);
@@ -534,17 +548,10 @@ __device__ void render_paths_regen(const SceneDev& S, const CameraDev& C, const 
       val = val * mis;
       L.ret = zero;
       L.ret = L.ret + val;
-      Rnd4 rb = rng4(P.seed, kRngInterior, L.path, 1u + 2u * L.b + 1u);
-      L.wib = bsdf_sample(S, its, wo, rb.a, rb.b, &L.pdfb);
-      L.stage = kStageBsdf;
+      L.d = bsdf_sample(S, its, wo, rn.a, rn.b, &L.pdfb);
+      L.set_stage(kStageBsdf);
       L.o = its.p;
-      L.d = L.wib;
-      L.tmax = 1e32f;
-      trace_begin(L.ts, L.tmax);
-    } else if (L.stage == kStageBsdf) {
-      Its its = post_intersect(S, L.tri, L.u, L.v);
-      const F3 wo = -L.raydir;
-      bsdf_hit<kCkpt>(S, P, ck, its, wo, h, L, meta, finish);
+      trace_begin(L.ts, 1e32f);
     }
   }
 }

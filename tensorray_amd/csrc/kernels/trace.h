@@ -164,7 +164,8 @@ __device__ inline Hit trace_hit(const TraceState& s) {
   return h;
 }
 
-// Closest hit (any = false) or first found hit (any = true) in (tmin, tmax).
+// Closest hit (any = false) or first found hit (any = true) in (tmin, tmax),
+// tmax being the far bound the query's state was begun with.
 // Ties in t resolve to the smallest triangle id, so the result does not
 // depend on traversal order.  A ray with tmax < tmin visits the root only
 // and misses (used for lanes without a query).
@@ -185,7 +186,7 @@ __device__ inline Hit trace_hit(const TraceState& s) {
 // to the end).  The state of unfinished queries is in `st`; calling again
 // with the same ray continues them exactly where they stopped, so the
 // result is the same however the traversal is split.
-__device__ inline void trace_run(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, float tmin, float tmax, bool any,
+__device__ inline void trace_run(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, float tmin, bool any,
                                  TraceState& st, int keep64) {
   RayPre rp = ray_pre(o, d);
   const float ix = safe_inv(d.x), iy = safe_inv(d.y), iz = safe_inv(d.z);
@@ -287,13 +288,16 @@ __device__ inline void trace_run(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, f
         float tt, uu, vv;
         if (!tri_intersect(rp, F3{a.x, a.y, a.z}, F3{b.x, b.y, b.z}, F3{e.x, e.y, e.z}, &tt, &uu, &vv)) continue;
         uint32_t id = __float_as_uint(a.w);
-        if (!(tt > tmin && tt < tmax)) continue;
+        // best_t is the far bound tmax until a hit is accepted, and below
+        // it afterwards, so these tests keep every hit inside (tmin, tmax)
+        if (!(tt > tmin)) continue;
         if (any) {
+          if (!(tt < best_t)) continue;
           best_id = id; best_t = tt; hu = uu; hv = vv;
           found = true;
           break;
         }
-        if (tt < best_t || (tt == best_t && id < best_id)) {
+        if (tt < best_t || (tt == best_t && best_id != kNoHit && id < best_id)) {
           best_t = tt; best_id = id; hu = uu; hv = vv;
         }
       }
@@ -314,7 +318,7 @@ __device__ inline Hit trace_ray_rt(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d,
   ctx.n_queries++;
   TraceState st;
   trace_begin(st, tmax);
-  trace_run(S, ctx, o, d, tmin, tmax, any, st, 0);
+  trace_run(S, ctx, o, d, tmin, any, st, 0);
   return trace_hit(st);
 }
 
