@@ -32,6 +32,8 @@ __device__ inline TraceCtx make_trace_ctx_b(uint32_t* lds, const LaunchCfg& L) {
   t.spill_stride = L.grid * kBlock;
   t.cap = L.cap;
   t.n_nodes = t.n_tris = t.n_queries = 0u;
+  t.slot_key = nullptr;
+  t.slot_uv = nullptr;
   return t;
 }
 

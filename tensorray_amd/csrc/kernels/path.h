@@ -463,7 +463,7 @@ __device__ void render_paths_regen(const SceneDev& S, const CameraDev& C, const 
     // The wave leaves the traversal loop once at most kTraceKeep64 / 64 of
     // its queries are unfinished; those lanes skip shading and resume their
     // query in the next iteration, beside the new rays of the others.
-    trace_run(S, tc, L.o, L.d, kShadowEpsilon, L.stage() == kStageShadow, L.ts, kTraceKeep64);
+    trace_run<true>(S, tc, L.o, L.d, kShadowEpsilon, L.stage() == kStageShadow, L.ts, kTraceKeep64);
     if (!trace_finished(L.ts)) continue;
     const Hit h = trace_hit(L.ts);
     // ---- shade ----

@@ -27,6 +27,18 @@ __device__ inline TraceCtx make_trace_ctx(uint32_t* lds, const LaunchCfg& L) {
   t.spill_stride = L.grid * kBlock;
   t.cap = L.cap;
   t.n_nodes = t.n_tris = t.n_queries = 0u;
+  t.slot_key = nullptr;
+  t.slot_uv = nullptr;
+  return t;
+}
+
+// The context of a kernel whose lds array also holds the wave-wide leaf
+// tests' slots: kTraceLdsWords + kTraceSlotWords words per lane.
+__device__ inline TraceCtx make_trace_ctx_wave(uint32_t* lds, const LaunchCfg& L) {
+  TraceCtx t = make_trace_ctx(lds, L);
+  unsigned long long* keys = reinterpret_cast<unsigned long long*>(lds + kTraceLdsWords * kBlock);
+  t.slot_key = keys + threadIdx.x;
+  t.slot_uv = reinterpret_cast<float2*>(keys + kBlock) + threadIdx.x;
   return t;
 }
 
@@ -50,8 +62,8 @@ template <bool kCkpt>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kFwdWaves))) k_render_paths(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
                                                          uint64_t first, uint64_t count, uint32_t tile, float* paths,
                                                          uint64_t paths_first, CkptDev ck) {
-  __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
-  TraceCtx tc = make_trace_ctx(lds_stack, L);
+  __shared__ __align__(8) uint32_t lds_stack[(kTraceLdsWords + kTraceSlotWords) * kBlock];
+  TraceCtx tc = make_trace_ctx_wave(lds_stack, L);
   // paths [first, first + count) in tiles of `tile` paths, dealt to the waves round-robin
   const uint64_t nwaves = (uint64_t)L.grid * (kBlock / 64);
   const uint64_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
@@ -329,8 +341,8 @@ __device__ inline int job_of_tile(const JobDev* jobs, int njobs, uint64_t t) {
 
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kFwdWaves))) k_render_batch_fwd(const JobDev* __restrict__ jobs, int njobs, uint64_t ntiles,
                                                              uint32_t tile, LaunchCfg L) {
-  __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
-  TraceCtx tc = make_trace_ctx(lds_stack, L);
+  __shared__ __align__(8) uint32_t lds_stack[(kTraceLdsWords + kTraceSlotWords) * kBlock];
+  TraceCtx tc = make_trace_ctx_wave(lds_stack, L);
   const uint64_t nwaves = (uint64_t)L.grid * (kBlock / 64);
   const uint64_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
   for (uint64_t t = wave; t < ntiles;) {

@@ -98,6 +98,10 @@ struct TraceCtx {
   uint32_t* spill;      // &spill[gtid]
   int spill_stride;     // total threads of the grid
   int cap;              // total stack capacity (lds_depth + spill depth)
+  // this lane's result slots in LDS for the wave-wide leaf tests of
+  // trace_run<true> (best hit key, its barycentrics); null where unused
+  unsigned long long* slot_key;
+  float2* slot_uv;
   // work counters (node visits, triangle tests, queries); read only by the
   // statistics kernel, otherwise dead code
   uint32_t n_nodes, n_tris, n_queries;
@@ -164,6 +168,113 @@ __device__ inline Hit trace_hit(const TraceState& s) {
   return h;
 }
 
+// Stalled lanes (queue full, or no inner node left with leaves queued), in
+// 64ths of the live lanes, at which trace_run<true> tests the queued leaves.
+constexpr int kWaveStall64 = 16;
+// Lanes with leaves queued, in 64ths of the live lanes, at which it does.
+constexpr int kWavePend64 = 48;
+
+// Leaf tests of the whole wave at once.  The queued leaves of all 64 lanes
+// form one list (lane by lane, in queue order), dealt round-robin over the
+// lanes: lane j tests items j, j + 64, ... against the ray of the item's
+// owner, read from the owner's registers, and offers its best hit to the
+// owner's slot in LDS with an atomic minimum on the key (t bits, triangle
+// id), which orders positive t first and ties by the smaller id, as the
+// lane-serial loop does.  The winner of each round also leaves its
+// barycentrics in the owner's slot.  So a flush takes ceil(total / 64)
+// rounds instead of the longest queue's length, and the hit of every query
+// is the one the lane-serial loop finds (closest hit) or a hit inside the
+// query's interval (any hit).  All 64 lanes of the wave must be active.
+__device__ inline void flush_wave(const SceneDev& S, TraceCtx& ctx, const RayPre& rp, float tmin, bool any,
+                                  int nq, float& best_t, float& hu, float& hv, uint32_t& best_id, bool& found) {
+  const int lane = (int)(threadIdx.x & 63u);
+  // exclusive prefix sum of the queue lengths over the wave (nq < 16: one
+  // ballot per bit)
+  const uint64_t below = (1ull << lane) - 1ull;
+  int pre = 0, total = 0;
+#pragma unroll
+  for (int bit = 0; bit < 4; ++bit) {
+    const uint64_t m = __ballot((nq >> bit) & 1);
+    pre += __popcll(m & below) << bit;
+    total += __popcll(m) << bit;
+  }
+  const int incl = pre + nq;
+  unsigned long long* keys = ctx.slot_key - lane;
+  float2* uvs = ctx.slot_uv - lane;
+  const unsigned long long key0 = ((unsigned long long)__float_as_uint(best_t) << 32) | best_id;
+  keys[lane] = key0;
+  // LDS accesses of the lanes of one wave execute in program order; the
+  // wavefront fences keep the compiler from moving the slot accesses across
+  // the rounds
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  const int kxyz = rp.kx | (rp.ky << 2) | (rp.kz << 4) | (any ? 64 : 0);
+  const uint32_t* leafq_w = ctx.leafq - lane;
+  const int qs = ctx.lds_stride;
+  for (int base = 0; base < total; base += 64) {
+    const int k = base + lane;
+    // owner: the number of lanes whose items all lie below k
+    int owner = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+      if (__shfl(incl, owner + step - 1) <= k) owner += step;
+    }
+    const int qi = k - __shfl(pre, owner);
+    RayPre r;
+    r.o = F3{__shfl(rp.o.x, owner), __shfl(rp.o.y, owner), __shfl(rp.o.z, owner)};
+    r.Sx = __shfl(rp.Sx, owner);
+    r.Sy = __shfl(rp.Sy, owner);
+    r.Sz = __shfl(rp.Sz, owner);
+    const int kk = __shfl(kxyz, owner);
+    r.kx = kk & 3;
+    r.ky = (kk >> 2) & 3;
+    r.kz = (kk >> 4) & 3;
+    const bool any_o = (kk & 64) != 0;
+    const float bt_o = __shfl(best_t, owner);
+    const uint32_t bid_o = (uint32_t)__shfl((int)best_id, owner);
+    unsigned long long bkey = ~0ull;
+    float bu = 0.0f, bv = 0.0f;
+    if (k < total) {
+      const uint32_t c = leafq_w[owner + qi * qs];
+      uint32_t cur = leaf_first(c);
+      for (uint32_t left = leaf_count(c); left > 0u; --left, ++cur) {
+        ctx.n_tris++;
+        const TriRecord& tr = S.tris[cur];
+        float4 a = *reinterpret_cast<const float4*>(tr.v0);
+        float4 b = *reinterpret_cast<const float4*>(tr.v1);
+        float4 e = *reinterpret_cast<const float4*>(tr.v2);
+        float tt, uu, vv;
+        if (!tri_intersect(r, F3{a.x, a.y, a.z}, F3{b.x, b.y, b.z}, F3{e.x, e.y, e.z}, &tt, &uu, &vv)) continue;
+        const uint32_t id = __float_as_uint(a.w);
+        // inside (tmin, tmax) and better than the owner's hit so far
+        if (!(tt > tmin)) continue;
+        const bool ok = any_o ? tt < bt_o : (tt < bt_o || (tt == bt_o && bid_o != kNoHit && id < bid_o));
+        if (!ok) continue;
+        const unsigned long long key = ((unsigned long long)__float_as_uint(tt) << 32) | id;
+        if (key < bkey) {
+          bkey = key;
+          bu = uu;
+          bv = vv;
+        }
+      }
+    }
+    if (bkey != ~0ull) {
+      atomicMin(&keys[owner], bkey);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if (keys[owner] == bkey) uvs[owner] = make_float2(bu, bv);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  const unsigned long long kf = keys[lane];
+  if (kf != key0) {
+    best_t = __uint_as_float((uint32_t)(kf >> 32));
+    best_id = (uint32_t)kf;
+    const float2 uv = uvs[lane];
+    hu = uv.x;
+    hv = uv.y;
+    if (any) found = true;
+  }
+}
+
 // Closest hit (any = false) or first found hit (any = true) in (tmin, tmax),
 // tmax being the far bound the query's state was begun with.
 // Ties in t resolve to the smallest triangle id, so the result does not
@@ -186,6 +297,11 @@ __device__ inline Hit trace_hit(const TraceState& s) {
 // to the end).  The state of unfinished queries is in `st`; calling again
 // with the same ray continues them exactly where they stopped, so the
 // result is the same however the traversal is split.
+//
+// kWave: the queued leaves are tested by the whole wave (flush_wave), which
+// needs all 64 lanes active and the LDS slots of the context; otherwise
+// each lane tests its own queue.
+template <bool kWave>
 __device__ inline void trace_run(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, float tmin, bool any,
                                  TraceState& st, int keep64) {
   RayPre rp = ray_pre(o, d);
@@ -266,8 +382,24 @@ __device__ inline void trace_run(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, f
     if (__popcll(m_live) <= n_keep) break;
     const uint64_t m_pend = __ballot(pend);
     const uint64_t m_urgent = __ballot(pend && (inner_done || nq > kLeafQueue - 4));
-    const bool flush = m_urgent != 0ull || 4 * __popcll(m_pend) >= 3 * __popcll(m_live);
-    if (flush && pend) {
+    bool flush;
+    if constexpr (kWave) {
+      // a wave-wide flush costs one round for up to 64 leaves: wait until
+      // kWaveStall64/64 of the live lanes are stalled on their queue,
+      // kWavePend64/64 have leaves queued, or no lane can visit a node
+      const uint64_t m_visit = __ballot(!inner_done && nq <= kLeafQueue - 4);
+      flush = m_visit == 0ull || __popcll(m_urgent) * 64 >= kWaveStall64 * __popcll(m_live) ||
+              __popcll(m_pend) * 64 >= kWavePend64 * __popcll(m_live);
+    } else {
+      flush = m_urgent != 0ull || 4 * __popcll(m_pend) >= 3 * __popcll(m_live);
+    }
+    if constexpr (kWave) {
+      if (flush) {
+        flush_wave(S, ctx, rp, tmin, any, nq, best_t, hu, hv, best_id, found);
+        nq = 0;
+        if (found) inner_done = true;
+      }
+    } else if (flush && pend) {
       int qi = 0;
       uint32_t cur = 0u, left = 0u;
       for (;;) {
@@ -318,7 +450,7 @@ __device__ inline Hit trace_ray_rt(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d,
   ctx.n_queries++;
   TraceState st;
   trace_begin(st, tmax);
-  trace_run(S, ctx, o, d, tmin, any, st, 0);
+  trace_run<false>(S, ctx, o, d, tmin, any, st, 0);
   return trace_hit(st);
 }
 

@@ -126,12 +126,16 @@ constexpr int kAdjWaves = 3;
 // natural 2 waves, 1.44 / 1.40 / 2.04 ms at 5.
 constexpr int kBndWaves = 5;
 constexpr int kStackLds = 16;
-// Postponed leaves per lane (LDS).  Forward on the metric scene at 8 / 12 /
-// 16 entries: 4.69 / 4.68 / 5.90 ms at depth 4, 5.34 / 5.24 / 6.50 ms at
-// depth 8 (16 entries leave room for fewer blocks per CU).
-constexpr int kLeafQueue = 12;
+// Postponed leaves per lane (LDS).  Forward on the metric scene with
+// lane-serial leaf tests at 8 / 12 / 16 entries: 4.69 / 4.68 / 5.90 ms at
+// depth 4, 5.34 / 5.24 / 6.50 ms at depth 8 (16 entries leave room for
+// fewer blocks per CU); 8 leave room for the wave-wide tests' slots.
+constexpr int kLeafQueue = 8;
 static_assert(kLeafQueue <= 15, "the queue length is kept in 4 bits of a query's state");
 constexpr int kTraceLdsWords = kStackLds + kLeafQueue;  // LDS words per lane for traversal
+// LDS words per lane of the wave-wide leaf tests' result slots (a 64-bit
+// key and two barycentrics), after the traversal words
+constexpr int kTraceSlotWords = 4;
 
 struct LaunchCfg {
   int grid;             // persistent grid size (blocks)
