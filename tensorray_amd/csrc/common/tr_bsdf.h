@@ -204,12 +204,12 @@ TR_HD F3 ggx_vndf_sample(F3 v, float alpha, float s0, float s1) {
   return normalize(F3{-sx, -sy, 1.0f});
 }
 
-// Microfacet::EvalInner (Microfacet.cpp:37-73): value per channel.
-TR_HD F3 microfacet_inner(F3 wo, F3 wi, float r, F3 kd, F3 F0) {
+// Microfacet::EvalInner (Microfacet.cpp:37-73): value per channel, given
+// the half vector wh = normalize(wo + wi).
+TR_HD F3 microfacet_inner_wh(F3 wo, F3 wi, F3 wh, float r, F3 kd, F3 F0) {
   float nv = wo.z, nl = wi.z;
   bool active = nv > 0.0f && nl > 0.0f;
   F3 diff = kd * kInvPi;
-  F3 wh = normalize(wo + wi);
   float nh = wh.z, vh = dot(wo, wh);
   float alpha = r * r;
   float k = (r + 1.0f) * (r + 1.0f) * 0.125f;
@@ -226,6 +226,9 @@ TR_HD F3 microfacet_inner(F3 wo, F3 wi, float r, F3 kd, F3 F0) {
   F3 spec = num / (den + 1e-6f);
   F3 val = diff + spec;
   return active ? val : F3{0.0f, 0.0f, 0.0f};
+}
+TR_HD F3 microfacet_inner(F3 wo, F3 wi, float r, F3 kd, F3 F0) {
+  return microfacet_inner_wh(wo, wi, normalize(wo + wi), r, kd, F0);
 }
 
 // BSDF::FresnelConductor (BSDF.cpp:304-323), one channel.
@@ -247,10 +250,9 @@ TR_HD float fresnel_conductor(float ci, float eta, float k) {
 
 // RoughConductor::EvalInner (Roughconductor.cpp:37-57): GGX with alpha =
 // roughness (no squaring, unlike Microfacet), conductor Fresnel.
-TR_HD F3 roughconductor_inner(F3 wo, F3 wi, float r, F3 eta, F3 k) {
+TR_HD F3 roughconductor_inner_wh(F3 wo, F3 wi, F3 wh, float r, F3 eta, F3 k) {
   const F3 zero{0.0f, 0.0f, 0.0f};
   if (!(wo.z > 0.0f && wi.z > 0.0f)) return zero;
-  F3 wh = normalize(wo + wi);
   float D = ggx_d(wh, r);
   if (D == 0.0f) return zero;
   float G = smith_g(wo, wh, r) * smith_g(wi, wh, r);
@@ -260,17 +262,20 @@ TR_HD F3 roughconductor_inner(F3 wo, F3 wi, float r, F3 eta, F3 k) {
   float den = 4.0f * fabsf(wi.z) * fabsf(wo.z);
   return Fr * D * G / den;
 }
+TR_HD F3 roughconductor_inner(F3 wo, F3 wi, float r, F3 eta, F3 k) {
+  return roughconductor_inner_wh(wo, wi, normalize(wo + wi), r, eta, k);
+}
 
 // Microfacet::PdfInner (Microfacet.cpp:152-165); RoughConductor::PdfInner
 // (Roughconductor.cpp:107-121) is the same expression.
-TR_HD float microfacet_pdf(F3 wo, F3 wi, float r) {
-  F3 wh = normalize(wo + wi);
+TR_HD float microfacet_pdf_wh(F3 wo, F3 wi, F3 wh, float r) {
   float mp = ggx_vndf_pdf(wo, wh, r);
   float dwh = 1.0f / (4.0f * dot(wi, wh));
   float pdf = fabsf(mp * dwh);
   bool refl = wo.z > 0.0f && mp != 0.0f && wi.z > 0.0f;
   return refl ? pdf : 0.0f;
 }
+TR_HD float microfacet_pdf(F3 wo, F3 wi, float r) { return microfacet_pdf_wh(wo, wi, normalize(wo + wi), r); }
 
 // BSDF::OutDirValid (BSDF.h:133) for non-Lambertian materials.
 TR_HD bool base_dir_valid(const Its& s, F3 w) { return dot(s.ns, w) * dot(s.ng, w) > 0.0f; }
@@ -295,6 +300,23 @@ TR_HD float bsdf_pdf(const SceneDev& S, const Its& s, F3 wo_w, F3 wi_w) {
   F3 wo = to_local(s, wo_w), wi = to_local(s, wi_w);
   if (m.type == kMatDiffuse) return diffuse_pdf(wo, wi);
   return microfacet_pdf(wo, wi, m.roughness);
+}
+
+// bsdf_eval and bsdf_pdf of one pair of directions (the same values as
+// the two calls), sharing the local frame and the half vector.
+TR_HD F3 bsdf_eval_pdf(const SceneDev& S, const Its& s, F3 wo_w, F3 wi_w, float* pdf) {
+  const Material& m = S.mats[s.mat];
+  F3 wo = to_local(s, wo_w), wi = to_local(s, wi_w);
+  if (m.type == kMatDiffuse) {
+    *pdf = diffuse_pdf(wo, wi);
+    return mat_albedo(S, m, s) * diffuse_inner(wo, wi);
+  }
+  const F3 wh = normalize(wo + wi);
+  *pdf = microfacet_pdf_wh(wo, wi, wh, m.roughness);
+  if (m.type == kMatRoughConductor)
+    return roughconductor_inner_wh(wo, wi, wh, m.roughness, F3{m.eta[0], m.eta[1], m.eta[2]},
+                                   F3{m.k[0], m.k[1], m.k[2]});
+  return microfacet_inner_wh(wo, wi, wh, m.roughness, mat_albedo(S, m, s), F3{m.spec[0], m.spec[1], m.spec[2]});
 }
 
 TR_HD bool bsdf_dir_valid(const SceneDev& S, const Its& s, F3 w) {

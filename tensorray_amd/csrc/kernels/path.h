@@ -141,11 +141,12 @@ __device__ void path_bounces(const SceneDev& S, Src& tc, Its its, F3 beta, F3 ra
       if (stable && noleak) visible = src_visible(S, tc, b, its.p, wi, dist);
       float valid = (stable && noleak && visible) ? 1.0f : 0.0f;
       float Gl = fabsf(dot(ls.n, -wi)) / d2;
-      F3 f = bsdf_eval(S, its, wo, wi) * fabsf(dsh);
+      float pbs;
+      F3 f = bsdf_eval_pdf(S, its, wo, wi, &pbs) * fabsf(dsh);
       F3 Le = emitted(S, ls.emitter, ls.n, -wi);
       float JL = S.shade[ls.tri].area / S.shade[ls.tri].area;
       F3 val = valid * beta * f * JL * Gl * Le / ls.pdf;
-      float pb = bsdf_pdf(S, its, wo, wi) * Gl;
+      float pb = pbs * Gl;
       float mis = (ls.pdf * ls.pdf) / (ls.pdf * ls.pdf + pb * pb);
       val = val * mis;
       ret = ret + val;
@@ -539,11 +540,12 @@ __device__ void render_paths_regen(const SceneDev& S, const CameraDev& C, const 
       if (kCkpt && visible) meta |= 1u << (8u + L.b());
       float valid = visible ? 1.0f : 0.0f;
       float Gl = fabsf(dot(g.ls.n, -g.wi)) / g.d2;
-      F3 f = bsdf_eval(S, its, wo, g.wi) * fabsf(g.dsh);
+      float pbs;
+      F3 f = bsdf_eval_pdf(S, its, wo, g.wi, &pbs) * fabsf(g.dsh);
       F3 Le = emitted(S, g.ls.emitter, g.ls.n, -g.wi);
       float JL = S.shade[g.ls.tri].area / S.shade[g.ls.tri].area;
       F3 val = valid * L.beta * f * JL * Gl * Le / g.ls.pdf;
-      float pb = bsdf_pdf(S, its, wo, g.wi) * Gl;
+      float pb = pbs * Gl;
       float mis = (g.ls.pdf * g.ls.pdf) / (g.ls.pdf * g.ls.pdf + pb * pb);
       val = val * mis;
       L.ret = zero;
