@@ -265,38 +265,39 @@ __device__ F3 path_forward(const SceneDev& S, const CameraDev& C, const PathPara
 //
 // path_forward runs one path to completion per lane, so a wave waits for its
 // longest path and every traversal is entered by whichever lanes still live.
-// Here each loop iteration advances one ray query per lane -- the camera ray,
-// an NEE shadow ray or a BSDF ray, depending on the lane's stage -- and
-// shades the queries that have finished; a lane whose path ends takes the
-// next path of its wave's chunk (ballot + popcount hand-out, no global
-// atomics).  A query that is still running when most of the wave's have
-// finished keeps its traversal state and continues in the next iteration.
-// Each path evaluates the same expressions in the same order as
-// path_forward<false>, so the per-path radiance is identical.
+// Here each loop iteration advances one query per lane -- the camera ray, or
+// the two rays of a bounce (the NEE shadow ray, then the BSDF-sampled ray,
+// as one chained query from the vertex) -- and shades the lanes whose query
+// has finished; a lane whose path ends takes the next path of its wave's
+// chunk (ballot + popcount hand-out, no global atomics).  A query that is
+// still running when most of the wave's have finished keeps its traversal
+// state and continues in the next iteration.  Each path evaluates the same
+// expressions in the same order as path_forward<false>, so the per-path
+// radiance is identical.
 // ---------------------------------------------------------------------------
-enum PathStage : uint32_t { kStageIdle = 0, kStageCamera = 1, kStageShadow = 2, kStageBsdf = 3 };
+enum PathStage : uint32_t { kStageIdle = 0, kStageCamera = 1, kStageBounce = 2 };
 
 // Fraction (of 64) of a wave's queries still unfinished at which the wave
-// stops traversing and shades the finished ones.  Forward on the metric
-// scene, depth 4, at 4 / 8 / 16 / 20 / 24: 4.77 / 4.69 / 4.60 / 4.63 /
-// 4.76 ms; running every query to its end (0) is 6 % slower than 16.
+// stops traversing and shades the finished ones.
 constexpr int kTraceKeep64 = 16;
 
 struct PathLane {
   uint64_t path;
-  uint32_t sb;           // stage (bits 0-1) and bounce index b (bits 2-31), one register
+  // stage (bits 0-1), the bounce's BSDF direction has a nonzero pdf (bit 2),
+  // bounce index b (bits 3-31): one register
+  uint32_t sb;
   __device__ uint32_t stage() const { return sb & 3u; }
-  __device__ uint32_t b() const { return sb >> 2; }
+  __device__ uint32_t b() const { return sb >> 3; }
   __device__ void set_stage(uint32_t s) { sb = (sb & ~3u) | s; }
   int32_t tri;           // current vertex x_b
   float u, v;
-  F3 beta, raydir, contrib, ret;
+  F3 beta, raydir, contrib;
   float la, lb;          // light-sample uniforms of bounce b
-  float pdfb;            // BSDF-sample pdf of bounce b
-  // ray of this iteration (from tmin = kShadowEpsilon; a shadow ray is an
-  // any-hit query) and its traversal state, which holds the far bound; in
-  // the BSDF stage d is the sampled direction of bounce b
-  F3 o, d;
+  float pdfb;            // pdf of the BSDF direction d2 of bounce b
+  // query of this iteration from o (tmin = kShadowEpsilon): the camera ray
+  // along d, or the bounce's shadow ray along d (any hit) chained with its
+  // BSDF-sampled ray along d2; the traversal state holds the far bound
+  F3 o, d, d2;
   TraceState ts;
 };
 
@@ -369,15 +370,16 @@ struct TileSupply {
 
 // The BSDF-sampled ray of bounce b from `its` came back with hit `h`:
 // MIS-weighted emission, throughput update and the move to the next vertex,
-// or the end of the path (Integrator.cpp:232-293).  Returns true, with
-// `its` replaced by the new vertex x_{b+1}, when the path continues.
+// or the end of the path (Integrator.cpp:232-293).  ret: the bounce's NEE
+// contribution.  Returns true, with `its` replaced by the new vertex
+// x_{b+1}, when the path continues.
 template <bool kCkpt, class Finish>
 __device__ inline bool bsdf_hit(const SceneDev& S, const PathParams& P, const CkptDev& ck, Its& its, F3 wo,
-                                const Hit& h, PathLane& L, uint32_t& meta, Finish& finish) {
+                                const Hit& h, F3 ret, PathLane& L, uint32_t& meta, Finish& finish) {
   const F3 zero{0.0f, 0.0f, 0.0f};
-  bool ok = h.tri >= 0 && L.pdfb > 0.0f && bsdf_dir_valid(S, its, L.d);
+  bool ok = h.tri >= 0 && L.pdfb > 0.0f && bsdf_dir_valid(S, its, L.d2);
   if (!ok) {
-    if (finite3(L.ret)) L.contrib = L.contrib + L.ret;
+    if (finite3(ret)) L.contrib = L.contrib + ret;
     if (kCkpt) ck.meta[L.path] = meta;
     finish(L.path, L.contrib);
     L.set_stage(kStageIdle);
@@ -393,7 +395,6 @@ __device__ inline bool bsdf_hit(const SceneDev& S, const PathParams& P, const Ck
   float J2 = S.shade[its2.tri].area / S.shade[its2.tri].area;
   F3 beta2 = L.beta * f2 * G2 * J2 / ppdf;
   if (!(G2 > 0.0f)) beta2 = zero;
-  F3 ret = L.ret;
   if (its2.emitter >= 0) {
     F3 Le2 = emitted(S, its2.emitter, its2.ng, -nd);
     F3 val2 = beta2 * Le2;
@@ -406,7 +407,7 @@ __device__ inline bool bsdf_hit(const SceneDev& S, const PathParams& P, const Ck
   L.tri = h.tri; L.u = h.u; L.v = h.v;
   L.beta = beta2;
   L.raydir = nd;
-  L.sb += 4u;  // b + 1
+  L.sb += 8u;  // b + 1
   if (kCkpt) {
     ckpt_put_vertex(ck, L.path, (int)L.b(), h.tri, h.u, h.v);
     meta += 1u;
@@ -423,7 +424,7 @@ __device__ inline bool bsdf_hit(const SceneDev& S, const PathParams& P, const Ck
 
 // The wave renders the paths of its tile supply.  A bounce is two ray
 // queries, the shadow ray of next-event estimation and the BSDF-sampled
-// ray, each traced in an iteration of its own.
+// ray, traced as one chained query and shaded together.
 template <bool kCkpt, class Finish>
 __device__ void render_paths_regen(const SceneDev& S, const CameraDev& C, const PathParams& P, TraceCtx& tc,
                                    TileSupply& sup, const CkptDev& ck, Finish finish) {
@@ -460,20 +461,21 @@ __device__ void render_paths_regen(const SceneDev& S, const CameraDev& C, const 
       sup.next = next + n_idle < end ? next + n_idle : end;
     }
     if (__ballot(L.stage() != kStageIdle) == 0ull) break;
-    // ---- one ray per lane ----
+    // ---- one query per lane ----
     // The wave leaves the traversal loop once at most kTraceKeep64 / 64 of
     // its queries are unfinished; those lanes skip shading and resume their
     // query in the next iteration, beside the new rays of the others.
-    trace_run<true>(S, tc, L.o, L.d, kShadowEpsilon, L.stage() == kStageShadow, L.ts, kTraceKeep64);
+    // (a BSDF direction of pdf 0 ends the path whatever it hits: its query
+    // is given an empty interval)
+    trace_run<true>(S, tc, L.o, L.d, L.d2, (L.sb & 4u) ? 1e32f : -1.0f, kShadowEpsilon,
+                    L.stage() == kStageBounce, L.ts, kTraceKeep64);
     if (!trace_finished(L.ts)) continue;
     const Hit h = trace_hit(L.ts);
     // ---- shade ----
-    // Every stage starts from one surface interaction -- the first vertex
-    // for a camera ray, the current vertex x_b for a shadow or BSDF ray --
-    // and ends with one Philox draw and one light-sample geometry: the next
-    // shadow ray's for camera and BSDF lanes, the traced one's for shadow
-    // lanes.  Those are evaluated once for the whole wave, whatever the
-    // lanes' stages, instead of once in each stage's masked branch.
+    // Both stages start from one surface interaction -- the first vertex for
+    // a camera ray, the current vertex x_b for a bounce -- and end with the
+    // next bounce's set-up (light sample, shadow ray, BSDF direction), each
+    // evaluated once for the whole wave.
     const uint32_t stage = L.stage();
     if (stage == kStageIdle) continue;  // no path left for this lane
     const bool cam = stage == kStageCamera;
@@ -515,28 +517,12 @@ __device__ void render_paths_regen(const SceneDev& S, const CameraDev& C, const 
         L.set_stage(kStageIdle);
         continue;
       }
-    } else if (stage == kStageBsdf) {
-      if (!bsdf_hit<kCkpt>(S, P, ck, its, -L.raydir, h, L, meta, finish)) continue;
-    }
-    // Next-event estimation at `its` (Integrator.cpp:196-230): camera and
-    // BSDF lanes draw the light sample of bounce b and set up its shadow
-    // ray; shadow lanes evaluate the sample they traced and draw the
-    // bounce's BSDF direction.
-    const bool shadow = stage == kStageShadow;
-    const Rnd4 rn = rng4(P.seed, kRngInterior, L.path, 1u + 2u * L.b() + (shadow ? 1u : 0u));
-    if (!shadow) {
-      L.la = rn.a;
-      L.lb = rn.b;
-    }
-    const NeeGeo g = nee_geo(S, its, L.la, L.lb);
-    if (!shadow) {
-      L.set_stage(kStageShadow);
-      L.o = its.p;
-      L.d = g.wi;
-      trace_begin(L.ts, g.ray ? g.dist - kShadowEpsilon : -1.0f);
     } else {
+      // next-event estimation of bounce b (Integrator.cpp:196-230) with the
+      // traced shadow ray's outcome, then the BSDF-sampled ray's hit
       const F3 wo = -L.raydir;
-      bool visible = g.ray && h.tri < 0;
+      const NeeGeo g = nee_geo(S, its, L.la, L.lb);
+      bool visible = g.ray && !trace_first_hit(L.ts);
       if (kCkpt && visible) meta |= 1u << (8u + L.b());
       float valid = visible ? 1.0f : 0.0f;
       float Gl = fabsf(dot(g.ls.n, -g.wi)) / g.d2;
@@ -548,13 +534,23 @@ __device__ void render_paths_regen(const SceneDev& S, const CameraDev& C, const 
       float pb = pbs * Gl;
       float mis = (g.ls.pdf * g.ls.pdf) / (g.ls.pdf * g.ls.pdf + pb * pb);
       val = val * mis;
-      L.ret = zero;
-      L.ret = L.ret + val;
-      L.d = bsdf_sample(S, its, wo, rn.a, rn.b, &L.pdfb);
-      L.set_stage(kStageBsdf);
-      L.o = its.p;
-      trace_begin(L.ts, 1e32f);
+      F3 ret = zero;
+      ret = ret + val;
+      if (!bsdf_hit<kCkpt>(S, P, ck, its, wo, h, ret, L, meta, finish)) continue;
     }
+    // ---- bounce b at `its`: light sample, shadow ray and BSDF direction ----
+    // (Integrator.cpp:196-213 and 232-245; the draws of the reference's
+    // loop in its order: light sample 1 + 2b, BSDF sample 2 + 2b)
+    const Rnd4 rl = rng4(P.seed, kRngInterior, L.path, 1u + 2u * L.b());
+    L.la = rl.a;
+    L.lb = rl.b;
+    const NeeGeo g2 = nee_geo(S, its, rl.a, rl.b);
+    const Rnd4 rb = rng4(P.seed, kRngInterior, L.path, 1u + 2u * L.b() + 1u);
+    L.d2 = bsdf_sample(S, its, -L.raydir, rb.a, rb.b, &L.pdfb);
+    L.sb = (L.sb & ~7u) | kStageBounce | (L.pdfb > 0.0f ? 4u : 0u);
+    L.o = its.p;
+    L.d = g2.wi;
+    trace_begin_chain(L.ts, g2.ray ? g2.dist - kShadowEpsilon : -1.0f);
   }
 }
 

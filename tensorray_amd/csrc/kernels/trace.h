@@ -127,11 +127,20 @@ struct TraceState {
   float best_t, u, v;  // closest accepted hit so far (t, barycentrics)
   uint32_t best_id;    // its triangle, kNoHit if none
   uint32_t node;       // next inner node to visit
-  uint32_t bits;       // stack depth (bits 0-7), queued leaves (8-11), inner_done (12), found (13)
+  uint32_t bits;       // stack depth (bits 0-7), queued leaves (8-11), inner_done (12), found (13),
+                       // chain (14), second (15), first query hit (16)
 };
 constexpr uint32_t kNoHit = 0xFFFFFFFFu;
 constexpr uint32_t kTsInnerDone = 1u << 12;
 constexpr uint32_t kTsFound = 1u << 13;
+// A chained pair of queries from one origin: an any-hit query (a shadow
+// ray) and then a closest-hit query in a second direction.  kTsChain: the
+// second query is still to start; kTsSecond: the current query is the
+// second; kTsFirstHit: the first query found a hit.
+constexpr uint32_t kTsChain = 1u << 14;
+constexpr uint32_t kTsSecond = 1u << 15;
+constexpr uint32_t kTsFirstHit = 1u << 16;
+constexpr uint32_t kTsKeep = kTsChain | kTsSecond | kTsFirstHit;  // bits carried across trace_run calls
 
 // A new query with far bound tmax.
 __device__ inline void trace_begin(TraceState& s, float tmax) {
@@ -142,15 +151,24 @@ __device__ inline void trace_begin(TraceState& s, float tmax) {
   s.node = 0u;
   s.bits = 0u;
 }
+// A chained pair: the any-hit query with far bound tmax, then the
+// closest-hit query (trace_run's d and d2, tmax2).
+__device__ inline void trace_begin_chain(TraceState& s, float tmax) {
+  trace_begin(s, tmax);
+  s.bits = kTsChain;
+}
 // No query (a lane without a ray): finished and missed.
 __device__ inline void trace_none(TraceState& s) {
   s.best_id = kNoHit;
   s.bits = kTsInnerDone;
 }
-// The query has finished: no inner node left and no leaf queued.
+// The query (or both of a chained pair) has finished: no inner node left,
+// no leaf queued, no chained query to start.
 __device__ inline bool trace_finished(const TraceState& s) {
-  return (s.bits & (kTsInnerDone | 0xF00u)) == kTsInnerDone;
+  return (s.bits & (kTsInnerDone | 0xF00u | kTsChain)) == kTsInnerDone;
 }
+// Of a finished chained pair: whether the first (any-hit) query hit.
+__device__ inline bool trace_first_hit(const TraceState& s) { return (s.bits & kTsFirstHit) != 0u; }
 // Hit record of a finished query (ptx.cu:92-113 semantics).
 __device__ inline Hit trace_hit(const TraceState& s) {
   Hit h;
@@ -301,11 +319,18 @@ __device__ inline void flush_wave(const SceneDev& S, TraceCtx& ctx, const RayPre
 // kWave: the queued leaves are tested by the whole wave (flush_wave), which
 // needs all 64 lanes active and the LDS slots of the context; otherwise
 // each lane tests its own queue.
+//
+// A chained pair (trace_begin_chain) runs its any-hit query along d, then,
+// within the same loop, its closest-hit query along d2 from the same origin
+// with far bound tmax2.
 template <bool kWave>
-__device__ inline void trace_run(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, float tmin, bool any,
-                                 TraceState& st, int keep64) {
-  RayPre rp = ray_pre(o, d);
-  const float ix = safe_inv(d.x), iy = safe_inv(d.y), iz = safe_inv(d.z);
+__device__ inline void trace_run(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, F3 d2, float tmax2, float tmin,
+                                 bool any, TraceState& st, int keep64) {
+  uint32_t keep = st.bits & kTsKeep;
+  const F3 dq = (keep & kTsSecond) ? d2 : d;
+  bool anyq = any && !(keep & kTsSecond);
+  RayPre rp = ray_pre(o, dq);
+  float ix = safe_inv(dq.x), iy = safe_inv(dq.y), iz = safe_inv(dq.z);
   float best_t = st.best_t, hu = st.u, hv = st.v;
   uint32_t best_id = st.best_id;
   uint32_t node = st.node;
@@ -314,9 +339,26 @@ __device__ inline void trace_run(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, f
   bool inner_done = (st.bits & kTsInnerDone) != 0u;
   bool found = (st.bits & kTsFound) != 0u;
   int n_keep = 0;
-  if (keep64 > 0) n_keep = (__popcll(__ballot(!inner_done || nq > 0)) * keep64) >> 6;
+  if (keep64 > 0) n_keep = (__popcll(__ballot(!inner_done || nq > 0 || (keep & kTsChain))) * keep64) >> 6;
   const int qs = ctx.lds_stride;
   for (;;) {
+    // ---- the first query of a chained pair has ended: start the second ----
+    if ((keep & kTsChain) && inner_done && nq == 0) {
+      keep = kTsSecond | (best_id != kNoHit ? kTsFirstHit : 0u);
+      anyq = false;
+      rp = ray_pre(o, d2);
+      ix = safe_inv(d2.x);
+      iy = safe_inv(d2.y);
+      iz = safe_inv(d2.z);
+      best_t = tmax2;
+      hu = 0.0f;
+      hv = 0.0f;
+      best_id = kNoHit;
+      node = 0u;
+      sp = 0;
+      inner_done = false;
+      found = false;
+    }
     // ---- visit one inner node (lanes with room in their leaf queue) ----
     if (!inner_done && nq <= kLeafQueue - 4) {
       ctx.n_nodes++;
@@ -376,7 +418,7 @@ __device__ inline void trace_run(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, f
     }
     // ---- decide, wave-wide, whether queued leaves are tested now ----
     const bool pend = nq > 0;
-    const bool live = !inner_done || pend;
+    const bool live = !inner_done || pend || (keep & kTsChain);
     const uint64_t m_live = __ballot(live);
     if (m_live == 0ull) break;
     if (__popcll(m_live) <= n_keep) break;
@@ -395,7 +437,7 @@ __device__ inline void trace_run(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, f
     }
     if constexpr (kWave) {
       if (flush) {
-        flush_wave(S, ctx, rp, tmin, any, nq, best_t, hu, hv, best_id, found);
+        flush_wave(S, ctx, rp, tmin, anyq, nq, best_t, hu, hv, best_id, found);
         nq = 0;
         if (found) inner_done = true;
       }
@@ -423,7 +465,7 @@ __device__ inline void trace_run(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, f
         // best_t is the far bound tmax until a hit is accepted, and below
         // it afterwards, so these tests keep every hit inside (tmin, tmax)
         if (!(tt > tmin)) continue;
-        if (any) {
+        if (anyq) {
           if (!(tt < best_t)) continue;
           best_id = id; best_t = tt; hu = uu; hv = vv;
           found = true;
@@ -442,7 +484,7 @@ __device__ inline void trace_run(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d, f
   st.v = hv;
   st.best_id = best_id;
   st.node = node;
-  st.bits = (uint32_t)sp | ((uint32_t)nq << 8) | (inner_done ? kTsInnerDone : 0u) | (found ? kTsFound : 0u);
+  st.bits = (uint32_t)sp | ((uint32_t)nq << 8) | (inner_done ? kTsInnerDone : 0u) | (found ? kTsFound : 0u) | keep;
 }
 
 // One query to the end.
@@ -450,7 +492,7 @@ __device__ inline Hit trace_ray_rt(const SceneDev& S, TraceCtx& ctx, F3 o, F3 d,
   ctx.n_queries++;
   TraceState st;
   trace_begin(st, tmax);
-  trace_run<false>(S, ctx, o, d, tmin, any, st, 0);
+  trace_run<false>(S, ctx, o, d, d, tmax, tmin, any, st, 0);
   return trace_hit(st);
 }
 
