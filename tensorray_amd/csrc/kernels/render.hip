@@ -22,8 +22,7 @@ __device__ inline TraceCtx make_trace_ctx(uint32_t* lds, const LaunchCfg& L) {
   t.lds_stride = kBlock;
   t.lds_depth = kStackLds;
   t.leafq = lds + kStackLds * kBlock + threadIdx.x;
-  const int gtid = blockIdx.x * kBlock + threadIdx.x;
-  t.spill = L.spill + gtid;
+  t.spill = L.spill;
   t.spill_stride = L.grid * kBlock;
   t.cap = L.cap;
   t.n_nodes = t.n_tris = t.n_queries = 0u;

@@ -95,7 +95,7 @@ struct TraceCtx {
   uint32_t* leafq;      // postponed-leaf queue of this lane (kLeafQueue entries, stride lds_stride)
   int lds_stride;       // blockDim.x
   int lds_depth;        // entries held in LDS
-  uint32_t* spill;      // &spill[gtid]
+  uint32_t* spill;      // the grid's spill buffer [depth][grid threads] (wave-uniform)
   int spill_stride;     // total threads of the grid
   int cap;              // total stack capacity (lds_depth + spill depth)
   // this lane's result slots in LDS for the wave-wide leaf tests of
@@ -107,14 +107,19 @@ struct TraceCtx {
   uint32_t n_nodes, n_tris, n_queries;
 };
 
+// This thread's column of the spill buffer, computed where it is used so
+// that no per-lane pointer stays live through the traversal.
+__device__ inline uint32_t spill_lane() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
 __device__ inline void stack_push(TraceCtx& c, int& sp, uint32_t v) {
   if (sp < c.lds_depth) c.lds[sp * c.lds_stride] = v;
-  else c.spill[(sp - c.lds_depth) * c.spill_stride] = v;
+  else c.spill[(size_t)(sp - c.lds_depth) * c.spill_stride + spill_lane()] = v;
   ++sp;
 }
 __device__ inline uint32_t stack_pop(TraceCtx& c, int& sp) {
   --sp;
-  return sp < c.lds_depth ? c.lds[sp * c.lds_stride] : c.spill[(sp - c.lds_depth) * c.spill_stride];
+  return sp < c.lds_depth ? c.lds[sp * c.lds_stride]
+                          : c.spill[(size_t)(sp - c.lds_depth) * c.spill_stride + spill_lane()];
 }
 
 __device__ inline F3 tri_vertex(const float* v) { return F3{v[0], v[1], v[2]}; }
