@@ -512,6 +512,9 @@ def main():
                     dram_frac=dram_gbs * 1e9 / MI355X_HBM_PEAK if dram_gbs else None,
                     achieved_is="modeled algorithmic bytes / kernel time; dram_gbs = measured DRAM bytes / "
                                 "kernel time",
+                    limiter="VALU issue under divergence and memory latency, not DRAM bandwidth: the "
+                            "modeled bytes are mostly BVH reads served by L2 / MALL (dram_frac); lane "
+                            "utilisation and issue counters in profiles/k_render_metric512.md",
                     kernel=kname, kernel_ms=kms,
                     algorithmic_bytes_per_launch=kbytes, fwd_kernel_ms=fwd_avg, adj_kernel_ms=adj_avg, **st)
     if a.copy_calibration:
