@@ -97,8 +97,10 @@ TR_HD F3 coordsys_vjp(F3 n, F3 adj_t, F3 adj_b) {
 // Microfacet::EvalInner VJP (forward in microfacet_inner).  Only called
 // when both cosines are positive (otherwise the value and its derivative
 // are zero).
-TR_HD void microfacet_inner_vjp(F3 wo, F3 wi, float r, F3 kd, F3 F0, F3 adj_f, F3* adj_wo, F3* adj_wi,
-                                float* adj_r, F3* adj_kd, F3* adj_F0) {
+// Returns the value microfacet_inner(wo, wi, r, kd, F0) of an active pair
+// (wo.z > 0, wi.z > 0), computed from the same intermediates.
+TR_HD F3 microfacet_inner_vjp(F3 wo, F3 wi, float r, F3 kd, F3 F0, F3 adj_f, F3* adj_wo, F3* adj_wi,
+                              float* adj_r, F3* adj_kd, F3* adj_F0) {
   const float c1 = -5.55473f, c2 = 6.8316f;
   float nv = wo.z, nl = wi.z;
   F3 hr = wo + wi;
@@ -148,6 +150,7 @@ TR_HD void microfacet_inner_vjp(F3 wo, F3 wi, float r, F3 kd, F3 F0, F3 adj_f, F
   F3 ahr = normalize_vjp(wh, hl, adj_wh);
   *adj_wo = F3{0.0f, 0.0f, adj_nv} + adj_vh * wh + ahr;
   *adj_wi = F3{0.0f, 0.0f, adj_nl} + ahr;
+  return kd * kInvPi + spec;
 }
 
 // d tan^2(theta) / d z for tan^2 = max(1 - z^2, 0) / z^2.
@@ -323,15 +326,19 @@ struct TangentSink {
   __device__ void vn(uint32_t v, F3 c) { acc += dot(c, load3(tvn, v)); }
 };
 
+// VJP of bsdf_eval(S, s, wo_w, wi_w) for the adjoint adj_f of its value;
+// returns the value (the same as bsdf_eval's).
 template <class Sink>
-__device__ inline void bsdf_eval_vjp(const SceneDev& S, const Its& s, F3 wo_w, F3 wi_w, F3 adj_f,
-                                     F3& adj_wo_w, F3& adj_wi_w, FrameAdj& fa, Sink& sk) {
+__device__ inline F3 bsdf_eval_vjp(const SceneDev& S, const Its& s, F3 wo_w, F3 wi_w, F3 adj_f,
+                                   F3& adj_wo_w, F3& adj_wi_w, FrameAdj& fa, Sink& sk) {
   const Material& m = S.mats[s.mat];
   F3 wo = to_local(s, wo_w), wi = to_local(s, wi_w);
   F3 awo{0.0f, 0.0f, 0.0f}, awi{0.0f, 0.0f, 0.0f};
+  F3 f;
   if (m.type == kMatDiffuse) {
     float inner = diffuse_inner(wo, wi);
-    if (inner == 0.0f) return;
+    f = mat_albedo(S, m, s) * inner;
+    if (inner == 0.0f) return f;
     F3 ga = adj_f * inner;
     if (m.tex >= 0) {
       sk.tex(S, m, s, ga);
@@ -340,10 +347,11 @@ __device__ inline void bsdf_eval_vjp(const SceneDev& S, const Its& s, F3 wo_w, F
       sk.mat(s.mat * kMatGradStride + 2, ga.y);
       sk.mat(s.mat * kMatGradStride + 3, ga.z);
     }
-    return;  // no directional dependence
+    return f;  // no directional dependence
   }
-  if (!(wo.z > 0.0f && wi.z > 0.0f)) return;
+  if (!(wo.z > 0.0f && wi.z > 0.0f)) return F3{0.0f, 0.0f, 0.0f};
   if (m.type == kMatRoughConductor) {
+    f = roughconductor_inner(wo, wi, m.roughness, F3{m.eta[0], m.eta[1], m.eta[2]}, F3{m.k[0], m.k[1], m.k[2]});
     float ar;
     F3 aeta, ak;
     roughconductor_inner_vjp(wo, wi, m.roughness, F3{m.eta[0], m.eta[1], m.eta[2]}, F3{m.k[0], m.k[1], m.k[2]},
@@ -359,7 +367,7 @@ __device__ inline void bsdf_eval_vjp(const SceneDev& S, const Its& s, F3 wo_w, F
     F3 F0{m.spec[0], m.spec[1], m.spec[2]};
     float ar;
     F3 akd, aF0;
-    microfacet_inner_vjp(wo, wi, m.roughness, kd, F0, adj_f, &awo, &awi, &ar, &akd, &aF0);
+    f = microfacet_inner_vjp(wo, wi, m.roughness, kd, F0, adj_f, &awo, &awi, &ar, &akd, &aF0);
     if (m.tex >= 0) sk.tex(S, m, s, akd);
     if (s.mat < kMaxMatGrad) {
       const int gb = s.mat * kMatGradStride;
@@ -376,12 +384,9 @@ __device__ inline void bsdf_eval_vjp(const SceneDev& S, const Its& s, F3 wo_w, F
   fa.t += awo.x * wo_w + awi.x * wi_w;
   fa.b += awo.y * wo_w + awi.y * wi_w;
   fa.ns += awo.z * wo_w + awi.z * wi_w;
+  return f;
 }
 
-// Forward value of the BSDF (same as bsdf_eval) -- kept next to its VJP.
-__device__ inline F3 bsdf_value(const SceneDev& S, const Its& s, F3 wo_w, F3 wi_w) {
-  return bsdf_eval(S, s, wo_w, wi_w);
-}
 
 template <class Sink>
 __device__ inline void tri_flush(const SceneDev& S, int32_t tri, const TriAdj& a, Sink& sk) {
@@ -457,14 +462,13 @@ __device__ void path_backward(const SceneDev& S, const CameraDev& C, const PathP
         float absSh = fabsf(dsh);
         float q = dot(lg.n, -wl);
         float GL = fabsf(q) / d2;
-        F3 f = bsdf_value(S, its, wo, wl);
-        a_val = f * absSh * GL * nc;
         F3 wn = (g * beta) * nc;
         F3 adj_f = wn * (absSh * GL);
+        F3 adj_wl = zero;
+        const F3 f = bsdf_eval_vjp(S, its, wo, wl, adj_f, adj_wo, adj_wl, fa, sk);
+        a_val = f * absSh * GL * nc;
         float sv = dot(wn, f);
         float adj_absSh = sv * GL, adj_GL = sv * absSh, adj_JL = sv * absSh * GL;
-        F3 adj_wl = zero;
-        bsdf_eval_vjp(S, its, wo, wl, adj_f, adj_wo, adj_wl, fa, sk);
         float adj_dsh = adj_absSh * sgnf(dsh);
         fa.ns += adj_dsh * wl;
         adj_wl += adj_dsh * its.ns;
@@ -496,17 +500,16 @@ __device__ void path_backward(const SceneDev& S, const CameraDev& C, const PathP
       F3 d = dr / dl;
       float qn = dot(ng.n, -d);
       float G = fabsf(qn) / dd2;
-      F3 f = bsdf_value(S, its, wo, d);
       float dc = dot(its.ns, d);
       float absc = fabsf(dc);
       float inv_pp = 1.0f / R.ppdf[i];
-      t_val = f * (absc * G * inv_pp);
       F3 gT = beta * Q;
       F3 adj_f = gT * (absc * G * inv_pp);
+      F3 adj_d = zero;
+      const F3 f = bsdf_eval_vjp(S, its, wo, d, adj_f, adj_wo, adj_d, fa, sk);
+      t_val = f * (absc * G * inv_pp);
       float sv = dot(gT, f) * inv_pp;
       float adj_absc = sv * G, adj_G = sv * absc, adj_J = sv * absc * G;
-      F3 adj_d = zero;
-      bsdf_eval_vjp(S, its, wo, d, adj_f, adj_wo, adj_d, fa, sk);
       float adj_dc = adj_absc * sgnf(dc);
       fa.ns += adj_dc * d;
       adj_d += adj_dc * its.ns;
