@@ -95,6 +95,11 @@ GradCopies make_grad_copies(int n_verts, hipStream_t st) {
   return g;
 }
 
+// Words of a stream's spill buffer: the traversal stacks' spill area, then
+// the forward's tile counter (kTileCtrWords words, one used).
+constexpr size_t kTileCtrWords = 64;
+static size_t spill_words(int max_grid) { return (size_t)kSpillDepth * (size_t)max_grid * kBlock; }
+
 // Launch shape for `work_items` items over scene `s`, with the spill buffer of
 // stream `st` (none when with_spill is false; the caller supplies one).
 LaunchCfg make_cfg(trc_ctx* ctx, const trc_scene* s, hipStream_t st, long long work_items, bool with_spill = true) {
@@ -108,6 +113,7 @@ LaunchCfg make_cfg(trc_ctx* ctx, const trc_scene* s, hipStream_t st, long long w
   L.cap = kStackLds + kSpillDepth;
   L.cus = ctx->num_cus;
   L.spill = nullptr;
+  L.tiles = nullptr;
   if (!with_spill) return L;
   // one spill buffer per (context, stream): launches on different streams
   // never share one (word 0 doubles as the chunk counter of the adjoint)
@@ -115,11 +121,11 @@ LaunchCfg make_cfg(trc_ctx* ctx, const trc_scene* s, hipStream_t st, long long w
   auto it = ctx->spill.find(st);
   if (it == ctx->spill.end()) {
     uint32_t* p = nullptr;
-    size_t words = (size_t)kSpillDepth * (size_t)max_grid * kBlock;
-    hip_check(hipMalloc(&p, words * 4), "hipMalloc(spill)");
+    hip_check(hipMalloc(&p, (spill_words(max_grid) + kTileCtrWords) * 4), "hipMalloc(spill)");
     it = ctx->spill.emplace(st, p).first;
   }
   L.spill = it->second;
+  L.tiles = it->second + spill_words(max_grid);
   return L;
 }
 

@@ -332,6 +332,10 @@ struct TileSupply {
   uint64_t next, end;            // paths not yet handed out of the current tile
   uint64_t t, tend, step;        // tile cursor
   uint64_t base, limit, tile, t0;
+  // tile counter: when set, the wave's tiles after its first (tile t, one
+  // per wave) are t0 + step + the counter's next value, taken as the wave
+  // needs one; otherwise t, t + step, ...
+  uint32_t* ctr;
   __device__ void load() {
     next = base + (t - t0) * tile;
     end = next + tile < limit ? next + tile : limit;
@@ -347,6 +351,7 @@ struct TileSupply {
     s.base = base;
     s.limit = limit;
     s.tile = tile;
+    s.ctr = nullptr;
     if (t < tend) s.load();
     else s.next = s.end = 0;
     return s;
@@ -358,7 +363,13 @@ struct TileSupply {
       next = end;
       return false;
     }
-    t += step;
+    if (ctr) {
+      uint32_t v = 0u;
+      if ((threadIdx.x & 63u) == 0u) v = atomicAdd(ctr, 1u);
+      t = t0 + step + (uint64_t)(uint32_t)__shfl((int)v, 0);
+    } else {
+      t += step;
+    }
     if (t >= tend) {
       next = end;
       return false;

@@ -68,6 +68,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kFw
   const uint64_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
   const uint64_t ntiles = (count + tile - 1) / tile;
   TileSupply sup = TileSupply::make(wave, ntiles, nwaves, 0, first, first + count, tile);
+  sup.ctr = L.tiles;
   render_paths_regen<kCkpt>(S, C, P, tc, sup, ck, [&](uint64_t p, F3 c) {
     const uint64_t q = p - paths_first;
     paths[3 * q] = c.x;
@@ -518,7 +519,11 @@ hipError_t launch_intersect(const SceneDev& S, const LaunchCfg& L, const float* 
 }
 
 // About four tiles per wave of the grid, 64..4096 paths each.
-static uint32_t forward_tile(uint64_t npaths, int grid) {
+// Paths per tile: with the tile counter, small tiles taken as waves need
+// them (kDynTile); dealt statically, about four tiles per wave.
+constexpr uint32_t kDynTile = 64;
+static uint32_t forward_tile(uint64_t npaths, int grid, bool dynamic) {
+  if (dynamic) return kDynTile;
   const uint64_t waves = (uint64_t)grid * (kBlock / 64);
   const uint64_t t = npaths / (waves * 4) / 64 * 64;
   return (uint32_t)(t < 64 ? 64 : (t > 4096 ? 4096 : t));
@@ -529,8 +534,9 @@ static void launch_paths(const SceneDev& S, const CameraDev& C, const PathParams
                          uint64_t first, uint64_t count, float* paths, uint64_t paths_first, const CkptDev& ck,
                          hipStream_t st) {
   const LaunchCfg Lk = fit_grid(k_render_paths<kCkpt>, L);
+  if (Lk.tiles) launch_fill_zero(reinterpret_cast<float*>(Lk.tiles), 1, st);
   hipLaunchKernelGGL((k_render_paths<kCkpt>), dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, first,
-                     count, forward_tile(count, Lk.grid), paths, paths_first, ck);
+                     count, forward_tile(count, Lk.grid, Lk.tiles != nullptr), paths, paths_first, ck);
 }
 
 hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,

@@ -142,6 +142,7 @@ struct LaunchCfg {
   uint32_t* spill;      // spill buffer [depth][grid*kBlock]
   int cap;              // stack capacity
   int cus;              // compute units of the device
+  uint32_t* tiles;      // the forward's tile counter (one word, zeroed per launch); null: tiles dealt statically
 };
 
 // A persistent grid-stride kernel should not launch more blocks than can be
