@@ -953,9 +953,10 @@ trc_status trc_plan_create(trc_ctx* ctx, const trc_job* jobs, int32_t n, int32_t
       // launch shape of the deepest scene, with a spill buffer owned by the plan
       // so that launches never allocate
       p->L = make_cfg(ctx, deepest, nullptr, (long long)p->total, false);
-      const size_t words = (size_t)kSpillDepth * (size_t)ctx->num_cus * kBlocksPerCU * kBlock;
-      hip_check(hipMalloc((void**)&p->spill, words * 4), "hipMalloc(plan spill)");
+      const size_t words = spill_words(ctx->num_cus * kBlocksPerCU);
+      hip_check(hipMalloc((void**)&p->spill, (words + kTileCtrWords) * 4), "hipMalloc(plan spill)");
       p->L.spill = p->spill;
+      p->L.tiles = p->spill + words;
       p->tiled = mark_solo(p->jd);
       plan_upload_tiles(p.get(), default_tile(ctx, p->tiled));
       hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");

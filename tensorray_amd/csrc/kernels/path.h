@@ -333,8 +333,9 @@ struct TileSupply {
   uint64_t t, tend, step;        // tile cursor
   uint64_t base, limit, tile, t0;
   // tile counter: when set, the wave's tiles after its first (tile t, one
-  // per wave) are t0 + step + the counter's next value, taken as the wave
-  // needs one; otherwise t, t + step, ...
+  // per wave: tiles 0 .. step-1 of the launch are dealt so) are step + the
+  // counter's next value, taken as the wave needs one; otherwise t,
+  // t + step, ...  Tile numbers count over the whole launch.
   uint32_t* ctr;
   __device__ void load() {
     next = base + (t - t0) * tile;
@@ -366,7 +367,7 @@ struct TileSupply {
     if (ctr) {
       uint32_t v = 0u;
       if ((threadIdx.x & 63u) == 0u) v = atomicAdd(ctr, 1u);
-      t = t0 + step + (uint64_t)(uint32_t)__shfl((int)v, 0);
+      t = step + (uint64_t)(uint32_t)__shfl((int)v, 0);
     } else {
       t += step;
     }

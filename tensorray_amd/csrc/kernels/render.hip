@@ -359,6 +359,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kFw
     // the wave's tiles of this job, rendered without draining between them
     const uint64_t tend = t0 + (count + tile - 1) / tile;
     TileSupply sup = TileSupply::make(t, tend, nwaves, t0, first, first + count, tile);
+    sup.ctr = L.tiles;
     render_paths_regen<true>(S, C, P, tc, sup, ck, [&](uint64_t p, F3 c) {
       paths[3 * p] = c.x;
       paths[3 * p + 1] = c.y;
@@ -486,6 +487,7 @@ hipError_t launch_render_batch(const JobDev* jobs_dev, const JobDev* jobs_host, 
   if (!adjoint) {
     if (ntiles == 0) return hipSuccess;
     const LaunchCfg Lk = fit_grid(k_render_batch_fwd, L);
+    if (Lk.tiles) launch_fill_zero(reinterpret_cast<float*>(Lk.tiles), 1, st);
     hipLaunchKernelGGL(k_render_batch_fwd, dim3(Lk.grid), dim3(kBlock), 0, st, jobs_dev, njobs, ntiles, tile, Lk);
     return hipGetLastError();
   }
