@@ -38,6 +38,29 @@ __device__ inline TraceCtx make_trace_ctx_b(uint32_t* lds, const LaunchCfg& L) {
 
 __device__ inline F3 cam_eye(const CameraDev& c) { return F3{c.eye[0], c.eye[1], c.eye[2]}; }
 
+// Calls f(i) for the items i of [first, end).  With a counter (zeroed before
+// the launch) a wave takes 64 consecutive items at a time as it finishes the
+// last ones, so no wave is left with a long tail of items dealt to it in
+// advance; without one, items are dealt grid-stride.  f returns to skip the
+// rest of an item.  All lanes of the wave must call it together.
+template <class F>
+__device__ inline void for_each_item(uint64_t first, uint64_t end, uint32_t* ctr, const LaunchCfg& L, F&& f) {
+  if (ctr) {
+    const uint32_t lane = threadIdx.x & 63u;
+    for (;;) {
+      uint32_t c = 0u;
+      if (lane == 0u) c = atomicAdd(ctr, 1u);
+      c = (uint32_t)__shfl((int)c, 0);
+      const uint64_t i0 = first + (uint64_t)c * 64u;
+      if (i0 >= end) break;
+      if (i0 + lane < end) f(i0 + lane);
+    }
+  } else {
+    const uint64_t stride = (uint64_t)L.grid * kBlock;
+    for (uint64_t i = first + (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < end; i += stride) f(i);
+  }
+}
+
 // Edge sampling with the length distribution; returns the edge index and
 // the pdf per unit length.
 __device__ inline int32_t sample_edge(const EdgeSet& E, float u, float* pdf) {
@@ -346,10 +369,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
   const F3 eye = cam_eye(C);
-  const uint64_t stride = (uint64_t)L.grid * kBlock;
   // no silhouette or boundary edge seen from this camera: no samples
   if ((E.dyn ? E.dyn->n : E.n) <= 0) return;
-  for (uint64_t i = first + (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < end; i += stride) {
+  for_each_item(first, end, L.tiles, L, [&](uint64_t i) {
     bool keep = false;
     Hit h;
     SensorSample sd;
@@ -385,7 +407,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
       w.u64(i); w.i(h.tri); w.f(h.u); w.f(h.v); w.f3(rd); w.f(sd.value); w.i(sd.pixel);
       w.f(bv); w.f(pdf); w.f3(n); w.i(ei); w.f(rb); w.f3(p);
     }
-  }
+  });
 }
 
 // eval: the radiance seen through the sensor point and the scatter.
@@ -447,8 +469,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
   const float plx = 2.0f * C.ratio * C.tan_half, ply = 2.0f * C.tan_half;
   const F3 vdir = normalize(C.ay * U), hdir = normalize(C.ax * R);
   const uint64_t per_anti = N / 4;
-  const uint64_t stride = (uint64_t)L.grid * kBlock;
-  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
+  for_each_item(0, N, L.tiles, L, [&](uint64_t i) {
     const uint64_t j = i % per_anti, anti = i / per_anti;
     float rr = rng4(P.seed, kRngPixelEdge, j, 0u).a;
     float rv;
@@ -471,7 +492,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
     float pdf = (side == 0 || side == 2) ? 0.25f / (ply * (float)C.width) : 0.25f / (plx * (float)C.height);
     F3 dir = normalize(p0 - eye);
     Hit h = trace_ray<false>(S, tc, p0, dir, kShadowEpsilon, 1e32f);
-    if (h.tri < 0) continue;
+    if (h.tri < 0) return;
     F3 hitP = p0 + h.t * dir;
     SensorSample sd = sensor_sample(C, hitP);
     F3 rd = camera_dir(C, sd.qx * (float)C.width, sd.qy * (float)C.height);
@@ -480,18 +501,18 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
     float bv = base_val_camera(its.p, its.ng, rd, p0, eye, edge, edge2, &n, true);
     F3 rad = boundary_radiance(S, tc, its, rd, sd.value, P.seed, kRngPixelEdge, j, P.max_bounces);
     F3 K = rad * bv / pdf;
-    if (!finite3(K)) continue;
+    if (!finite3(K)) return;
     if constexpr (kTangent) {
       TangentSink sk{tpos, nullptr, nullptr, 0.0f};
       scatter_xdn(S, h.tri, eye, p0, n, 1.0f, nullptr, 0.0f, -1, -1, 0.0f, 0.0f, sk);
       add_deriv(dimg, WH, (int32_t)pix, K * (sk.acc / (float)P.spp));
     } else {
       float w = dot(dLdI_at(dLdI, WH, (int32_t)pix), K) / (float)P.spp;
-      if (w == 0.0f) continue;
+      if (w == 0.0f) return;
       GradSink sk{dpos, nullptr, nullptr, nullptr};
       scatter_xdn(S, h.tri, eye, p0, n, w, nullptr, 0.0f, -1, -1, 0.0f, 0.0f, sk);
     }
-  }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -630,8 +651,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
   const EdgeSet& E = S.sec;
-  const uint64_t stride = (uint64_t)L.grid * kBlock;
-  for (uint64_t i = first + (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < end; i += stride) {
+  for_each_item(first, end, L.tiles, L, [&](uint64_t i) {
     bool keep = false;
     Hit h0, h;
     F3 K0, rdir, p, n;
@@ -688,7 +708,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
       w.u64(i); w.f3(K0); w.i(h.tri); w.f(h.u); w.f(h.v); w.f3(rdir); w.i(h0.tri); w.f3(p); w.f3(n); w.i(ei);
       w.f(t);
     }
-  }
+  });
 }
 
 // eval: EvalImportance from the sensor-side hit for B bounces and the scatter.
@@ -759,8 +779,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
   __shared__ uint32_t lds_stack[kTraceLdsWords * kBlock];
   TraceCtx tc = make_trace_ctx_b(lds_stack, L);
   const EdgeSet& E = S.sec;
-  const uint64_t stride = (uint64_t)L.grid * kBlock;
-  for (uint64_t i = first + (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < end; i += stride) {
+  for_each_item(first, end, L.tiles, L, [&](uint64_t i) {
     bool keep = false;
     Hit he, hs;
     F3 dir, p, n;
@@ -822,7 +841,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBn
       w.u64(i); w.i(he.tri); w.f(he.u); w.f(he.v); w.i(hs.tri); w.f(hs.u); w.f(hs.v); w.f3(dir); w.f(v0);
       w.f3(p); w.f3(n); w.i(ei); w.f(t);
     }
-  }
+  });
 }
 
 // eval: the emitter-side radiance after 1..B-1 bounces, the sensor-side
@@ -936,6 +955,10 @@ static hipError_t launch_boundary_t(int kind, const SceneDev& S, const CameraDev
                                     hipStream_t st) {
   if (kind == 1) {
     const LaunchCfg Lk = fit_grid(k_boundary_pixel<kTangent>, L);
+    if (Lk.tiles) {
+      const hipError_t e = launch_fill_zero(reinterpret_cast<float*>(Lk.tiles), 1, st);
+      if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(k_boundary_pixel<kTangent>, dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, N, dLdI, dpos,
                        tpos, dimg);
     return hipGetLastError();
@@ -953,16 +976,19 @@ static hipError_t launch_boundary_t(int kind, const SceneDev& S, const CameraDev
     const BndQueue Q{counts + c, buf, cap};
     if (kind == 0) {
       const LaunchCfg Ls = fit_grid(k_bnd_primary_select, L), Le = fit_grid(k_bnd_primary_eval<kTangent>, L);
+      if (Ls.tiles && (e = launch_fill_zero(reinterpret_cast<float*>(Ls.tiles), 1, st)) != hipSuccess) break;
       hipLaunchKernelGGL(k_bnd_primary_select, dim3(Ls.grid), dim3(kBlock), 0, st, S, C, P, Ls, prim, first, end, Q);
       hipLaunchKernelGGL(k_bnd_primary_eval<kTangent>, dim3(Le.grid), dim3(kBlock), 0, st, S, C, P, Le, prim, Q,
                          dLdI, dpos, tpos, dimg);
     } else if (kind == 2) {
       const LaunchCfg Ls = fit_grid(k_bnd_direct_select, L), Le = fit_grid(k_bnd_direct_eval<kTangent>, L);
+      if (Ls.tiles && (e = launch_fill_zero(reinterpret_cast<float*>(Ls.tiles), 1, st)) != hipSuccess) break;
       hipLaunchKernelGGL(k_bnd_direct_select, dim3(Ls.grid), dim3(kBlock), 0, st, S, C, P, Ls, guide, first, end, Q);
       hipLaunchKernelGGL(k_bnd_direct_eval<kTangent>, dim3(Le.grid), dim3(kBlock), 0, st, S, C, P, Le, Q, dLdI, dpos,
                          tpos, dimg);
     } else {
       const LaunchCfg Ls = fit_grid(k_bnd_indirect_select, L), Le = fit_grid(k_bnd_indirect_eval<kTangent>, L);
+      if (Ls.tiles && (e = launch_fill_zero(reinterpret_cast<float*>(Ls.tiles), 1, st)) != hipSuccess) break;
       hipLaunchKernelGGL(k_bnd_indirect_select, dim3(Ls.grid), dim3(kBlock), 0, st, S, C, P, Ls, first, end, Q);
       hipLaunchKernelGGL(k_bnd_indirect_eval<kTangent>, dim3(Le.grid), dim3(kBlock), 0, st, S, C, P, Le, Q, dLdI,
                          dpos, tpos, dimg);
