@@ -392,7 +392,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAd
   uint32_t* bnext = bcount + kLenBuckets;
   const uint64_t nwaves = (uint64_t)L.grid * (kBlock / 64);
   const uint64_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
-  for (uint64_t t = wave; t < ntiles; t += nwaves) {
+  // the wave's first tile is tile `wave`; later ones come from the tile
+  // counter as the wave needs them (round-robin without one)
+  for (uint64_t t = wave; t < ntiles;) {
     const int j = job_of_tile(jobs, njobs, t);
     // read in place from the (read-only) job table: copying the records
     // costs registers, and spill, in the path loop
@@ -465,6 +467,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAd
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
+    if (L.tiles) {
+      uint32_t v = 0u;
+      if (lane == 0) v = atomicAdd(L.tiles, 1u);
+      t = nwaves + (uint64_t)(uint32_t)__shfl((int)v, 0);
+    } else {
+      t += nwaves;
+    }
   }
 }
 
@@ -487,12 +496,13 @@ hipError_t launch_render_batch(const JobDev* jobs_dev, const JobDev* jobs_host, 
   if (!adjoint) {
     if (ntiles == 0) return hipSuccess;
     const LaunchCfg Lk = fit_grid(k_render_batch_fwd, L);
-    if (Lk.tiles) launch_fill_zero(reinterpret_cast<float*>(Lk.tiles), 1, st);
+    if (Lk.tiles && (e = launch_fill_zero(reinterpret_cast<float*>(Lk.tiles), 1, st)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_render_batch_fwd, dim3(Lk.grid), dim3(kBlock), 0, st, jobs_dev, njobs, ntiles, tile, Lk);
     return hipGetLastError();
   }
   if (ntiles > 0) {
     const LaunchCfg Lk = fit_grid(k_render_batch_adj, L);
+    if (Lk.tiles && (e = launch_fill_zero(reinterpret_cast<float*>(Lk.tiles), 1, st)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_render_batch_adj, dim3(Lk.grid), dim3(kBlock), 0, st, jobs_dev, njobs, ntiles, tile, Lk);
     e = hipGetLastError();
   }
@@ -532,13 +542,17 @@ static uint32_t forward_tile(uint64_t npaths, int grid, bool dynamic) {
 }
 
 template <bool kCkpt>
-static void launch_paths(const SceneDev& S, const CameraDev& C, const PathParams& P, const LaunchCfg& L,
-                         uint64_t first, uint64_t count, float* paths, uint64_t paths_first, const CkptDev& ck,
-                         hipStream_t st) {
+static hipError_t launch_paths(const SceneDev& S, const CameraDev& C, const PathParams& P, const LaunchCfg& L,
+                               uint64_t first, uint64_t count, float* paths, uint64_t paths_first, const CkptDev& ck,
+                               hipStream_t st) {
   const LaunchCfg Lk = fit_grid(k_render_paths<kCkpt>, L);
-  if (Lk.tiles) launch_fill_zero(reinterpret_cast<float*>(Lk.tiles), 1, st);
+  if (Lk.tiles) {
+    const hipError_t e = launch_fill_zero(reinterpret_cast<float*>(Lk.tiles), 1, st);
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL((k_render_paths<kCkpt>), dim3(Lk.grid), dim3(kBlock), 0, st, S, C, P, Lk, first,
                      count, forward_tile(count, Lk.grid, Lk.tiles != nullptr), paths, paths_first, ck);
+  return hipGetLastError();
 }
 
 hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t seed, int max_bounces, int spp,
@@ -551,9 +565,8 @@ hipError_t launch_render_paths(const SceneDev& S, const CameraDev& C, uint32_t s
     count = np;
   }
   const CkptDev ck = ckpt ? ckpt_view(ckpt, np, max_bounces) : CkptDev{};
-  if (ckpt) launch_paths<true>(S, C, P, L, first, count, paths, paths_first, ck, st);
-  else launch_paths<false>(S, C, P, L, first, count, paths, paths_first, ck, st);
-  return hipGetLastError();
+  if (ckpt) return launch_paths<true>(S, C, P, L, first, count, paths, paths_first, ck, st);
+  return launch_paths<false>(S, C, P, L, first, count, paths, paths_first, ck, st);
 }
 
 uint64_t checkpoint_bytes(uint64_t npaths, int max_bounces) { return ckpt_bytes(npaths, max_bounces); }
