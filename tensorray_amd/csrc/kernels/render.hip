@@ -155,6 +155,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAd
 constexpr int kAdjSortR = 8;                   // paths per thread per chunk
 constexpr int kAdjChunk = kBlock * kAdjSortR;
 constexpr int kLenBuckets = kMaxBounces + 2;   // K + 1 in 0 .. max_bounces + 1
+constexpr uint64_t kAdjBig64 = 56;             // share (of 64) of the paths in full chunks
+constexpr uint64_t kAdjTail = 256;             // paths per chunk after them
 
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdjWaves))) k_render_adj_ckpt(SceneDev S, CameraDev C, PathParams P, LaunchCfg L,
                                                             uint64_t first, uint64_t count, CkptDev ck,
@@ -169,20 +171,26 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAd
   const uint32_t WH = (uint32_t)C.width * (uint32_t)C.height;
   const float inv_spp = 1.0f / (float)P.spp;
   const uint64_t end = first + count;
-  const uint64_t nchunks = (count + kAdjChunk - 1) / kAdjChunk;
+  // chunks of kAdjChunk paths for the first kAdjBig64/64 of the paths, then
+  // chunks of kAdjTail paths, so that the last chunks the blocks take are
+  // short and the blocks finish together
+  const uint64_t nbig = count * kAdjBig64 / 64 / kAdjChunk;
+  const uint64_t nchunks = nbig + (count - nbig * kAdjChunk + kAdjTail - 1) / kAdjTail;
   for (;;) {
     if (threadIdx.x == 0) s_ch = atomicAdd(chunk_ctr, 1u);
     if (threadIdx.x < kLenBuckets) bcount[threadIdx.x] = 0u;
     __syncthreads();
     const uint64_t ch = s_ch;
     if (ch >= nchunks) break;  // s_ch is the block's: every thread leaves together
-    const uint64_t base = first + ch * kAdjChunk;
+    const uint64_t base = ch < nbig ? first + ch * kAdjChunk : first + nbig * kAdjChunk + (ch - nbig) * kAdjTail;
+    const uint64_t cend0 = base + (ch < nbig ? kAdjChunk : kAdjTail);
+    const uint64_t cend = cend0 < end ? cend0 : end;
     uint32_t kb[kAdjSortR];
 #pragma unroll
     for (int r = 0; r < kAdjSortR; ++r) {
       const uint64_t p = base + threadIdx.x + (uint64_t)r * kBlock;
       kb[r] = 0u;
-      if (p < end) {
+      if (p < cend) {
         const uint32_t pix = (uint32_t)(p % WH);
         const bool zero = dLdI[pix] == 0.0f && dLdI[WH + pix] == 0.0f && dLdI[2 * WH + pix] == 0.0f;
         const uint32_t k1 = ck.meta[p] & 0xFFu;
