@@ -94,25 +94,72 @@ hipError_t launch_adam_uniform(float* x, const float* g, float* m, float* v, uin
 // definite (CSR), three right-hand sides at once in the [V][3] layout of the
 // vertex buffers.  Scalars live on the device (CgScalars); every kernel
 // returns at once when all three columns have converged, so the host can
-// queue max_iter iterations without reading anything back.
+// queue max_iter iterations without reading anything back.  A dot product
+// is a partial per block (no atomics: the partials are summed in block order
+// by every kernel that needs the total, so all of them see the same bits).
 // ---------------------------------------------------------------------------
-__device__ inline void block_add3(float a, float b, float c, float* dst) {
+constexpr int kCgThreads = 256;
+
+// The block's sum of (a, b, c) into part[blockIdx.x], in a fixed order.
+__device__ inline void block_partial3(float a, float b, float c, float (*part)[3]) {
+  __shared__ float w[kCgThreads / 64][3];
   for (int o = 32; o > 0; o >>= 1) {
     a += __shfl_xor(a, o);
     b += __shfl_xor(b, o);
     c += __shfl_xor(c, o);
   }
   if ((threadIdx.x & 63u) == 0u) {
-    atomicAdd(&dst[0], a);
-    atomicAdd(&dst[1], b);
-    atomicAdd(&dst[2], c);
+    w[threadIdx.x >> 6][0] = a;
+    w[threadIdx.x >> 6][1] = b;
+    w[threadIdx.x >> 6][2] = c;
   }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+    for (int k = 0; k < kCgThreads / 64; ++k) {
+      s0 += w[k][0];
+      s1 += w[k][1];
+      s2 += w[k][2];
+    }
+    part[blockIdx.x][0] = s0;
+    part[blockIdx.x][1] = s1;
+    part[blockIdx.x][2] = s2;
+  }
+}
+
+// Sum of part[0 .. nb) per column, the same bits in every block: the first
+// wave sums a fixed subset per lane, then reduces across lanes.
+__device__ inline void sum_partials3(const float (*part)[3], int nb, float out[3]) {
+  __shared__ float tot[3];
+  if (threadIdx.x < 64u) {
+    float a = 0.0f, b = 0.0f, c = 0.0f;
+    for (int k = (int)threadIdx.x; k < nb; k += 64) {
+      a += part[k][0];
+      b += part[k][1];
+      c += part[k][2];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      a += __shfl_xor(a, o);
+      b += __shfl_xor(b, o);
+      c += __shfl_xor(c, o);
+    }
+    if (threadIdx.x == 0u) {
+      tot[0] = a;
+      tot[1] = b;
+      tot[2] = c;
+    }
+  }
+  __syncthreads();
+  out[0] = tot[0];
+  out[1] = tot[1];
+  out[2] = tot[2];
 }
 
 __device__ inline bool cg_done(const CgScalars* s) { return s->active == 0u; }
 
-// y = M x (and, with acc, acc[c] += x_c . y_c)
-__global__ void k_csr_spmv3(CsrDev M, const float* x, float* y, CgScalars* s, int use_flag, int with_dot) {
+// y = M x (and, with with_dot, the partials of x_c . y_c)
+__global__ void __launch_bounds__(kCgThreads) k_csr_spmv3(CsrDev M, const float* x, float* y, CgScalars* s,
+                                                          int use_flag, int with_dot) {
   if (use_flag && cg_done(s)) return;
   float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f;
   const int stride = gridDim.x * blockDim.x;
@@ -134,11 +181,11 @@ __global__ void k_csr_spmv3(CsrDev M, const float* x, float* y, CgScalars* s, in
       d2 = d2 + x[3 * (size_t)i + 2] * a2;
     }
   }
-  if (with_dot) block_add3(d0, d1, d2, s->pap);
+  if (with_dot) block_partial3(d0, d1, d2, s->part_pap);
 }
 
-// r = b - Mx (Mx in r on entry), p = r, rr = r.r, rr0 = b.b
-__global__ void k_cg_init(const float* b, float* r, float* p, int n, CgScalars* s) {
+// r = b - Mx (Mx in r on entry), p = r; partials of r.r and b.b
+__global__ void __launch_bounds__(kCgThreads) k_cg_init(const float* b, float* r, float* p, int n, CgScalars* s) {
   float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f, e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
   const int stride = gridDim.x * blockDim.x;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -155,16 +202,19 @@ __global__ void k_cg_init(const float* b, float* r, float* p, int n, CgScalars* 
     e1 = e1 + b[3 * (size_t)i + 1] * b[3 * (size_t)i + 1];
     e2 = e2 + b[3 * (size_t)i + 2] * b[3 * (size_t)i + 2];
   }
-  block_add3(d0, d1, d2, s->rr);
-  block_add3(e0, e1, e2, s->rr0);
+  block_partial3(d0, d1, d2, s->part_rr);
+  __syncthreads();  // block_partial3's LDS is reused
+  block_partial3(e0, e1, e2, s->part_b);
 }
 
-// x += alpha p, r -= alpha Ap, rr_new += r.r   (alpha = rr / pAp per column)
-__global__ void k_cg_update(float* x, float* r, const float* p, const float* ap, int n, CgScalars* s) {
+// x += alpha p, r -= alpha Ap; partials of the new r.r   (alpha = rr / pAp)
+__global__ void __launch_bounds__(kCgThreads) k_cg_update(float* x, float* r, const float* p, const float* ap, int n,
+                                                          CgScalars* s) {
   if (cg_done(s)) return;
+  float pap[3];
+  sum_partials3(s->part_pap, (int)gridDim.x, pap);
   float alpha[3];
-  for (int c = 0; c < 3; ++c)
-    alpha[c] = ((s->active >> c) & 1u) && s->pap[c] > 0.0f ? s->rr[c] / s->pap[c] : 0.0f;
+  for (int c = 0; c < 3; ++c) alpha[c] = ((s->active >> c) & 1u) && pap[c] > 0.0f ? s->rr[c] / pap[c] : 0.0f;
   float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f;
   const int stride = gridDim.x * blockDim.x;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -177,14 +227,16 @@ __global__ void k_cg_update(float* x, float* r, const float* p, const float* ap,
     d1 = d1 + r[3 * (size_t)i + 1] * r[3 * (size_t)i + 1];
     d2 = d2 + r[3 * (size_t)i + 2] * r[3 * (size_t)i + 2];
   }
-  block_add3(d0, d1, d2, s->rr_new);
+  block_partial3(d0, d1, d2, s->part_rr);
 }
 
 // p = r + beta p  (beta = rr_new / rr per column)
-__global__ void k_cg_direction(const float* r, float* p, int n, CgScalars* s) {
+__global__ void __launch_bounds__(kCgThreads) k_cg_direction(const float* r, float* p, int n, CgScalars* s) {
   if (cg_done(s)) return;
+  float rr_new[3];
+  sum_partials3(s->part_rr, (int)gridDim.x, rr_new);
   float beta[3];
-  for (int c = 0; c < 3; ++c) beta[c] = s->rr[c] > 0.0f ? s->rr_new[c] / s->rr[c] : 0.0f;
+  for (int c = 0; c < 3; ++c) beta[c] = s->rr[c] > 0.0f ? rr_new[c] / s->rr[c] : 0.0f;
   const int stride = gridDim.x * blockDim.x;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     for (int c = 0; c < 3; ++c) {
@@ -193,23 +245,33 @@ __global__ void k_cg_direction(const float* r, float* p, int n, CgScalars* s) {
     }
 }
 
-// One thread: rr = rr_new, clear the accumulators, retire converged columns.
-__global__ void k_cg_scalars(CgScalars* s, float tol2, int first) {
+// One block: rr = the summed r.r partials (rr0 = b.b at the start), retire
+// converged columns.
+__global__ void __launch_bounds__(kCgThreads) k_cg_scalars(CgScalars* s, float tol2, int first, int nb) {
+  if (!first && cg_done(s)) return;
+  float rr[3];
+  sum_partials3(s->part_rr, nb, rr);
   if (first) {
+    float b2[3];
+    __syncthreads();  // sum_partials3's LDS is reused
+    sum_partials3(s->part_b, nb, b2);
+    if (threadIdx.x != 0u) return;
     s->active = 0u;
-    for (int c = 0; c < 3; ++c)
-      if (s->rr[c] > tol2 * s->rr0[c] && s->rr[c] > 0.0f) s->active |= 1u << c;
+    for (int c = 0; c < 3; ++c) {
+      s->rr[c] = rr[c];
+      s->rr0[c] = b2[c];
+      s->pap[c] = s->rr_new[c] = 0.0f;
+      if (rr[c] > tol2 * b2[c] && rr[c] > 0.0f) s->active |= 1u << c;
+    }
     s->iters = 0u;
-    for (int c = 0; c < 3; ++c) s->pap[c] = s->rr_new[c] = 0.0f;
     return;
   }
-  if (s->active == 0u) return;
+  if (threadIdx.x != 0u) return;
   s->iters += 1u;
   for (int c = 0; c < 3; ++c) {
-    s->rr[c] = s->rr_new[c];
-    s->rr_new[c] = 0.0f;
-    s->pap[c] = 0.0f;
-    if (!(s->rr[c] > tol2 * s->rr0[c])) s->active &= ~(1u << c);
+    s->rr[c] = rr[c];
+    s->rr_new[c] = rr[c];
+    if (!(rr[c] > tol2 * s->rr0[c])) s->active &= ~(1u << c);
   }
 }
 
@@ -219,27 +281,31 @@ __global__ void k_cg_reset(CgScalars* s) {
   s->iters = 0u;
 }
 
+static int cg_grid(int n) {
+  const int g = (n + kCgThreads - 1) / kCgThreads;
+  return g < 1 ? 1 : (g > kCgBlocks ? kCgBlocks : g);
+}
+
 hipError_t launch_csr_spmv3(const CsrDev& M, const float* x, float* y, hipStream_t st) {
-  const int grid = (M.n + 255) / 256 < 8192 ? (M.n + 255) / 256 : 8192;
-  hipLaunchKernelGGL(k_csr_spmv3, dim3(grid > 0 ? grid : 1), dim3(256), 0, st, M, x, y, (CgScalars*)nullptr, 0, 0);
+  hipLaunchKernelGGL(k_csr_spmv3, dim3(cg_grid(M.n)), dim3(kCgThreads), 0, st, M, x, y, (CgScalars*)nullptr, 0, 0);
   return hipGetLastError();
 }
 
 hipError_t launch_cg(const CsrDev& M, const float* b, float* x, float* r, float* p, float* ap, CgScalars* s,
                      int max_iter, float tol, hipStream_t st) {
   const int n = M.n;
-  const int grid = (n + 255) / 256 < 8192 ? (n + 255) / 256 : 8192;
-  const dim3 g(grid > 0 ? grid : 1), b256(256);
+  const int nb = cg_grid(n);
+  const dim3 g(nb), blk(kCgThreads);
   const float tol2 = tol * tol;
   hipLaunchKernelGGL(k_cg_reset, dim3(1), dim3(1), 0, st, s);
-  hipLaunchKernelGGL(k_csr_spmv3, g, b256, 0, st, M, x, r, s, 0, 0);  // r = M x
-  hipLaunchKernelGGL(k_cg_init, g, b256, 0, st, b, r, p, n, s);
-  hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(1), 0, st, s, tol2, 1);
+  hipLaunchKernelGGL(k_csr_spmv3, g, blk, 0, st, M, x, r, s, 0, 0);  // r = M x
+  hipLaunchKernelGGL(k_cg_init, g, blk, 0, st, b, r, p, n, s);
+  hipLaunchKernelGGL(k_cg_scalars, dim3(1), blk, 0, st, s, tol2, 1, nb);
   for (int it = 0; it < max_iter; ++it) {
-    hipLaunchKernelGGL(k_csr_spmv3, g, b256, 0, st, M, p, ap, s, 1, 1);
-    hipLaunchKernelGGL(k_cg_update, g, b256, 0, st, x, r, p, ap, n, s);
-    hipLaunchKernelGGL(k_cg_direction, g, b256, 0, st, r, p, n, s);
-    hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(1), 0, st, s, tol2, 0);
+    hipLaunchKernelGGL(k_csr_spmv3, g, blk, 0, st, M, p, ap, s, 1, 1);
+    hipLaunchKernelGGL(k_cg_update, g, blk, 0, st, x, r, p, ap, n, s);
+    hipLaunchKernelGGL(k_cg_direction, g, blk, 0, st, r, p, n, s);
+    hipLaunchKernelGGL(k_cg_scalars, dim3(1), blk, 0, st, s, tol2, 0, nb);
   }
   return hipGetLastError();
 }

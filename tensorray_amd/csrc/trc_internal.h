@@ -189,10 +189,17 @@ struct CsrDev {
   const int* col;    // [nnz]
   const float* val;  // [nnz]
 };
+// Blocks of the conjugate-gradient kernels (grid-stride over the rows); each
+// writes its share of a dot product to a partial, and the partials are
+// summed in block order, so the solve is deterministic.
+constexpr int kCgBlocks = 1024;
 struct CgScalars {
   float rr[3], rr_new[3], pap[3], rr0[3];  // rr0: |b|^2 per column
   unsigned int active;  // bit c: column c not converged
   unsigned int iters;
+  float part_pap[kCgBlocks][3];  // p . Ap
+  float part_rr[kCgBlocks][3];   // r . r
+  float part_b[kCgBlocks][3];    // b . b
 };
 hipError_t launch_csr_spmv3(const CsrDev& M, const float* x, float* y, hipStream_t st);
 // M x = b for the three columns of [n][3] b, x (x: initial guess on entry);

@@ -74,6 +74,36 @@ def test_laplacian_solver(gpu_ctx, oracle_mod):
     S.close()
 
 
+def test_laplacian_solver_large_mesh_deterministic(gpu_ctx, oracle_mod):
+    """A mesh with more rows than the solver's threads (289,002 vertices: the
+    grid-stride loops and all block partials in use): the residual of the
+    answer, checked in float64 on the host, is within the tolerance, and a
+    second solve from the same start gives the same bits."""
+    import torch
+    from tensorray_amd import scenes
+    pos, idx = scenes.geodesic_sphere(170)
+    nv, lam = len(pos), 20.0
+    assert nv > 1024 * 256
+    M = oracle_mod.laplacian_system(idx, nv, lam)
+    S = gpu_ctx.lsolver(idx, nv, lam)
+    rng = np.random.Generator(np.random.PCG64(5))
+    b = rng.standard_normal((nv, 3)).astype(np.float32)
+    db = torch.from_numpy(b).cuda()
+    sols = []
+    for _ in range(2):
+        sol = torch.zeros_like(db)
+        S.solve(db.data_ptr(), sol.data_ptr(), max_iter=1000, tol=1e-5)
+        torch.cuda.synchronize()
+        iters, rel = S.status()
+        assert 0 < iters < 1000 and (rel <= 1e-5).all(), (iters, rel)
+        sols.append(sol.cpu().numpy())
+    assert np.array_equal(sols[0], sols[1])
+    res = M @ sols[0].astype(np.float64) - b
+    rel_true = np.linalg.norm(res, axis=0) / np.linalg.norm(b.astype(np.float64), axis=0)
+    assert (rel_true <= 1e-4).all(), rel_true
+    S.close()
+
+
 def test_pytensorray_largesteps(gpu_ctx, oracle_mod):
     """pyTensorRay's LargeSteps keeps the reference's flat [3 * n] layout:
     one step = M^-1 on the gradient, uniform Adam on u = M x, x = M^-1 u."""
