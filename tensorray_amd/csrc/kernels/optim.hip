@@ -59,9 +59,17 @@ __global__ void k_adam_uniform_moments(const float* g, float* m, float* v, uint6
     v[i] = vi;
     local = fmaxf(local, vi);
   }
-  // v >= 0, so the order of the float bit patterns is the order of the values
+  // v >= 0, so the order of the float bit patterns is the order of the
+  // values; one atomic per block (atomics on one word from every wave of a
+  // large grid serialise)
+  __shared__ float wmax[4];
   for (int o = 32; o > 0; o >>= 1) local = fmaxf(local, __shfl_xor(local, o));
-  if ((threadIdx.x & 63u) == 0u) atomicMax(vmax_bits, __float_as_uint(local));
+  if ((threadIdx.x & 63u) == 0u) wmax[threadIdx.x >> 6] = local;
+  __syncthreads();
+  if (threadIdx.x == 0u) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) local = fmaxf(local, wmax[w]);
+    atomicMax(vmax_bits, __float_as_uint(local));
+  }
 }
 
 __global__ void k_adam_uniform_apply(float* x, const float* m, uint64_t n, float lr, float bc1, float bc2,
@@ -84,7 +92,8 @@ hipError_t launch_adam_uniform(float* x, const float* g, float* m, float* v, uin
   if (n == 0) return hipGetLastError();
   const uint64_t blocks = (n + 255) / 256;
   const dim3 grid((unsigned)(blocks < 8192 ? blocks : 8192));
-  hipLaunchKernelGGL(k_adam_uniform_moments, grid, dim3(256), 0, st, g, m, v, n, b1, b2, vmax_bits);
+  const dim3 grid_m((unsigned)(blocks < 1024 ? blocks : 1024));  // 256 threads = 4 waves per block
+  hipLaunchKernelGGL(k_adam_uniform_moments, grid_m, dim3(256), 0, st, g, m, v, n, b1, b2, vmax_bits);
   hipLaunchKernelGGL(k_adam_uniform_apply, grid, dim3(256), 0, st, x, m, n, lr, bc1, bc2, vmax_bits);
   return hipGetLastError();
 }
